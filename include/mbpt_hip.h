@@ -1,0 +1,134 @@
+/*
+ * mbpt_hip.h -- C ABI of libmbpt_hip.so, the gfx950 (MI355X) kernels of the
+ * ISDF BSE sigma-vector and the space-time polarizability chi0(i tau).
+ *
+ * Conventions
+ *   - Matrices are row-major, FP64 at the boundary.  The library converts to
+ *     its working precision (MBPT_FP64, MBPT_FP32 or MBPT_BF16X3) into buffers
+ *     a handle owns.
+ *   - Pointers named *_dev are device pointers (HBM), *_host host pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL: the null stream).
+ *     Every call is stream-ordered and asynchronous unless stated.
+ *   - Pair layout z[k][i][a]: occupied index slow, virtual fast.
+ *   - Every function returns MBPT_OK or an error code; mbpt_last_error()
+ *     gives the message of the calling thread's last failure.
+ *   - A handle is not re-entrant: one handle per stream.
+ */
+#ifndef MBPT_HIP_H
+#define MBPT_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* precision of the bulk GEMMs */
+#define MBPT_FP64 0   /* FP64 MFMA (v_mfma_f64_16x16x4f64)                     */
+#define MBPT_FP32 1   /* FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulation */
+#define MBPT_BF16X3 2 /* split BF16 hi+lo, 3 BF16 MFMAs per product, FP32 acc   */
+
+/* status codes */
+#define MBPT_OK 0
+#define MBPT_ERR_VALUE 1       /* bad shape / argument: the reference raises ValueError */
+#define MBPT_ERR_HIP 2         /* a HIP runtime call failed                             */
+#define MBPT_ERR_UNSUPPORTED 3 /* a valid request this build does not implement         */
+
+const char *mbpt_last_error(void);
+int mbpt_abi_version(void);
+
+/* ------------------------------------------------------------------------
+ * Sigma-vector of the Tamm-Dancoff ISDF BSE/Casida operator
+ *
+ *   A z = d o z + kappa X_o^T (u o X_v) - (X_o^T (W o (X_o z X_v^T))) X_v,
+ *   u = V p,  p_mu = sum_j X_o[mu, j] (z X_v^T)[j, mu],  d_ia = eps_a - eps_i
+ *
+ * on the interpolation grid (W = D W_aux D^T, V = D D^T of the reference's
+ * factors).  Replaces the closure `apply_AB` returned by
+ * `isdf_block_action` (src/SingleReference/LinearResponse/block_actions.py:
+ * 365, the serial tda=True branch :563-661).
+ * ------------------------------------------------------------------------ */
+typedef struct mbpt_sigma mbpt_sigma;
+
+/* Build a handle: the action's set-up, block_actions.py:365-543.
+ *   M, nocc, nvir  grid points, occupied and virtual orbitals (nocc <= 128)
+ *   precision      MBPT_FP64 | MBPT_FP32 | MBPT_BF16X3
+ *   kappa          Hartree prefactor, KAPPA[spin] (2 singlet, 0 triplet)
+ *   X_dev          (M, nocc + nvir) collocation, occupied columns first
+ *   W_dev          (M, M) screened interaction on the grid, symmetric
+ *   V_dev          (M, M) bare Coulomb on the grid (unread when kappa == 0)
+ *   eps_host       (nocc + nvir) orbital energies
+ * The inputs are copied; the caller may free them after the call returns
+ * and the stream has reached it. */
+int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir,
+                      int precision, double kappa, const double *X_dev,
+                      const double *W_dev, const double *V_dev,
+                      const double *eps_host, void *stream);
+
+/* Az = A z for k trial vectors, z_dev and Az_dev (k, nocc, nvir) FP64,
+ * distinct buffers: `apply_AB(z)[0]`, block_actions.py:563-661. */
+int mbpt_sigma_apply(mbpt_sigma *h, int k, const double *z_dev,
+                     double *Az_dev, void *stream);
+
+/* out = d - (ii|W|aa), (nocc, nvir) FP64: `screened_diagonal()`,
+ * block_actions.py:663-689 (grid gauge: (X_o o X_o)^T W (X_v o X_v)). */
+int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream);
+
+/* Device bytes the handle holds (`apply_AB.held_bytes`, block_actions.py:705). */
+size_t mbpt_sigma_held_bytes(const mbpt_sigma *h);
+
+/* Per-stage device timing, the counterpart of the action's `piece_clock`
+ * (block_actions.py:56-70).  enable != 0 starts (and zeroes) accumulation:
+ * every later apply records HIP events around its four launches on its
+ * stream and waits for the last one.  ms_out[4] receives the accumulated
+ * milliseconds of head, hartree, exchange and tail; launches the number of
+ * applies timed; exchange_flops the algorithmic flops of the exchange
+ * launches, sum of k (4 nocc M^2 + M^2). */
+int mbpt_sigma_profile(mbpt_sigma *h, int enable);
+int mbpt_sigma_profile_read(const mbpt_sigma *h, double *ms_out, long *launches,
+                            double *exchange_flops);
+
+int mbpt_sigma_destroy(mbpt_sigma *h);
+
+/* ------------------------------------------------------------------------
+ * Space-time polarizability on the grid
+ *
+ *   out = [accumulate ? out : 0] + sum_t weight[t] * (-2) Go(t) o Gv(t),
+ *   Go(t) = X_o e^{(eps_o - mu) tau_t} X_o^T,  Gv(t) = X_v e^{-(eps_v - mu) tau_t} X_v^T
+ *
+ * weight = e_t gives proj(tau_t) (without the D projection, which is the
+ * identity in the grid gauge); weight = cosft_wt[k, :] gives chi0(i w_k).
+ * Replaces `polarizability_projected_tau` (src/SingleReference/
+ * LinearResponse/space_time.py:478) and the tau sums of
+ * `chi0_tau_terms` / `static_screening_matrix` (:634, :1003).
+ * ------------------------------------------------------------------------ */
+typedef struct mbpt_chi0 mbpt_chi0;
+
+#define MBPT_CHI0_MAX_TAU 64
+
+/* X_dev (M, nocc + nvir) device FP64, eps_host (nocc + nvir), mu the shift
+ * of `split_branches` (space_time.py:448) -- any mu inside the gap. */
+int mbpt_chi0_create(mbpt_chi0 **out, int M, int nocc, int nvir,
+                     int precision, const double *X_dev,
+                     const double *eps_host, double mu, void *stream);
+
+/* ntau <= MBPT_CHI0_MAX_TAU time points tau_host and weights weight_host;
+ * out_dev (M, M) FP64, symmetric. */
+int mbpt_chi0_build(mbpt_chi0 *h, int ntau, const double *tau_host,
+                    const double *weight_host, double *out_dev, int accumulate,
+                    void *stream);
+
+/* Device timing of the fused chi0 launches (HIP events on the build's
+ * stream, waited for): accumulated milliseconds, launches, and their
+ * algorithmic flops, sum over time points of M(M+1)(nocc + nvir + 1)
+ * for the upper triangle (with diagonal) each launch computes. */
+int mbpt_chi0_profile(mbpt_chi0 *h, int enable);
+int mbpt_chi0_profile_read(const mbpt_chi0 *h, double *ms, long *launches,
+                           double *flops);
+
+int mbpt_chi0_destroy(mbpt_chi0 *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MBPT_HIP_H */

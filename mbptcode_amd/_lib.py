@@ -1,0 +1,112 @@
+"""ctypes binding of libmbpt_hip.so, the package's gfx950 kernels.
+
+The library is loaded after ``import torch`` so that it binds to the HIP
+runtime torch already loaded (both carry the soname libamdhip64.so.7): device
+pointers and streams then come from one runtime.  There is no fallback: if
+the library is missing, ``lib()`` raises.
+"""
+import ctypes
+import os
+
+import torch  # noqa: F401  (loads the HIP runtime the library binds to)
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                        'libmbpt_hip.so')
+
+FP64, FP32, BF16X3 = 0, 1, 2
+PRECISIONS = {'fp64': FP64, 'fp32': FP32, 'bf16x3': BF16X3}
+OK, ERR_VALUE, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
+CHI0_MAX_TAU = 64
+
+#: Every symbol include/mbpt_hip.h declares.
+EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
+           'mbpt_sigma_apply', 'mbpt_sigma_screened_diag',
+           'mbpt_sigma_held_bytes', 'mbpt_sigma_profile',
+           'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
+           'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
+           'mbpt_chi0_destroy')
+
+_lib = None
+
+_vp, _i, _d, _dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)
+_SIGNATURES = {
+    'mbpt_last_error': (ctypes.c_char_p, []),
+    'mbpt_abi_version': (_i, []),
+    'mbpt_sigma_create': (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _vp, _vp,
+                               _vp, _dp, _vp]),
+    'mbpt_sigma_apply': (_i, [_vp, _i, _vp, _vp, _vp]),
+    'mbpt_sigma_screened_diag': (_i, [_vp, _vp, _vp]),
+    'mbpt_sigma_held_bytes': (ctypes.c_size_t, [_vp]),
+    'mbpt_sigma_profile': (_i, [_vp, _i]),
+    'mbpt_sigma_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
+    'mbpt_sigma_destroy': (_i, [_vp]),
+    'mbpt_chi0_create': (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _vp, _dp, _d,
+                              _vp]),
+    'mbpt_chi0_build': (_i, [_vp, _i, _dp, _dp, _vp, _i, _vp]),
+    'mbpt_chi0_profile': (_i, [_vp, _i]),
+    'mbpt_chi0_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
+    'mbpt_chi0_destroy': (_i, [_vp]),
+}
+
+
+class KernelError(RuntimeError):
+    """A HIP runtime failure inside libmbpt_hip."""
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f'{LIB_PATH} is missing: build it with '
+                '`python -c "import __graft_entry__ as g; g.build()"` '
+                '(hipcc --offload-arch=gfx950). There is no CPU fallback.')
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc, what):
+    """Map a status code to the reference's exception classes."""
+    if rc == OK:
+        return
+    msg = f'{what}: {lib().mbpt_last_error().decode(errors="replace")}'
+    if rc == ERR_VALUE:
+        raise ValueError(msg)
+    if rc == ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise KernelError(msg)
+
+
+def precision_code(precision):
+    if isinstance(precision, int):
+        return precision
+    try:
+        return PRECISIONS[precision.lower()]
+    except KeyError:
+        raise ValueError(f'precision {precision!r}: one of {sorted(PRECISIONS)}') from None
+
+
+def stream_ptr(stream=None):
+    s = torch.cuda.current_stream() if stream is None else stream
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def dptr(t):
+    """Device pointer of a contiguous CUDA tensor."""
+    if not t.is_cuda:
+        raise ValueError('expected a device tensor')
+    if not t.is_contiguous():
+        raise ValueError('expected a contiguous tensor')
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def host_doubles(a):
+    import numpy as np
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)

@@ -1,0 +1,44 @@
+"""Named numbers of the ISDF BSE sigma-vector / space-time chi0 path.
+
+Each value is the one the reference reads (src/Base/numerics/constants.py, line
+given per entry), so a caller moving between the two sees the same defaults.
+"""
+
+#: 1 GB as the tile budgets count it (constants.py:17).
+BYTES_PER_GB = 1e9
+
+#: Hartree-term prefactor per spin channel: singlet 2, triplet 0 (constants.py:287).
+KAPPA = {'singlet': 2.0, 'triplet': 0.0}
+
+#: Grid-row tile budget of the CPU kernels, in GB (constants.py:292). The GPU
+#: kernels tile by their own register/LDS shapes; the argument is accepted
+#: for interface parity and checked, nothing more.
+ISDF_TILE_GB = 4.0
+
+#: Gauss-Legendre frequencies of the dRPA energy quadrature (constants.py:469).
+RPA_ENERGY_NFREQ = 40
+
+#: Truncated-SVD cut of the minimax transform fit (constants.py:1148).
+TRANSFORM_FIT_RCOND = 1e-100
+
+#: Laplace-test target the automatic tau-count resolver aims for (constants.py:1844).
+DEFAULT_TAU_TARGET = 1e-10
+
+# Davidson (constants.py:1170-2193)
+DAVIDSON_FLOOR_EPS_MULTIPLE = 1e3       # :1170 residual floor, eps multiples
+DAVIDSON_TOL_RESIDUAL = 1e-6            # :1194
+DAVIDSON_MAX_CYCLE = 200                # :1195
+DAVIDSON_LINDEP_FLOOR = 1e-22           # :1196
+DAVIDSON_LINDEP_CEILING = 1e-14         # :1197
+DAVIDSON_NULL_DIRECTION = 1e-8          # :1198
+DAVIDSON_PRECONDITIONER = 'screened'    # :1380
+DAVIDSON_PRECONDITIONERS = ('bare', 'screened', 'learned')  # :1381 + the MLP
+DAVIDSON_EXTRA_ROOTS_FLOOR = 2          # :1648
+DAVIDSON_EXTRA_ROOTS_DIVISOR = 2        # :1650
+DAVIDSON_GUESS_FACTOR = 2               # :1886
+DAVIDSON_GUESS_DEG_TOL = 1e-3           # :2190
+DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
+
+#: Hartree -> meV and -> eV.
+HARTREE_TO_MEV = 27211.386245988
+HARTREE_TO_EV = 27.211386245988

@@ -1,0 +1,634 @@
+// Sigma-vector of the Tamm-Dancoff ISDF BSE operator on gfx950.
+//
+// One application to k trial vectors is four stream-ordered launches:
+//
+//   head      zXv[b] = z[b] X_v^T                   (k n_o x M), K = n_v, MFMA
+//   hartree   p[b]_mu = sum_j X_o[mu,j] zXv[b][j,mu];  u = V p          (k x M)
+//   exchange  T[b] = X_o^T (W o (X_o zXv[b]))          (n_o x M), fused:
+//             the P tile leaves one MFMA in registers, is multiplied by W in
+//             registers and is the B operand of the next MFMA -- P and W o P
+//             never touch memory; split over grid rows for parallelism
+//   tail      Az[b] = d o z[b] - (sum_s T_s[b] - kappa X_o^T o u[b]) X_v
+//
+// The reference's serial loop (block_actions.py:563-661) runs these per vector
+// and per grid-row tile; here all k vectors go in every launch, so the W tile
+// a workgroup reads is shared by the k workgroups of one XCD through its L2.
+#include "common.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace mbpt {
+
+// ---- conversions -------------------------------------------------------------
+template <typename T>
+__global__ void k_split_X(int M, int no, int nv, const double *__restrict__ X,
+                          T *__restrict__ Xo, T *__restrict__ Xv,
+                          T *__restrict__ XoT) {
+  const long n = (long)M * (no + nv);
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int mu = (int)(e / (no + nv)), p = (int)(e % (no + nv));
+    const T x = (T)X[e];
+    if (p < no) {
+      Xo[(long)mu * no + p] = x;
+      XoT[(long)p * M + mu] = x;
+    } else {
+      Xv[(long)mu * nv + (p - no)] = x;
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_convert(long n, const double *__restrict__ src, T *__restrict__ dst) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x)
+    dst[e] = (T)src[e];
+}
+
+// ---- loaders -------------------------------------------------------------------
+template <typename T> struct LdZ {  // A(m, k) = z[m][k], FP64 in
+  const double *z; int rows, K;
+  __device__ T operator()(int m, int k) const {
+    return (m < rows && k < K) ? (T)z[(long)m * K + k] : T(0);
+  }
+};
+template <typename T> struct LdBnk {  // B(k, n) = P[n][k]
+  const T *p; int ld, K, cols;
+  __device__ T operator()(int k, int n) const {
+    return (k < K && n < cols) ? p[(long)n * ld + k] : T(0);
+  }
+};
+template <typename T> struct LdBkn {  // B(k, n) = P[k][n]
+  const T *p; int ld, K, cols;
+  __device__ T operator()(int k, int n) const {
+    return (k < K && n < cols) ? p[(long)k * ld + n] : T(0);
+  }
+};
+template <typename T> struct LdArm {  // A(m, k) = P[m][k]
+  const T *p; int ld, rows, K;
+  __device__ T operator()(int m, int k) const {
+    return (m < rows && k < K) ? p[(long)m * ld + k] : T(0);
+  }
+};
+template <typename T> struct LdBsq {  // B(k, n) = P[k][n]^2
+  const T *p; int ld, K, cols;
+  __device__ T operator()(int k, int n) const {
+    if (k >= K || n >= cols) return T(0);
+    const T x = p[(long)k * ld + n];
+    return x * x;
+  }
+};
+template <typename T> struct LdAsq {  // A(m, k) = P[m][k]^2
+  const T *p; int ld, rows, K;
+  __device__ T operator()(int m, int k) const {
+    if (m >= rows || k >= K) return T(0);
+    const T x = p[(long)m * ld + k];
+    return x * x;
+  }
+};
+// A(m = (b, i), k = mu) = sum_s T_s[b][i][mu] - kappa X_o^T[i][mu] u[b][mu]
+template <typename T> struct LdTail {
+  const T *Tp, *XoT, *u;
+  T kappa;
+  int no, rows, M, nopad, nsplit;
+  long sstride;
+  __device__ T operator()(int m, int kk) const {
+    if (m >= rows || kk >= M) return T(0);
+    const int b = m / no, i = m - b * no;
+    const long off = ((long)b * nopad + i) * M + kk;
+    T s = Tp[off];
+    for (int q = 1; q < nsplit; ++q) s += Tp[off + q * sstride];
+    if (u) s -= kappa * XoT[(long)i * M + kk] * u[(long)b * M + kk];
+    return s;
+  }
+};
+
+// ---- head: zXv = z X_v^T ---------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_head(int rows, int M, int nv,
+                                              const double *__restrict__ z,
+                                              const T *__restrict__ Xv,
+                                              T *__restrict__ zXv, int ntm,
+                                              int ntn, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= ntm * ntn) return;
+  const int tm = lid / ntn, tn = lid % ntn;
+  __shared__ TileSmem<T> sm;
+  typename MF<T>::acc acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) zero(acc[i][j]);
+  gemm64<T, true, true>(LdZ<T>{z, rows, nv}, LdBnk<T>{Xv, nv, nv, M}, tm * kTile,
+                        tn * kTile, nv, acc, sm);
+  for (int fm = 0; fm < 2; ++fm)
+    for (int fn = 0; fn < 2; ++fn) {
+      const int col = tile_col(tn * kTile, fn);
+      for (int r = 0; r < 4; ++r) {
+        const int row = tile_row(tm * kTile, fm, r);
+        if (row < rows && col < M) zXv[(long)row * M + col] = acc[fm][fn][r];
+      }
+    }
+}
+
+// ---- hartree ----------------------------------------------------------------------
+template <typename T>
+__global__ void k_hartree_p(int M, int no, int k, const T *__restrict__ XoT,
+                            const T *__restrict__ zXv, T *__restrict__ p) {
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)k * M) return;
+  const int b = (int)(idx / M), mu = (int)(idx % M);
+  T s = 0;
+  for (int j = 0; j < no; ++j)
+    s += XoT[(long)j * M + mu] * zXv[((long)b * no + j) * M + mu];
+  p[idx] = s;
+}
+
+// u[b][mu] = sum_nu V[mu][nu] p[b][nu]: one wave per grid row, 8 vectors at a
+// time, lanes striding the row (V is read once per 8 vectors, coalesced).
+template <typename T>
+__global__ __launch_bounds__(256) void k_hartree_u(int M, int k, const T *__restrict__ V,
+                                                   const T *__restrict__ p,
+                                                   T *__restrict__ u) {
+  const int lane = threadIdx.x & 63;
+  const int mu = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (mu >= M) return;
+  const T *row = V + (long)mu * M;
+  for (int b0 = 0; b0 < k; b0 += 8) {
+    const int nb = min(8, k - b0);
+    T acc[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = 0;
+    for (int nu = lane; nu < M; nu += 64) {
+      const T v = row[nu];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q < nb) acc[q] += v * p[(long)(b0 + q) * M + nu];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      T s = acc[q];
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+      if (lane == 0 && q < nb) u[(long)(b0 + q) * M + mu] = s;
+    }
+  }
+}
+
+// ---- exchange: T[b] = X_o^T (W o (X_o zXv[b])) ------------------------------------
+// Workgroup = (64 grid columns nu, one vector b, one split s of the grid rows mu).
+// Wave w owns columns nu0 + 16w .. +16.  Per 16 rows of mu:
+//   P (16 mu x 16 nu)   = X_o[mu, :] zXv[b][:, nu]       NOPAD/4 MFMAs, K = n_o
+//   S_r = P_r * W[mu_r, nu]                              in registers
+//   T[:, nu] += X_o[mu, :]^T S                           NOPAD/16 x 4 MFMAs
+// The P result layout (row 4*(l/16)+r of lane l) is used directly as the B
+// operand of step r of the second product, whose k index is that same row,
+// so no shuffle is needed between the two MFMAs.
+template <typename T, int NOPAD>
+__global__ __launch_bounds__(256) void k_exchange(
+    int M, int no, int k, const T *__restrict__ Xo, const T *__restrict__ W,
+    const T *__restrict__ zXv, T *__restrict__ Tpart, int nsplit,
+    int rows_per_split, int nvb, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= nvb * k * nsplit) return;
+  const int b = lid % k, t = lid / k, vb = t % nvb, s = t / nvb;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int col = vb * 64 + wave * 16 + lr;
+  const bool colok = col < M;
+  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, CH = 4, ROWS = CH * 16,
+                LD = NOPAD + 1;
+  __shared__ T xs[ROWS * LD];
+
+  T bz[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) {
+    const int j = 4 * q + lq;
+    bz[q] = (j < no && colok) ? zXv[((long)b * no + j) * M + col] : T(0);
+  }
+  typename MF<T>::acc acc[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) zero(acc[it]);
+
+  const int mu_beg = s * rows_per_split;
+  const int mu_end = min(M, mu_beg + rows_per_split);
+  for (int mg = mu_beg; mg < mu_end; mg += ROWS) {
+    __syncthreads();
+    for (int e = tid; e < ROWS * NOPAD; e += 256) {
+      const int r = e / NOPAD, c = e % NOPAD, mu = mg + r;
+      xs[r * LD + c] = (mu < mu_end && c < no) ? Xo[(long)mu * no + c] : T(0);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+      const int r0 = ch * 16;
+      if (mg + r0 >= mu_end) break;
+      typename MF<T>::acc P;
+      zero(P);
+#pragma unroll
+      for (int q = 0; q < NS; ++q)
+        P = MF<T>::mma(xs[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
+      T S[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int mu = mg + r0 + 4 * lq + r;
+        const T w = (mu < mu_end && colok) ? W[(long)mu * M + col] : T(0);
+        S[r] = P[r] * w;
+      }
+#pragma unroll
+      for (int it = 0; it < NI; ++it)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          acc[it] = MF<T>::mma(xs[(r0 + 4 * lq + r) * LD + 16 * it + lr], S[r],
+                               acc[it]);
+    }
+  }
+  if (colok) {
+#pragma unroll
+    for (int it = 0; it < NI; ++it)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Tpart[(((long)s * k + b) * NOPAD + 16 * it + 4 * lq + r) * M + col] =
+            acc[it][r];
+  }
+}
+
+// ---- tail: Az = d o z - T' X_v ------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail(int rows, int no, int nv, int M,
+                                              LdTail<T> la, const T *__restrict__ Xv,
+                                              const double *__restrict__ d,
+                                              const double *__restrict__ z,
+                                              double *__restrict__ Az, int ntm,
+                                              int ntn, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= ntm * ntn) return;
+  const int tm = lid / ntn, tn = lid % ntn;
+  __shared__ TileSmem<T> sm;
+  typename MF<T>::acc acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) zero(acc[i][j]);
+  gemm64<T, true, false>(la, LdBkn<T>{Xv, nv, M, nv}, tm * kTile, tn * kTile, M,
+                         acc, sm);
+  for (int fm = 0; fm < 2; ++fm)
+    for (int fn = 0; fn < 2; ++fn) {
+      const int a = tile_col(tn * kTile, fn);
+      for (int r = 0; r < 4; ++r) {
+        const int m = tile_row(tm * kTile, fm, r);
+        if (m < rows && a < nv) {
+          const int i = m % no;
+          const long o = (long)m * nv + a;
+          Az[o] = d[(long)i * nv + a] * z[o] - (double)acc[fm][fn][r];
+        }
+      }
+    }
+}
+
+// ---- screened diagonal: d - (X_o o X_o)^T W (X_v o X_v) -------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_wsq(int M, int nv, const T *__restrict__ W,
+                                             const T *__restrict__ Xv,
+                                             T *__restrict__ Q, int ntm, int ntn,
+                                             int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= ntm * ntn) return;
+  const int tm = lid / ntn, tn = lid % ntn;
+  __shared__ TileSmem<T> sm;
+  typename MF<T>::acc acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) zero(acc[i][j]);
+  gemm64<T, true, false>(LdArm<T>{W, M, M, M}, LdBsq<T>{Xv, nv, M, nv},
+                         tm * kTile, tn * kTile, M, acc, sm);
+  for (int fm = 0; fm < 2; ++fm)
+    for (int fn = 0; fn < 2; ++fn) {
+      const int a = tile_col(tn * kTile, fn);
+      for (int r = 0; r < 4; ++r) {
+        const int mu = tile_row(tm * kTile, fm, r);
+        if (mu < M && a < nv) Q[(long)mu * nv + a] = acc[fm][fn][r];
+      }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sdiag(int M, int no, int nv,
+                                               const T *__restrict__ XoT,
+                                               const T *__restrict__ Q,
+                                               const double *__restrict__ d,
+                                               double *__restrict__ out, int ntm,
+                                               int ntn, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= ntm * ntn) return;
+  const int tm = lid / ntn, tn = lid % ntn;
+  __shared__ TileSmem<T> sm;
+  typename MF<T>::acc acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) zero(acc[i][j]);
+  gemm64<T, true, false>(LdAsq<T>{XoT, M, no, M}, LdBkn<T>{Q, nv, M, nv},
+                         tm * kTile, tn * kTile, M, acc, sm);
+  for (int fm = 0; fm < 2; ++fm)
+    for (int fn = 0; fn < 2; ++fn) {
+      const int a = tile_col(tn * kTile, fn);
+      for (int r = 0; r < 4; ++r) {
+        const int i = tile_row(tm * kTile, fm, r);
+        if (i < no && a < nv)
+          out[(long)i * nv + a] = d[(long)i * nv + a] - (double)acc[fm][fn][r];
+      }
+    }
+}
+
+}  // namespace mbpt
+
+// ===================================================================================
+// handle
+// ===================================================================================
+using namespace mbpt;
+
+struct mbpt_sigma {
+  int M, no, nv, nopad, prec;
+  double kappa;
+  size_t esz;              // bytes per working-precision element
+  void *Xo, *Xv, *XoT, *W, *V;
+  double *d;
+  void *zXv, *p, *u, *Tpart;
+  long cap_k, cap_tp;      // vectors / (split x vector) slots the workspace holds
+  size_t held;
+  int prof;                // stage timing on
+  hipEvent_t ev[5];
+  double prof_ms[4], prof_flops;
+  long prof_launches;
+};
+
+namespace {
+
+constexpr int kMaxNocc = 128;
+constexpr int kTargetBlocks = 2048;
+
+int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
+
+void free_ws(mbpt_sigma *h) {
+  (void)hipFree(h->zXv); (void)hipFree(h->p); (void)hipFree(h->u); (void)hipFree(h->Tpart);
+  h->zXv = h->p = h->u = h->Tpart = nullptr;
+  h->cap_k = h->cap_tp = 0;
+}
+
+int nsplit_for(const mbpt_sigma *h, int k) {
+  const int nvb = (h->M + 63) / 64;
+  int ns = (kTargetBlocks + nvb * k - 1) / (nvb * k);
+  ns = std::max(1, std::min(ns, (h->M + 63) / 64));
+  return ns;
+}
+
+int ensure_ws(mbpt_sigma *h, int k, int nsplit) {
+  const long need_tp = (long)k * nsplit;
+  if (k <= h->cap_k && need_tp <= h->cap_tp) return MBPT_OK;
+  const long ck = std::max<long>(h->cap_k, ((long)k + 7) / 8 * 8);
+  const long ctp = std::max<long>(h->cap_tp, std::max<long>(need_tp, ck));
+  free_ws(h);
+  const size_t e = h->esz;
+  MBPT_HIP_TRY(hipMalloc(&h->zXv, e * ck * h->no * h->M));
+  MBPT_HIP_TRY(hipMalloc(&h->p, e * ck * h->M));
+  MBPT_HIP_TRY(hipMalloc(&h->u, e * ck * h->M));
+  MBPT_HIP_TRY(hipMalloc(&h->Tpart, e * ctp * h->nopad * h->M));
+  h->cap_k = ck;
+  h->cap_tp = ctp;
+  return MBPT_OK;
+}
+
+template <typename T> int create_impl(mbpt_sigma *h, const double *X, const double *W,
+                                      const double *V, hipStream_t st) {
+  const long M = h->M;
+  k_split_X<T><<<grid1(M * (h->no + h->nv)), 256, 0, st>>>(
+      h->M, h->no, h->nv, X, (T *)h->Xo, (T *)h->Xv, (T *)h->XoT);
+  int rc = launch_status("k_split_X");
+  if (rc) return rc;
+  k_convert<T><<<grid1(M * M), 256, 0, st>>>(M * M, W, (T *)h->W);
+  if ((rc = launch_status("k_convert(W)"))) return rc;
+  if (h->kappa != 0.0) {
+    k_convert<T><<<grid1(M * M), 256, 0, st>>>(M * M, V, (T *)h->V);
+    if ((rc = launch_status("k_convert(V)"))) return rc;
+  }
+  return MBPT_OK;
+}
+
+template <typename T, int NOPAD>
+void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
+  const int nvb = (h->M + 63) / 64;
+  const int nblk = pad_grid((long)nvb * k * nsplit);
+  k_exchange<T, NOPAD><<<nblk, 256, 0, st>>>(h->M, h->no, k, (const T *)h->Xo,
+                                             (const T *)h->W, (const T *)h->zXv,
+                                             (T *)h->Tpart, nsplit, rps, nvb, nblk);
+}
+
+template <typename T>
+int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st) {
+  const int M = h->M, no = h->no, nv = h->nv;
+  const int nsplit = nsplit_for(h, k);
+  int rc = ensure_ws(h, k, nsplit);
+  if (rc) return rc;
+  // rows per split, a multiple of the 64-row LDS stage
+  const int rps = (((M + nsplit - 1) / nsplit) + 63) / 64 * 64;
+  const int nsplit_eff = (M + rps - 1) / rps;
+  const int rows = k * no;
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[0], st));
+
+  {  // head
+    const int ntm = (rows + 63) / 64, ntn = (M + 63) / 64;
+    const int nblk = pad_grid((long)ntm * ntn);
+    k_head<T><<<nblk, 256, 0, st>>>(rows, M, nv, z, (const T *)h->Xv, (T *)h->zXv,
+                                    ntm, ntn, nblk);
+    if ((rc = launch_status("k_head"))) return rc;
+  }
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[1], st));
+  if (h->kappa != 0.0) {  // hartree
+    k_hartree_p<T><<<grid1((long)k * M), 256, 0, st>>>(M, no, k, (const T *)h->XoT,
+                                                        (const T *)h->zXv, (T *)h->p);
+    if ((rc = launch_status("k_hartree_p"))) return rc;
+    k_hartree_u<T><<<(M + 3) / 4, 256, 0, st>>>(M, k, (const T *)h->V,
+                                                (const T *)h->p, (T *)h->u);
+    if ((rc = launch_status("k_hartree_u"))) return rc;
+  }
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
+  switch (h->nopad) {  // exchange
+    case 16: launch_exchange<T, 16>(h, k, nsplit_eff, rps, st); break;
+    case 32: launch_exchange<T, 32>(h, k, nsplit_eff, rps, st); break;
+    case 48: launch_exchange<T, 48>(h, k, nsplit_eff, rps, st); break;
+    case 64: launch_exchange<T, 64>(h, k, nsplit_eff, rps, st); break;
+    case 80: launch_exchange<T, 80>(h, k, nsplit_eff, rps, st); break;
+    case 96: launch_exchange<T, 96>(h, k, nsplit_eff, rps, st); break;
+    case 112: launch_exchange<T, 112>(h, k, nsplit_eff, rps, st); break;
+    case 128: launch_exchange<T, 128>(h, k, nsplit_eff, rps, st); break;
+    default: set_error("internal: nopad"); return MBPT_ERR_VALUE;
+  }
+  if ((rc = launch_status("k_exchange"))) return rc;
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
+  {  // tail
+    LdTail<T> la{(const T *)h->Tpart, (const T *)h->XoT,
+                 h->kappa != 0.0 ? (const T *)h->u : nullptr, (T)h->kappa, no, rows,
+                 M, h->nopad, nsplit_eff, (long)k * h->nopad * M};
+    const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
+    const int nblk = pad_grid((long)ntm * ntn);
+    k_tail<T><<<nblk, 256, 0, st>>>(rows, no, nv, M, la, (const T *)h->Xv, h->d, z, Az,
+                                    ntm, ntn, nblk);
+    if ((rc = launch_status("k_tail"))) return rc;
+  }
+  if (h->prof) {
+    MBPT_HIP_TRY(hipEventRecord(h->ev[4], st));
+    MBPT_HIP_TRY(hipEventSynchronize(h->ev[4]));
+    for (int q = 0; q < 4; ++q) {
+      float ms = 0.f;
+      MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->ev[q], h->ev[q + 1]));
+      h->prof_ms[q] += ms;
+    }
+    h->prof_flops += (double)k * (4.0 * no * (double)M * M + (double)M * M);
+    h->prof_launches += 1;
+  }
+  return MBPT_OK;
+}
+
+template <typename T> int sdiag_impl(mbpt_sigma *h, double *out, hipStream_t st) {
+  const int M = h->M, no = h->no, nv = h->nv;
+  void *Q = nullptr;
+  MBPT_HIP_TRY(hipMalloc(&Q, sizeof(T) * (size_t)M * nv));
+  int rc;
+  {
+    const int ntm = (M + 63) / 64, ntn = (nv + 63) / 64;
+    const int nblk = pad_grid((long)ntm * ntn);
+    k_wsq<T><<<nblk, 256, 0, st>>>(M, nv, (const T *)h->W, (const T *)h->Xv, (T *)Q,
+                                   ntm, ntn, nblk);
+    rc = launch_status("k_wsq");
+  }
+  if (!rc) {
+    const int ntm = (no + 63) / 64, ntn = (nv + 63) / 64;
+    const int nblk = pad_grid((long)ntm * ntn);
+    k_sdiag<T><<<nblk, 256, 0, st>>>(M, no, nv, (const T *)h->XoT, (const T *)Q, h->d,
+                                     out, ntm, ntn, nblk);
+    rc = launch_status("k_sdiag");
+  }
+  hipError_t e = hipStreamSynchronize(st);
+  (void)hipFree(Q);
+  if (!rc && e != hipSuccess) {
+    set_error(std::string("screened diagonal: ") + hipGetErrorString(e));
+    rc = MBPT_ERR_HIP;
+  }
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision,
+                      double kappa, const double *X_dev, const double *W_dev,
+                      const double *V_dev, const double *eps_host, void *stream) {
+  MBPT_REQUIRE(out != nullptr, "mbpt_sigma_create: out is NULL");
+  *out = nullptr;
+  MBPT_REQUIRE(M > 0 && nocc > 0 && nvir > 0, "mbpt_sigma_create: M, nocc and nvir must be positive");
+  MBPT_REQUIRE(nocc <= kMaxNocc, "mbpt_sigma_create: nocc > 128 is not supported by the exchange kernel");
+  MBPT_REQUIRE(X_dev && W_dev && eps_host, "mbpt_sigma_create: NULL X, W or eps");
+  MBPT_REQUIRE(kappa == 0.0 || V_dev, "mbpt_sigma_create: kappa != 0 needs V");
+  if (precision != MBPT_FP64 && precision != MBPT_FP32) {
+    set_error("mbpt_sigma_create: precision " + std::to_string(precision) +
+              " is not built (MBPT_FP64 or MBPT_FP32)");
+    return MBPT_ERR_UNSUPPORTED;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  mbpt_sigma *h = new mbpt_sigma();
+  h->M = M; h->no = nocc; h->nv = nvir; h->nopad = (nocc + 15) / 16 * 16;
+  h->prec = precision; h->kappa = kappa;
+  h->esz = precision == MBPT_FP64 ? 8 : 4;
+  const size_t e = h->esz;
+  auto fail = [&](int rc) { mbpt_sigma_destroy(h); return rc; };
+#define TRY_ALLOC(ptr, bytes)                                                   \
+  do {                                                                          \
+    hipError_t _e = hipMalloc(&(ptr), (bytes));                                 \
+    if (_e != hipSuccess) {                                                     \
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(_e));            \
+      return fail(MBPT_ERR_HIP);                                                \
+    }                                                                           \
+  } while (0)
+  TRY_ALLOC(h->Xo, e * M * nocc);
+  TRY_ALLOC(h->Xv, e * M * nvir);
+  TRY_ALLOC(h->XoT, e * M * nocc);
+  TRY_ALLOC(h->W, e * M * M);
+  if (kappa != 0.0) TRY_ALLOC(h->V, e * M * M);
+  TRY_ALLOC(h->d, sizeof(double) * nocc * nvir);
+#undef TRY_ALLOC
+  h->held = e * ((size_t)M * (2 * nocc + nvir) + (size_t)M * M * (kappa != 0.0 ? 2 : 1)) +
+            sizeof(double) * nocc * nvir;
+  std::vector<double> dh((size_t)nocc * nvir);
+  for (int i = 0; i < nocc; ++i)
+    for (int a = 0; a < nvir; ++a) dh[(size_t)i * nvir + a] = eps_host[nocc + a] - eps_host[i];
+  hipError_t he = hipMemcpyAsync(h->d, dh.data(), sizeof(double) * dh.size(),
+                                 hipMemcpyHostToDevice, st);
+  if (he == hipSuccess) he = hipStreamSynchronize(st);  // dh leaves scope
+  if (he != hipSuccess) {
+    set_error(std::string("upload of d: ") + hipGetErrorString(he));
+    return fail(MBPT_ERR_HIP);
+  }
+  int rc = precision == MBPT_FP64 ? create_impl<double>(h, X_dev, W_dev, V_dev, st)
+                                  : create_impl<float>(h, X_dev, W_dev, V_dev, st);
+  if (rc) return fail(rc);
+  *out = h;
+  return MBPT_OK;
+}
+
+int mbpt_sigma_apply(mbpt_sigma *h, int k, const double *z_dev, double *Az_dev,
+                     void *stream) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_sigma_apply: NULL handle");
+  MBPT_REQUIRE(k >= 0, "mbpt_sigma_apply: k < 0");
+  if (k == 0) return MBPT_OK;
+  MBPT_REQUIRE(z_dev && Az_dev, "mbpt_sigma_apply: NULL z or Az");
+  MBPT_REQUIRE(z_dev != Az_dev, "mbpt_sigma_apply: z and Az must be distinct buffers");
+  hipStream_t st = (hipStream_t)stream;
+  return h->prec == MBPT_FP64 ? apply_impl<double>(h, k, z_dev, Az_dev, st)
+                              : apply_impl<float>(h, k, z_dev, Az_dev, st);
+}
+
+int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream) {
+  MBPT_REQUIRE(h != nullptr && out_dev != nullptr, "mbpt_sigma_screened_diag: NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  return h->prec == MBPT_FP64 ? sdiag_impl<double>(h, out_dev, st)
+                              : sdiag_impl<float>(h, out_dev, st);
+}
+
+size_t mbpt_sigma_held_bytes(const mbpt_sigma *h) {
+  if (!h) return 0;
+  return h->held + h->esz * ((size_t)h->cap_k * (h->no + 2) * h->M +
+                             (size_t)h->cap_tp * h->nopad * h->M);
+}
+
+int mbpt_sigma_profile(mbpt_sigma *h, int enable) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_sigma_profile: NULL handle");
+  if (enable && !h->prof) {
+    for (int q = 0; q < 5; ++q) MBPT_HIP_TRY(hipEventCreate(&h->ev[q]));
+  }
+  if (!enable && h->prof) {
+    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
+  }
+  h->prof = enable ? 1 : 0;
+  for (int q = 0; q < 4; ++q) h->prof_ms[q] = 0.0;
+  h->prof_flops = 0.0;
+  h->prof_launches = 0;
+  return MBPT_OK;
+}
+
+int mbpt_sigma_profile_read(const mbpt_sigma *h, double *ms_out, long *launches,
+                            double *exchange_flops) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_sigma_profile_read: NULL handle");
+  if (ms_out)
+    for (int q = 0; q < 4; ++q) ms_out[q] = h->prof_ms[q];
+  if (launches) *launches = h->prof_launches;
+  if (exchange_flops) *exchange_flops = h->prof_flops;
+  return MBPT_OK;
+}
+
+int mbpt_sigma_destroy(mbpt_sigma *h) {
+  if (!h) return MBPT_OK;
+  if (h->prof)
+    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
+  free_ws(h);
+  (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
+  (void)hipFree(h->d);
+  delete h;
+  return MBPT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,455 @@
+"""The Tamm-Dancoff BSE Davidson on the GPU.
+
+Drop-ins for the reference's Hermitian Davidson stack:
+
+* `solve_symmetric` (src/Solvers/davidson.py:179) -- lowest eigenpairs of a
+  symmetric matrix-free operator, Jacobi-Davidson preconditioner (:43-77),
+  linear-dependence threshold scaled to the tolerance (:156-176);
+* `run_davidson` / `_tda_davidson_root` (LR/davidson.py:631, :462) -- the
+  TDA loop on an `apply_AB`: unit-vector guess (`guess_indices`,
+  davidson_shared.py:19-54), tracked roots above the requested ones
+  (`tracked_roots` :146-154), 'bare' / 'screened' preconditioner diagonal
+  (:125-143), unconfirmed-root bound (:538-546);
+* `solve_casida_davidson` (LR/davidson.py:252) with `tda=True` and
+  `check_residual_floor` (:428-450).
+
+Everything the iteration does is FP64 and stays on the device: the trial
+space V and its image AV, the Rayleigh-Ritz matrix, residuals, corrections
+and their orthogonalisation.  Only the m x m Ritz matrix and the residual
+norms cross to the host once per cycle (the m x m eigensolve is done in
+LAPACK there).  The operator may apply A in FP64, FP32 or split BF16; with
+`refine=True` the roots converged on a lower-precision operator are
+re-checked with an FP64 operator (A x - theta x recomputed in FP64) and
+iterated on it until the FP64 residual meets the tolerance.
+"""
+import time
+import warnings
+
+import numpy as np
+import torch
+
+from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
+                        DAVIDSON_GUESS_DEG_TOL, DAVIDSON_GUESS_FACTOR,
+                        DAVIDSON_LINDEP_CEILING, DAVIDSON_LINDEP_FLOOR,
+                        DAVIDSON_MAX_CYCLE, DAVIDSON_PRECONDITIONER,
+                        DAVIDSON_PRECONDITIONERS, DAVIDSON_TOL_RESIDUAL,
+                        DAVIDSON_EXTRA_ROOTS_DIVISOR, DAVIDSON_EXTRA_ROOTS_FLOOR)
+
+
+class DavidsonNotConverged(RuntimeError):
+    """Roots left above the residual tolerance with refuse_unconverged set
+    (src/Base/step_failed.py:109)."""
+
+
+# ---------------------------------------------------------------------------
+# shared pieces (LR/davidson_shared.py)
+# ---------------------------------------------------------------------------
+
+def guess_indices(diag_d, nroots, guess_factor=DAVIDSON_GUESS_FACTOR,
+                  deg_tol=DAVIDSON_GUESS_DEG_TOL, counters=None):
+    """Pairs carrying the unit-vector guess: max(guess_factor k, k + 4) lowest,
+    extended through a degenerate cluster up to max(4k, 20)
+    (davidson_shared.py:19-54)."""
+    d = np.asarray(diag_d, float).ravel()
+    order = np.argsort(d)
+    n = min(max(guess_factor * nroots, nroots + 4), d.size)
+    cut = d[order[n - 1]] + deg_tol
+    n_max = min(max(4 * nroots, 20, n), d.size)
+    while n < n_max and d[order[n]] <= cut:
+        n += 1
+    if n == n_max < d.size and d[order[n]] <= cut:
+        if counters is not None:
+            counters['guess_truncated'] = True
+        warnings.warn(f'Davidson guess truncated at {n} vectors inside a cluster '
+                      f'of pair energies degenerate to {deg_tol:g} Ha',
+                      RuntimeWarning, stacklevel=3)
+    return order[:n]
+
+
+def tracked_roots(nroots, n_pair, extra=None):
+    """nroots + max(2, nroots // 2), at most the pair space (:146-154)."""
+    if extra is None:
+        extra = max(DAVIDSON_EXTRA_ROOTS_FLOOR, nroots // DAVIDSON_EXTRA_ROOTS_DIVISOR)
+    return min(nroots + int(extra), n_pair)
+
+
+def lindep_for(tol_residual):
+    """Squared-norm linear-dependence threshold: clamp(1e-2 tol^2, 1e-22, 1e-14)
+    (Solvers/davidson.py:156-176)."""
+    return float(max(DAVIDSON_LINDEP_FLOOR,
+                     min(DAVIDSON_LINDEP_CEILING, 1e-2 * tol_residual ** 2)))
+
+
+def residual_floor(diag_d):
+    """1e3 eps_FP64 max|d| (LR/davidson.py:453-457)."""
+    return DAVIDSON_FLOOR_EPS_MULTIPLE * np.finfo(float).eps * float(np.abs(diag_d).max())
+
+
+def check_residual_floor(conv_tol, diag_d):
+    """ValueError when conv_tol is below the FP64 residual floor (:428-450)."""
+    floor = residual_floor(diag_d)
+    if not conv_tol >= floor:
+        raise ValueError(f'Davidson conv_tol={conv_tol:g} is below the residual '
+                         f'this pair space can reach in double precision, '
+                         f'{floor:.1e} Ha; ask for at least that.')
+    return floor
+
+
+def check_preconditioner(preconditioner):
+    if preconditioner not in DAVIDSON_PRECONDITIONERS:
+        raise ValueError(f'preconditioner={preconditioner!r}; choose one of '
+                         f"{', '.join(map(repr, DAVIDSON_PRECONDITIONERS))}.")
+
+
+def jacobi_davidson_correction(R, U, diag, theta, floor=DAVIDSON_DENOMINATOR_FLOOR):
+    """Columns t = K^-1 r - alpha K^-1 u, alpha = u.K^-1 r / u.K^-1 u,
+    K = diag - theta floored at `floor` in magnitude (Solvers/davidson.py:43-77),
+    for a block of residuals R and Ritz vectors U (n, m) on the device."""
+    den = diag[:, None] - theta[None, :]
+    den = torch.where(den.abs() < floor, torch.full_like(den, floor), den)
+    Kr = R / den
+    Ku = U / den
+    uKu = (U * Ku).sum(0)
+    uKr = (U * Kr).sum(0)
+    alpha = torch.where(uKu.abs() > 1e-300, uKr / uKu, torch.zeros_like(uKu))
+    return Kr - alpha[None, :] * Ku
+
+
+# ---------------------------------------------------------------------------
+# the iteration
+# ---------------------------------------------------------------------------
+
+def _orthonormal_new(T, V, drop):
+    """T projected off V (twice) and orthonormalised among itself through the
+    eigendecomposition of its Gram matrix; directions whose norm after the
+    projection is below `drop` (relative to their norm before it) are
+    discarded.  FP64 throughout."""
+    T = T / T.norm(dim=0).clamp_min(1e-300)
+    if V is not None and V.shape[1]:
+        T = T - V @ (V.T @ T)
+        T = T - V @ (V.T @ T)
+    nrm = T.norm(dim=0)
+    keep = nrm > drop
+    if not bool(keep.any()):
+        return T[:, :0]
+    T = T[:, keep] / nrm[keep]
+    G = T.T @ T
+    s, U = torch.linalg.eigh(G)
+    good = s > 1e-10 * s.max()
+    T = T @ (U[:, good] / s[good].sqrt())
+    if V is not None and V.shape[1]:
+        T = T - V @ (V.T @ T)
+    T = T / T.norm(dim=0)
+    return T
+
+
+class DavidsonResult:
+    __slots__ = ('e', 'X', 'conv', 'rnorm', 'cycles', 'vectors', 'seconds',
+                 'trace', 'stalled')
+
+    def __init__(self, **kw):
+        for k in self.__slots__:
+            setattr(self, k, kw.get(k))
+
+
+def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
+             max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
+             lindep=None, stall_cycles=None, on_stall=None):
+    """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
+    operator, on device tensors.
+
+    matmat(V (n, m)) -> A V (n, m), FP64 device tensors.  diag (n,) device.
+    x0 (n, m0) guess columns.  precond(R, U, theta, cycle) -> corrections; the
+    default is the Jacobi-Davidson one on `diag`.  When the space would exceed
+    `max_space` it collapses onto the current Ritz vectors.  `on_stall` is
+    called when the largest unconverged residual has not fallen by 10x over
+    `stall_cycles` cycles (the learned preconditioner's fallback hook).
+    """
+    n = diag.numel()
+    nroots = max(1, min(int(nroots), n))
+    if max_space is None:
+        max_space = max(10 * nroots + 30, 60)
+    max_space = int(min(max_space, n))
+    lindep = lindep_for(tol) if lindep is None else lindep
+    drop = float(np.sqrt(lindep))
+    if precond is None:
+        def precond(R, U, theta, cycle):
+            return jacobi_davidson_correction(R, U, diag, theta)
+    t0 = time.time()
+    V = _orthonormal_new(x0, None, 1e-12)
+    AV = matmat(V)
+    vectors = V.shape[1]
+    trace = []
+    conv = np.zeros(nroots, bool)
+    w = rn = None
+    X = AX = None
+    best_hist = []
+    stalled = False
+    cycles = 0
+    for cycle in range(max_cycle):
+        cycles = cycle + 1
+        H = V.T @ AV
+        H = 0.5 * (H + H.T)
+        Hh = H.cpu().numpy()
+        evals, evecs = np.linalg.eigh(Hh)
+        k = min(nroots, len(evals))
+        y = torch.as_tensor(evecs[:, :k], device=V.device)
+        w = evals[:k]
+        X = V @ y
+        AX = AV @ y
+        wt = torch.as_tensor(w, device=V.device)
+        R = AX - X * wt[None, :]
+        rn = R.norm(dim=0).cpu().numpy()
+        conv = rn <= tol
+        trace.append({'cycle': cycle, 'space': int(V.shape[1]),
+                      'max_residual': float(rn.max()), 'converged': int(conv.sum())})
+        if conv.all():
+            break
+        worst = float(rn[~conv].max())
+        best_hist.append(worst)
+        if stall_cycles and len(best_hist) > stall_cycles:
+            if worst > 0.1 * best_hist[-1 - stall_cycles] and on_stall is not None:
+                stalled = True
+                new_pre = on_stall(cycle)
+                if new_pre is not None:
+                    precond = new_pre
+                best_hist = []
+        idx = torch.as_tensor(np.flatnonzero(~conv), device=V.device)
+        T = precond(R[:, idx], X[:, idx], wt[idx], cycle)
+        if V.shape[1] + T.shape[1] > max_space:
+            V, AV = X, AX
+        T = _orthonormal_new(T, V, drop)
+        if T.shape[1] == 0:
+            break
+        AT = matmat(T)
+        vectors += T.shape[1]
+        V = torch.cat([V, T], dim=1)
+        AV = torch.cat([AV, AT], dim=1)
+    return DavidsonResult(e=np.asarray(w), X=X, conv=conv, rnorm=rn, cycles=cycles,
+                          vectors=vectors, seconds=time.time() - t0, trace=trace,
+                          stalled=stalled)
+
+
+def solve_symmetric(matvec, diag, nroots=1, x0=None, pick=None,
+                    tol_residual=DAVIDSON_TOL_RESIDUAL, tol_eig=None,
+                    max_cycle=DAVIDSON_MAX_CYCLE, max_subspace=None, precond=None,
+                    lindep=None, warn_unconverged=True, label='Davidson',
+                    **unused):
+    """(e, X, conv) of the `nroots` lowest eigenpairs (Solvers/davidson.py:179).
+
+    matvec: an object with `.matmat` (n, m) -> (n, m) on device FP64 tensors,
+    or a callable on columns.  diag (n,).  x0 None seeds the 2 nroots + 4
+    lowest-diagonal unit vectors.  `pick` (root following) is not built.
+    Returns NumPy e, conv and a device X.
+    """
+    if pick is not None:
+        raise NotImplementedError('root following (pick) is not built')
+    block = getattr(matvec, 'matmat', matvec)
+    dev = torch.device('cuda', torch.cuda.current_device())
+    diag_t = torch.as_tensor(np.asarray(diag, float).ravel(), device=dev)
+    n = diag_t.numel()
+    nroots = max(1, min(int(nroots), n))
+    if x0 is None:
+        cols = np.argsort(diag_t.cpu().numpy())[:min(n, 2 * nroots + 4)]
+        x0 = torch.zeros((n, len(cols)), dtype=torch.float64, device=dev)
+        x0[torch.as_tensor(cols, device=dev), torch.arange(len(cols), device=dev)] = 1.0
+    elif not isinstance(x0, torch.Tensor):
+        x0 = torch.as_tensor(np.asarray(x0, float).reshape(n, -1), device=dev)
+    pre = None
+    if precond is not None:
+        def pre(R, U, theta, cycle):
+            return precond(R, U, theta)
+    res = davidson(block, diag_t, nroots, x0, tol=tol_residual, max_cycle=max_cycle,
+                   max_space=max_subspace, precond=pre, lindep=lindep)
+    if warn_unconverged and not res.conv.all():
+        warnings.warn(f'{label} left {int((~res.conv).sum())} of {len(res.conv)} '
+                      f'roots unconverged after {max_cycle} cycles',
+                      RuntimeWarning, stacklevel=2)
+    return res.e, res.X, res.conv
+
+
+# ---------------------------------------------------------------------------
+# the TDA driver (LR/davidson.py)
+# ---------------------------------------------------------------------------
+
+def _operator_of(apply_AB):
+    op = getattr(apply_AB, 'operator', None)
+    if op is None:
+        raise ValueError('run_davidson on the GPU needs a device block action '
+                         '(mbptcode_amd.block_actions.isdf_block_action)')
+    return op
+
+
+def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
+                 guess_factor=DAVIDSON_GUESS_FACTOR, stats=None, comm=None,
+                 timings=None, refuse_unconverged=False, max_memory=None,
+                 preconditioner=DAVIDSON_PRECONDITIONER, read_roots=None,
+                 read_tol=None, guess=None, tda=False, tracked=None,
+                 extra_roots=None, max_space=None, refine_apply=None,
+                 learned=None, stall_cycles=3):
+    """(omega, X, Y): the TDA roots from a GPU block action
+    (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
+
+    preconditioner: 'bare' (d), 'screened' (d - (ii|W|aa)) or 'learned' (the
+    MLP correction of `learned`, a `LearnedPreconditioner`, with the screened
+    diagonal as its fallback whenever the residual stalls for `stall_cycles`
+    cycles).  refine_apply: an FP64 block action of the same operator; the
+    roots are then re-checked and polished in FP64 (the FP64 final residual
+    check).  Returns NumPy omega (nroots,), X (n_pair, nroots), Y = 0.
+    """
+    if not tda:
+        raise NotImplementedError('the full (non-TDA) Casida Davidson is not built')
+    if x_sym is not None:
+        raise ValueError('the Tamm-Dancoff Davidson takes no pair symmetry')
+    if comm is not None and comm.Get_size() > 1:
+        raise NotImplementedError('one GPU: no pair-row split')
+    check_preconditioner(preconditioner)
+    op = _operator_of(apply_AB)
+    no, nv = np.shape(diag_d)
+    npair = no * nv
+    dev = op.device
+    t_all = time.time()
+    counters = {}
+    t0 = time.time()
+    if preconditioner == 'bare':
+        pdiag = torch.as_tensor(np.asarray(diag_d, float).ravel(), device=dev)
+    else:
+        pdiag = op.screened_diagonal_device().reshape(-1)
+    t_precond = time.time() - t0
+    order = guess_indices(diag_d, nroots, guess_factor, counters=counters)
+    track = tracked_roots(nroots, npair, extra_roots)
+    x0 = torch.zeros((npair, len(order)), dtype=torch.float64, device=dev)
+    x0[torch.as_tensor(order, device=dev), torch.arange(len(order), device=dev)] = 1.0
+    if guess is not None:
+        gx = torch.as_tensor(np.asarray(guess[0], float).reshape(npair, -1), device=dev)
+        x0 = torch.cat([gx, x0], dim=1)
+
+    def matmat_of(o):
+        def mm(V):
+            k = V.shape[1]
+            Z = V.T.contiguous().reshape(k, no, nv)
+            return o.apply(Z).reshape(k, npair).T
+        return mm
+
+    matmat = matmat_of(op)
+    precond = None
+    on_stall = None
+    if preconditioner == 'learned':
+        if learned is None:
+            raise ValueError("preconditioner='learned' needs `learned` "
+                             '(a LearnedPreconditioner)')
+        state = {'fallback': False}
+
+        def precond(R, U, theta, cycle):
+            if state['fallback']:
+                return jacobi_davidson_correction(R, U, pdiag, theta)
+            return learned.correction(R, U, theta, pdiag)
+
+        def on_stall(cycle):
+            state['fallback'] = True
+            counters['learned_fallback_cycle'] = cycle
+            return None
+    res = davidson(matmat, pdiag, track, x0, tol=conv_tol, max_cycle=max_cycle,
+                   max_space=max_space, precond=precond,
+                   stall_cycles=stall_cycles if preconditioner == 'learned' else None,
+                   on_stall=on_stall)
+    cycles, vectors = res.cycles, res.vectors
+    e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
+    refine = {}
+    if refine_apply is not None:
+        op64 = _operator_of(refine_apply)
+        mm64 = matmat_of(op64)
+        AX = mm64(X)
+        theta = (X * AX).sum(0)
+        R = AX - X * theta[None, :]
+        rn64 = R.norm(dim=0).cpu().numpy()
+        refine['rnorm_before'] = rn64.copy()
+        if not (rn64[:nroots] <= conv_tol).all():
+            res2 = davidson(mm64, pdiag, track, X, tol=conv_tol, max_cycle=max_cycle,
+                            max_space=max_space)
+            e, X, conv, rn64 = res2.e, res2.X, res2.conv, res2.rnorm
+            refine['cycles'] = res2.cycles
+            refine['vectors'] = res2.vectors
+            cycles += res2.cycles
+            vectors += res2.vectors
+        else:
+            e = theta.cpu().numpy()
+            conv = rn64 <= conv_tol
+        rn = rn64
+        refine['rnorm_after'] = np.asarray(rn).copy()
+    unconfirmed = np.zeros(nroots, bool)
+    extra = np.flatnonzero(~conv[nroots:]) + nroots
+    if extra.size:
+        lower = np.min(e[extra] - rn[extra])
+        unconfirmed = e[:nroots] > lower
+    seconds = time.time() - t_all
+    if stats is not None:
+        stats.update({'davidson_vind_calls': cycles, 'davidson_block_actions': vectors,
+                      'davidson_dense_completion': False})
+    if timings is not None:
+        timings.update({'davidson_s': seconds, 'davidson_iterations': cycles,
+                        'davidson_vectors': vectors, 'davidson_precond_s': t_precond,
+                        'davidson_converged': conv[:nroots].tolist(),
+                        'davidson_residuals': np.asarray(rn[:nroots]).tolist(),
+                        'davidson_unconfirmed': unconfirmed.tolist(),
+                        'davidson_trace': res.trace,
+                        'davidson_refine': {k: (v.tolist() if hasattr(v, 'tolist') else v)
+                                            for k, v in refine.items()},
+                        'davidson_flags': counters})
+    if tracked is not None:
+        tracked.update(omega=np.asarray(e).copy(), X=X.cpu().numpy(),
+                       Y=np.zeros((npair, len(e))))
+    bad = ~conv[:nroots]
+    if bad.any():
+        msg = (f'Tamm-Dancoff Davidson left {int(bad.sum())} of {nroots} roots '
+               f'above |r| <= {conv_tol:g} after {cycles} cycles')
+        if refuse_unconverged:
+            raise DavidsonNotConverged(msg)
+        warnings.warn(msg, RuntimeWarning, stacklevel=2)
+    Xn = X[:, :nroots].cpu().numpy()
+    return np.asarray(e[:nroots]), Xn, np.zeros_like(Xn)
+
+
+def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
+                          W_aux=None, orbsym=None, isdf_factors=None, stats=None,
+                          spin='singlet', comm=None, timings=None,
+                          refuse_unconverged=False, max_memory=None,
+                          read_roots=None, read_tol=None, guess=None, tda=False,
+                          one_body=None, tracked=None, probe=False, nroots=5,
+                          conv_tol=1e-8, max_cycle=150, guess_factor=2,
+                          preconditioner=DAVIDSON_PRECONDITIONER,
+                          precision='fp32', refine=True, learned=None):
+    """The matrix-free Davidson entry (LR/davidson.py:252) on the GPU, TDA only.
+
+    `numerics` may carry a `bse` section with nroots, conv_tol, max_cycle,
+    guess_factor and preconditioner (numerics.py:241-286); otherwise the
+    keyword values (the reference defaults) are used.  precision: the block
+    action's; with refine (default) a lower-precision solve is re-checked and
+    polished on an FP64 action.  Returns (omega, X, Y).
+    """
+    from .block_actions import block_action
+    bse = getattr(numerics, 'bse', None)
+    if bse is not None:
+        nroots, conv_tol, max_cycle = bse.nroots, bse.conv_tol, bse.max_cycle
+        guess_factor, preconditioner = bse.guess_factor, bse.preconditioner
+    check_preconditioner(preconditioner)
+    if orbsym is not None and tda:
+        raise ValueError('the Tamm-Dancoff Davidson takes no pair symmetry: its '
+                         'trial space is one Hermitian block')
+    if one_body is not None:
+        raise NotImplementedError('one_body coupling is not built')
+    from .orbital_spaces import bse_pair_diagonal
+    check_residual_floor(conv_tol, bse_pair_diagonal(lr_solver.eps, nocc))
+    apply_AB, diag_d = block_action(lr_solver, nocc, polarizability, W_aux,
+                                    isdf_factors, spin=spin, comm=comm, tda=tda,
+                                    precision=precision)
+    refine_apply = None
+    if refine and precision != 'fp64':
+        refine_apply = block_action(lr_solver, nocc, polarizability, W_aux,
+                                    isdf_factors, spin=spin, comm=comm, tda=tda,
+                                    precision='fp64')[0]
+    return run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle,
+                        guess_factor=guess_factor, stats=stats, comm=comm,
+                        timings=timings, refuse_unconverged=refuse_unconverged,
+                        preconditioner=preconditioner, guess=guess, tda=tda,
+                        tracked=tracked, refine_apply=refine_apply, learned=learned)

@@ -1,0 +1,337 @@
+"""Space-time polarizability, static screening and the dRPA energy on the GPU.
+
+Drop-ins for src/SingleReference/LinearResponse/space_time.py:
+
+* `split_branches` (:448), `polarizability_projected_tau` (:478),
+  `polarizability_projected_sweep` (:533), `chi0_imaginary_frequency` (:607),
+  `static_screening_matrix` (:1003), `dyson_screening` (:956),
+  `dielectric_log_determinant` (:1073), `rpa_correlation_energy_space_time`
+  (:1100);
+* plus the grid-gauge forms the synthetic workloads use
+  (`chi0_grid`, `screened_interaction_grid`, `rpa_correlation_energy_grid`),
+  where the interaction is given on the points (V) instead of as a factor D.
+
+The N^3 step -- every tau point's Go o Gv -- runs in libmbpt_hip.so (chi0.hip,
+FP64 or FP32 MFMA, tau loop fused).  The steps the reference keeps in FP64
+and that are N^3 only once (the D projection, the Dyson solve, log-dets) run
+in FP64 on the device through torch (rocBLAS / rocSOLVER): W through a
+Cholesky factorisation of the positive-definite I - D^T chi0 D, never an
+explicit inverse.
+"""
+import ctypes
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib
+from .constants import ISDF_TILE_GB
+from .grids import static_screening_grid
+from .orbital_spaces import particle_hole_window
+
+
+class NonPositiveDielectric(RuntimeWarning):
+    """det(1 - chi0) is not positive at an imaginary frequency."""
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError('the space-time chi0 kernels run on a ROCm GPU; none is visible')
+    return torch.device('cuda', torch.cuda.current_device())
+
+
+def _dev(a, dev):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=torch.float64).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+
+
+def split_branches(X, eps, nocc, mu=None):
+    """(X_o, X_v, e_o, e_v, mu, occ, virt): the collocation split at the gap,
+    energies shifted by mu (default mid-gap) (space_time.py:448-475)."""
+    eps = np.asarray(eps, float)
+    if mu is None:
+        mu = particle_hole_window(eps, nocc)[2]
+    occ, virt = np.arange(nocc), np.arange(nocc, len(eps))
+    if isinstance(X, torch.Tensor):
+        X_o, X_v = X[:, :nocc].contiguous(), X[:, nocc:].contiguous()
+    else:
+        X = np.asarray(X, float)
+        X_o, X_v = np.ascontiguousarray(X[:, occ]), np.ascontiguousarray(X[:, virt])
+    return X_o, X_v, eps[occ] - mu, eps[virt] - mu, mu, occ, virt
+
+
+class Chi0Builder:
+    """Device-resident collocation for repeated chi0 builds (one handle)."""
+
+    def __init__(self, X, eps, nocc, mu=None, precision='fp32', device=None):
+        dev = _device() if device is None else torch.device(device)
+        eps = np.asarray(eps, float)
+        if mu is None:
+            mu = particle_hole_window(eps, nocc)[2]
+        Xd = _dev(X, dev)
+        self.M, norb = Xd.shape
+        if norb != len(eps):
+            raise ValueError(f'X has {norb} orbitals, eps has {len(eps)}')
+        self.nocc, self.nvir = nocc, norb - nocc
+        self.device, self.precision, self.mu = dev, precision, float(mu)
+        self._lib = _lib.lib()
+        h = ctypes.c_void_p()
+        _, eps_p = _lib.host_doubles(eps)
+        self._eps_keep = _
+        with torch.cuda.device(dev):
+            _lib.check(self._lib.mbpt_chi0_create(
+                ctypes.byref(h), self.M, nocc, self.nvir,
+                _lib.precision_code(precision), _lib.dptr(Xd), eps_p, self.mu,
+                _lib.stream_ptr()), 'mbpt_chi0_create')
+            torch.cuda.current_stream().synchronize()
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, '_h', None)
+        if h is not None and h.value:
+            try:
+                torch.cuda.synchronize(self.device)
+            except Exception:
+                pass
+            self._lib.mbpt_chi0_destroy(h)
+            self._h = None
+
+    def profile(self, enable=True):
+        _lib.check(self._lib.mbpt_chi0_profile(self._h, int(enable)), 'mbpt_chi0_profile')
+
+    def profile_read(self):
+        """{'ms', 'launches', 'flops'} of the fused chi0 launches since `profile()`."""
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_long()
+        _lib.check(self._lib.mbpt_chi0_profile_read(self._h, ctypes.byref(ms),
+                                                     ctypes.byref(n), ctypes.byref(fl)),
+                   'mbpt_chi0_profile_read')
+        return {'ms': float(ms.value), 'launches': int(n.value), 'flops': float(fl.value)}
+
+    def build(self, tau, weights, out=None, accumulate=False):
+        """out (M, M) FP64 = [out] + sum_t weights[t] (-2) Go(t) o Gv(t)."""
+        tau = np.atleast_1d(np.asarray(tau, float))
+        weights = np.atleast_1d(np.asarray(weights, float))
+        if tau.shape != weights.shape:
+            raise ValueError('tau and weights differ in length')
+        if out is None:
+            out = torch.empty((self.M, self.M), dtype=torch.float64, device=self.device)
+            accumulate = False
+        for t0 in range(0, len(tau), _lib.CHI0_MAX_TAU):
+            tt, tp = _lib.host_doubles(tau[t0:t0 + _lib.CHI0_MAX_TAU])
+            ww, wp = _lib.host_doubles(weights[t0:t0 + _lib.CHI0_MAX_TAU])
+            _lib.check(self._lib.mbpt_chi0_build(self._h, len(tt), tp, wp,
+                                                  _lib.dptr(out),
+                                                  int(accumulate or t0 > 0),
+                                                  _lib.stream_ptr()),
+                       'mbpt_chi0_build')
+        return out
+
+
+def chi0_grid(X, eps, nocc, tau, weights, mu=None, precision='fp32', builder=None):
+    """sum_t weights[t] (-2) Go(t) o Gv(t) on the grid, (M, M) FP64 device."""
+    b = builder or Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    return b.build(tau, weights)
+
+
+def _project(P, D):
+    """D^T P D in FP64 (D None: the grid gauge, P itself)."""
+    if D is None:
+        return P
+    return D.T @ (P @ D)
+
+
+def polarizability_projected_tau(X_o, X_v, e_o, e_v, D, tau,
+                                 tile_memory_gb=ISDF_TILE_GB, out=None, work=None,
+                                 tiles=None, precision='fp64'):
+    """proj(tau) = -2 D^T (Go o Gv) D at one time point (space_time.py:478-530).
+
+    X_o, X_v, e_o, e_v as `split_branches` returns them (energies already
+    shifted).  D (M, naux), or None for the grid gauge.  `out` (naux, naux)
+    is overwritten when given; `work` is accepted and never read (the kernel
+    tiles in registers and LDS); `tiles` must be None (the whole sum).
+    Returns a NumPy array for NumPy inputs, else a device tensor.
+    """
+    if tiles is not None:
+        raise NotImplementedError('partial tile sums are a multi-rank feature')
+    numpy_in = not isinstance(X_o, torch.Tensor)
+    dev = _device()
+    Xo, Xv = _dev(X_o, dev), _dev(X_v, dev)
+    X = torch.cat([Xo, Xv], dim=1)
+    eps = np.concatenate([np.asarray(e_o, float), np.asarray(e_v, float)])
+    P = chi0_grid(X, eps, Xo.shape[1], [float(tau)], [1.0], mu=0.0,
+                  precision=precision)
+    Dd = None if D is None else _dev(D, dev)
+    res = _project(P, Dd)
+    if out is not None:
+        if isinstance(out, torch.Tensor):
+            out.copy_(res)
+        else:
+            out[...] = res.cpu().numpy()
+        return out
+    return res.cpu().numpy() if numpy_in else res
+
+
+def polarizability_projected_sweep(X, D, eps, nocc, tau_points, mu=None,
+                                   tau_indices=None, tile_memory_gb=ISDF_TILE_GB,
+                                   precision='fp64'):
+    """proj(tau) on every time point, (ntau, naux, naux) (space_time.py:533-560);
+    slots outside `tau_indices` stay zero."""
+    dev = _device()
+    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    Dd = None if D is None else _dev(D, dev)
+    tau_points = np.asarray(tau_points, float)
+    naux = b.M if Dd is None else Dd.shape[1]
+    proj = torch.zeros((len(tau_points), naux, naux), dtype=torch.float64, device=dev)
+    which = range(len(tau_points)) if tau_indices is None else np.atleast_1d(tau_indices)
+    for k in which:
+        proj[k] = _project(b.build([tau_points[k]], [1.0]), Dd)
+    return proj
+
+
+def chi0_imaginary_frequency(X, D, eps, nocc, grid, mu=None, tau_indices=None,
+                             tile_memory_gb=ISDF_TILE_GB, precision='fp64'):
+    """chi0(i w_k) = sum_tau cosft_wt[k, tau] proj(tau), (nfreq, naux, naux)
+    (space_time.py:607-631); each frequency is one fused kernel launch over
+    all time points."""
+    dev = _device()
+    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    Dd = None if D is None else _dev(D, dev)
+    naux = b.M if Dd is None else Dd.shape[1]
+    which = np.arange(grid.ntau) if tau_indices is None else np.atleast_1d(tau_indices)
+    tau = grid.tau_points[which]
+    chi = torch.empty((grid.nfreq, naux, naux), dtype=torch.float64, device=dev)
+    for k in range(grid.nfreq):
+        chi[k] = _project(b.build(tau, grid.cosft_wt[k, which]), Dd)
+    return chi
+
+
+def dyson_screening(chi0, transform=None, correlation=False):
+    """W = [I - chi0]^-1 (space_time.py:956-968), through the Cholesky factor of
+    the positive-definite I - chi0 (chi0 is negative semidefinite on the
+    imaginary axis)."""
+    c = chi0 if isinstance(chi0, torch.Tensor) else torch.as_tensor(np.asarray(chi0, float))
+    if transform is not None:
+        T = transform if isinstance(transform, torch.Tensor) else torch.as_tensor(
+            np.asarray(transform, float), device=c.device)
+        c = T.T @ c @ T
+    n = c.shape[-1]
+    A = torch.eye(n, dtype=c.dtype, device=c.device) - 0.5 * (c + c.T)
+    L, info = torch.linalg.cholesky_ex(A)
+    if int(info) != 0:
+        raise np.linalg.LinAlgError('I - chi0 is not positive definite')
+    W = torch.cholesky_inverse(L)
+    if correlation:
+        W.diagonal().sub_(1.0)
+    return W.cpu().numpy() if not isinstance(chi0, torch.Tensor) else W
+
+
+def static_screening_matrix(X, D, eps, nocc, grid=None, mu=None, comm=None,
+                            tile_memory_gb=ISDF_TILE_GB, branches=None,
+                            precision='fp64'):
+    """W_aux = [1 - chi0(i w = 0)]^-1 in D's auxiliary basis
+    (space_time.py:1003-1050).  Returns NumPy for NumPy inputs."""
+    if comm is not None and comm.Get_size() > 1:
+        raise NotImplementedError('one GPU: no tau split over ranks')
+    grid = static_screening_grid(eps, nocc) if grid is None else grid
+    chi0 = chi0_imaginary_frequency(X, D, eps, nocc, grid, mu=mu,
+                                    precision=precision)[0]
+    W = dyson_screening(chi0)
+    numpy_in = not isinstance(X, torch.Tensor)
+    return W.cpu().numpy() if numpy_in else W
+
+
+def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
+                              precision='fp32', chi0=None):
+    """(W, chi0) on the grid: W = (I - V chi0(0))^-1 V computed as
+    L (I - L^T chi0 L)^-1 L^T with L = chol(V) -- two FP64 Cholesky
+    factorisations, a triangular solve and a symmetric product on the device;
+    chi0(0) from the fused kernel at `precision`."""
+    dev = _device()
+    grid = static_screening_grid(eps, nocc) if grid is None else grid
+    Vd = _dev(V, dev)
+    if chi0 is None:
+        chi0 = chi0_grid(X, eps, nocc, grid.tau_points, grid.cosft_wt[0], mu=mu,
+                         precision=precision)
+    L = torch.linalg.cholesky(Vd)
+    K = L.T @ (chi0 @ L)
+    n = K.shape[0]
+    A = torch.eye(n, dtype=K.dtype, device=dev) - 0.5 * (K + K.T)
+    C, info = torch.linalg.cholesky_ex(A)
+    if int(info) != 0:
+        raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
+    # W = L A^-1 L^T = (L C^-T)(L C^-T)^T
+    Y = torch.linalg.solve_triangular(C, L.T, upper=False).T   # L C^-T
+    W = Y @ Y.T
+    return 0.5 * (W + W.T), chi0
+
+
+def dielectric_log_determinant(one_minus_c, where, flags=None):
+    """log|det(1 - c)| through a Cholesky factor (space_time.py:1073-1097);
+    a matrix that is not positive definite falls to slogdet, warns
+    `NonPositiveDielectric` when the determinant is not positive and records
+    `where` in flags['nonpositive_det']."""
+    A = one_minus_c
+    L, info = torch.linalg.cholesky_ex(0.5 * (A + A.T))
+    if int(info) == 0:
+        return float(2.0 * torch.log(torch.diagonal(L)).sum())
+    sign, logdet = torch.linalg.slogdet(A)
+    if not float(sign) > 0:
+        warnings.warn(f'det(1 - chi0) is not positive at {where} (sign {float(sign)})',
+                      NonPositiveDielectric, stacklevel=2)
+        if flags is not None:
+            flags.setdefault('nonpositive_det', []).append(where)
+    return float(logdet)
+
+
+def rpa_correlation_energy_space_time(X, D, eps, nocc, grid, mu=None,
+                                      tile_gb=ISDF_TILE_GB, screening=None,
+                                      counter_term=None, comm=None,
+                                      want_tape=False, flags=None,
+                                      precision='fp64'):
+    """E_c = (1/2 pi) sum_k w_k [log det(I - chi0_k) + Tr chi0_k]
+    (space_time.py:1100-1183), chi0_k in D's auxiliary basis.  The
+    rescaled-interaction and counter-term variants of the solvation routes are
+    not part of this path."""
+    if screening is not None or counter_term is not None or want_tape:
+        raise NotImplementedError('screening / counter_term / want_tape are not built')
+    if comm is not None and comm.Get_size() > 1:
+        raise NotImplementedError('one GPU: no frequency split over ranks')
+    dev = _device()
+    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    Dd = None if D is None else _dev(D, dev)
+    mine = {}
+    e_c = 0.0
+    for k in range(grid.nfreq):
+        c = _project(b.build(grid.tau_points, grid.cosft_wt[k]), Dd)
+        n = c.shape[0]
+        logdet = dielectric_log_determinant(torch.eye(n, dtype=c.dtype, device=dev) - c,
+                                            k, mine)
+        e_c += grid.omega_weights[k] * (logdet + float(torch.trace(c)))
+    if flags is not None and mine.get('nonpositive_det'):
+        flags.setdefault('nonpositive_det', []).extend(mine['nonpositive_det'])
+    return e_c / (2.0 * np.pi)
+
+
+def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32',
+                                flags=None):
+    """E_c with the interaction on the grid: log det(I - L^T chi0 L) with
+    L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); one fused chi0 launch per
+    frequency, FP64 GEMMs and Cholesky log-dets."""
+    dev = _device()
+    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    Vd = _dev(V, dev)
+    L = torch.linalg.cholesky(Vd)
+    n = L.shape[0]
+    eye = torch.eye(n, dtype=torch.float64, device=dev)
+    chi = torch.empty((n, n), dtype=torch.float64, device=dev)
+    e_c, mine = 0.0, {}
+    for k in range(grid.nfreq):
+        b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
+        K = L.T @ (chi @ L)
+        logdet = dielectric_log_determinant(eye - K, k, mine)
+        e_c += grid.omega_weights[k] * (logdet + float(torch.trace(K)))
+    if flags is not None and mine.get('nonpositive_det'):
+        flags.setdefault('nonpositive_det', []).extend(mine['nonpositive_det'])
+    return e_c / (2.0 * np.pi)

@@ -1,0 +1,392 @@
+"""TEST INFRASTRUCTURE ONLY -- the FP64 NumPy oracle of the ISDF BSE sigma-vector
+and space-time chi0 path.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg
+may import this module, and only as the checker (or as the timed CPU
+baseline).  The product (``mbptcode_amd``) never imports it and has no CPU
+fallback through it.
+
+Two kinds of functions live here:
+
+1. Restatements of the reference's own algorithm, step for step, in FP64
+   (file:line given per function):
+   ``isdf_block_action`` (serial, Tamm-Dancoff branch), ``split_branches``,
+   ``polarizability_projected_tau``, ``static_screening_matrix``,
+   ``chi0_imaginary_frequency``, ``rpa_correlation_energy_space_time``,
+   ``davidson_tda`` (the Jacobi-Davidson TDA iteration).
+2. Closed forms that pin them independently of the restatement:
+   ``dense_tda_matrix`` (A from the ISDF factors, O(n_pair^2 M)),
+   ``proj_tau_bruteforce`` (the pair sum of chi0(i tau)),
+   ``chi0_omega_closed_form`` (the exact frequency-axis chi0),
+   ``rpa_energy_casida`` (Furche's trace formula).
+
+Pinning status (what checks each function; see tests/test_oracle.py):
+
+* isdf_block_action       -- dense A (closed form), tiling invariance
+* screened_diagonal       -- diagonal of the dense direct term
+* polarizability_projected_tau -- brute-force pair sum, tiling invariance
+* chi0_imaginary_frequency -- closed form chi0(i w) at the fit's error
+* static_screening_matrix -- Dyson identity W (I - chi0) = I, closed form chi0(0)
+* rpa_correlation_energy_space_time -- Casida trace at quadrature error
+* davidson_tda            -- dense eigh of the same A
+* the cosine-transform weights' exact bits -- parity unpinned (only an import
+  of the reference could pin them; pinned here to the fit's model error)
+"""
+import numpy as np
+
+BYTES_PER_GB = 1e9
+
+
+# ---------------------------------------------------------------------------
+# conventions
+# ---------------------------------------------------------------------------
+
+def pair_diagonal(eps, nocc):
+    """d[i, a] = eps_a - eps_i (block_actions.py:99-115, orbital_spaces.py:31)."""
+    eps = np.asarray(eps, float)
+    return eps[nocc:][None, :] - eps[:nocc][:, None]
+
+
+def particle_hole_window(eps, nocc):
+    """(gap, span, mu) (orbital_spaces.py:12-20)."""
+    eps = np.asarray(eps, float)
+    homo, lumo = eps[:nocc].max(), eps[nocc:].min()
+    return lumo - homo, eps[nocc:].max() - eps[:nocc].min(), 0.5 * (homo + lumo)
+
+
+def grid_factors(V):
+    """D = chol(V): the factor under which the grid-gauge V = D D^T; the
+    reference's auxiliary basis is then the grid itself (naux = M)."""
+    return np.linalg.cholesky(V)
+
+
+# ---------------------------------------------------------------------------
+# the block action (LR/block_actions.py:365-714, serial, tda=True)
+# ---------------------------------------------------------------------------
+
+def screened_rows(D, W_aux):
+    """Zt = D (W_aux D^T), the serial association (block_actions.py:275-294)."""
+    return D @ (W_aux @ D.T)
+
+
+def isdf_block_action(eps, nocc, X, D, W_aux, kappa=2.0, tile_memory_gb=4.0):
+    """(apply_A, diag_d) of the Tamm-Dancoff ISDF action.
+
+    apply_A(z (k, nocc, nvir)) -> A z, following block_actions.py:563-661 with
+    one grid-row block (serial) and tda=True:
+
+        A z = d o z + kappa X_o^T (u o X_v) - (X_o^T (Zt o (X_o z X_v^T))) X_v
+        u   = D (D^T p),  p_mu = sum_j X_o[mu, j] (z X_v^T)[j, mu]
+
+    The Hadamard product is tiled over grid rows of `rows` (:528-530) and the
+    exchange partial X_o^T (Zt o P) is accumulated tile by tile (:622-631).
+    """
+    X = np.asarray(X, float)
+    X_o = np.ascontiguousarray(X[:, :nocc])
+    X_v = np.ascontiguousarray(X[:, nocc:])
+    npts = X.shape[0]
+    d = pair_diagonal(eps, nocc)
+    asym = np.abs(W_aux - W_aux.T).max()
+    if asym > 1e-10 * np.abs(W_aux).max():
+        raise ValueError(f'W_aux is not symmetric (max asymmetry {asym:.2e})')
+    Zt = screened_rows(D, W_aux)
+    rows = max(1, min(npts, int(tile_memory_gb * BYTES_PER_GB / max(npts * 8, 1))))
+
+    def apply_A(z):
+        z = np.asarray(z, float)
+        Az = d[None] * z
+        for n, zn in enumerate(z):
+            zXv = zn @ X_v.T                                    # (nocc, M)
+            if kappa:
+                p = np.einsum('kj,jk->k', X_o, zXv)
+                u = D @ (p @ D)
+                Az[n] += kappa * (X_o.T @ (u[:, None] * X_v))
+            T = np.empty((nocc, npts))
+            for p0 in range(0, npts, rows):
+                p1 = min(p0 + rows, npts)
+                blk = X_o[p0:p1] @ zXv                          # P's rows
+                blk *= Zt[p0:p1]                                # Zt o P
+                Tb = X_o[p0:p1].T @ blk
+                if p0 == 0:
+                    T[...] = Tb
+                else:
+                    T += Tb
+            Az[n] -= T @ X_v
+        return Az
+
+    def screened_diagonal():
+        """d - (ii|W|aa) with (ii|W|aa) = [D^T (X_o o X_o)]^T W_aux
+        [D^T (X_v o X_v)] (block_actions.py:297-320, :663-689)."""
+        B_o = D.T @ (X_o * X_o)
+        B_v = D.T @ (X_v * X_v)
+        return d - (B_o.T @ W_aux) @ B_v
+
+    apply_A.screened_diagonal = screened_diagonal
+    return apply_A, d
+
+
+def dense_tda_matrix(eps, nocc, X, D, W_aux, kappa=2.0):
+    """A[ia, jb] = d_ia delta + kappa (ia|jb) - (ij|W|ab) built whole from the
+    factors (linear_response.py:291-409 in the separable-RI form):
+
+        (ia|jb)   = sum_{mu nu} C[mu, ia] Z[mu, nu] C[nu, jb],  C = X_o o X_v
+        (ij|W|ab) = sum_{mu nu} X_o[mu,i] X_o[mu,j] Zt[mu,nu] X_v[nu,a] X_v[nu,b]
+
+    Z = D D^T, Zt = D W_aux D^T.  O(n_pair^2 M) memory-light for the small
+    workloads; the independent check of the block action.
+    """
+    X = np.asarray(X, float)
+    X_o, X_v = X[:, :nocc], X[:, nocc:]
+    no, nv = X_o.shape[1], X_v.shape[1]
+    npair = no * nv
+    C = (X_o[:, :, None] * X_v[:, None, :]).reshape(-1, npair)
+    CD = D.T @ C
+    hartree = CD.T @ CD
+    Zt = D @ W_aux @ D.T
+    Bo = (X_o[:, :, None] * X_o[:, None, :]).reshape(-1, no * no)
+    Bv = (X_v[:, :, None] * X_v[:, None, :]).reshape(-1, nv * nv)
+    direct = (Bo.T @ Zt @ Bv).reshape(no, no, nv, nv)     # [i, j, a, b]
+    direct = direct.transpose(0, 2, 1, 3).reshape(npair, npair)
+    A = kappa * hartree - direct
+    A[np.diag_indices(npair)] += pair_diagonal(eps, nocc).ravel()
+    return A
+
+
+# ---------------------------------------------------------------------------
+# space-time chi0 (LR/space_time.py)
+# ---------------------------------------------------------------------------
+
+def split_branches(X, eps, nocc, mu=None):
+    """(X_o, X_v, e_o, e_v, mu): the split at the gap, energies shifted by the
+    mid-gap mu (space_time.py:448-475)."""
+    eps = np.asarray(eps, float)
+    if mu is None:
+        mu = particle_hole_window(eps, nocc)[2]
+    X = np.asarray(X, float)
+    return (np.ascontiguousarray(X[:, :nocc]), np.ascontiguousarray(X[:, nocc:]),
+            eps[:nocc] - mu, eps[nocc:] - mu, mu)
+
+
+def tile_rows(M, tile_memory_gb, per_row_bytes):
+    """Rows per grid tile under the budget (space_time.py:409-412)."""
+    return max(1, min(M, int(tile_memory_gb * BYTES_PER_GB / max(per_row_bytes, 1))))
+
+
+def polarizability_projected_tau(X_o, X_v, e_o, e_v, D, tau, tile_memory_gb=4.0):
+    """proj(tau) = -2 D^T [(X_o e^{e_o tau} X_o^T) o (X_v e^{-e_v tau} X_v^T)] D,
+    tiled over grid rows of 3 M 8 bytes each (space_time.py:478-530)."""
+    M = X_o.shape[0]
+    naux = D.shape[1]
+    rows = tile_rows(M, tile_memory_gb, 3 * M * 8)
+    out = np.zeros((naux, naux))
+    eo_t, ev_t = np.exp(e_o * tau), np.exp(-e_v * tau)
+    for p0 in range(0, M, rows):
+        p1 = min(p0 + rows, M)
+        Go = (X_o[p0:p1] * eo_t) @ X_o.T
+        Gv = (X_v[p0:p1] * ev_t) @ X_v.T
+        Go *= Gv
+        out += D[p0:p1].T @ (Go @ D)
+    out *= -2.0
+    return out
+
+
+def polarizability_projected_sweep(X, D, eps, nocc, tau_points, mu=None,
+                                   tile_memory_gb=4.0):
+    """proj(tau) on every time point, (ntau, naux, naux) (space_time.py:533-560)."""
+    X_o, X_v, e_o, e_v, _ = split_branches(X, eps, nocc, mu)
+    return np.stack([polarizability_projected_tau(X_o, X_v, e_o, e_v, D, t,
+                                                  tile_memory_gb)
+                     for t in np.asarray(tau_points, float)])
+
+
+def chi0_imaginary_frequency(X, D, eps, nocc, tau_points, cosft_wt, mu=None,
+                             tile_memory_gb=4.0):
+    """chi0(i w_k) = sum_tau cosft_wt[k, tau] proj(tau), (nfreq, naux, naux)
+    (space_time.py:607-651)."""
+    proj = polarizability_projected_sweep(X, D, eps, nocc, tau_points, mu,
+                                          tile_memory_gb)
+    return np.tensordot(np.asarray(cosft_wt, float), proj, axes=(1, 0))
+
+
+def dyson_screening(chi0):
+    """W = [I - chi0]^-1 (space_time.py:956-968)."""
+    return np.linalg.inv(np.eye(chi0.shape[-1]) - chi0)
+
+
+def static_screening_matrix(X, D, eps, nocc, tau_points, cosft_wt0, mu=None,
+                            tile_memory_gb=4.0):
+    """W_aux = [I - chi0(i w = 0)]^-1 with chi0(0) = sum_tau c_tau proj(tau)
+    (space_time.py:1003-1050); cosft_wt0 is the (ntau,) row of omega = 0."""
+    chi0 = chi0_imaginary_frequency(X, D, eps, nocc, tau_points,
+                                    np.atleast_2d(cosft_wt0), mu,
+                                    tile_memory_gb)[0]
+    W = dyson_screening(chi0)
+    # The inverse is symmetric only to round-off (1e-8 relative at M = 1200),
+    # above the 1e-10 guard the block action applies (block_actions.py:520);
+    # the symmetric part is the same matrix to that round-off.
+    return 0.5 * (W + W.T)
+
+
+def rpa_correlation_energy_space_time(X, D, eps, nocc, tau_points, cosft_wt,
+                                      omega_weights, mu=None, tile_memory_gb=4.0):
+    """E_c = (1/2 pi) sum_k w_k [log det(I - chi0_k) + Tr chi0_k]
+    (space_time.py:1100-1183, serial; slogdet as dielectric_log_determinant
+    :1073-1097)."""
+    chi = chi0_imaginary_frequency(X, D, eps, nocc, tau_points, cosft_wt, mu,
+                                   tile_memory_gb)
+    e_c = 0.0
+    eye = np.eye(chi.shape[-1])
+    for k, c in enumerate(chi):
+        sign, logdet = np.linalg.slogdet(eye - c)
+        if not sign > 0:
+            raise FloatingPointError(f'det(1 - chi0) not positive at {k}')
+        e_c += omega_weights[k] * (logdet + np.trace(c))
+    return e_c / (2.0 * np.pi)
+
+
+def grid_screened_interaction(W_aux, D):
+    """Zt = D W_aux D^T: the grid-space W the device kernels take."""
+    return D @ W_aux @ D.T
+
+
+# ---------------------------------------------------------------------------
+# closed forms
+# ---------------------------------------------------------------------------
+
+def _pair_densities(X, nocc, D):
+    X = np.asarray(X, float)
+    X_o, X_v = X[:, :nocc], X[:, nocc:]
+    C = (X_o[:, :, None] * X_v[:, None, :]).reshape(X.shape[0], -1)
+    return D.T @ C                                           # (naux, npair)
+
+
+def proj_tau_bruteforce(X, D, eps, nocc, tau):
+    """-2 sum_ia e^{-(eps_a - eps_i) tau} (D^T C_ia)(D^T C_ia)^T: the mid-gap
+    shift cancels between the two exponentials."""
+    CD = _pair_densities(X, nocc, D)
+    f = np.exp(-pair_diagonal(eps, nocc).ravel() * tau)
+    return -2.0 * (CD * f) @ CD.T
+
+
+def chi0_omega_closed_form(X, D, eps, nocc, omega):
+    """-4 sum_ia d/(d^2 + w^2) (D^T C_ia)(D^T C_ia)^T, the exact chi0(i w)
+    (imaginary_frequency.py f_rpa convention: 2 C f C^T with f = -2d/(d^2+w^2))."""
+    CD = _pair_densities(X, nocc, D)
+    d = pair_diagonal(eps, nocc).ravel()
+    return -4.0 * (CD * (d / (d ** 2 + omega ** 2))) @ CD.T
+
+
+def rpa_energy_casida(X, D, eps, nocc):
+    """E_c = 1/2 (sum Omega - Tr A), A = d + 2K, B = 2K, K = (ia|jb)
+    (rpa_energy.py:50-60); Omega^2 = eig(d^1/2 (d + 4K) d^1/2)."""
+    CD = _pair_densities(X, nocc, D)
+    K = CD.T @ CD
+    d = pair_diagonal(eps, nocc).ravel()
+    sq = np.sqrt(d)
+    H = (sq[:, None] * (np.diag(d) + 4.0 * K)) * sq[None, :]
+    omega = np.sqrt(np.linalg.eigvalsh(H))
+    return 0.5 * (omega.sum() - (d.sum() + 2.0 * np.trace(K)))
+
+
+# ---------------------------------------------------------------------------
+# the Davidson (Solvers/davidson.py, LR/davidson_shared.py, LR/davidson.py)
+# ---------------------------------------------------------------------------
+
+def guess_indices(d, nroots, guess_factor=2, deg_tol=1e-3):
+    """Unit-vector guess pairs (davidson_shared.py:19-54)."""
+    d = np.asarray(d, float).ravel()
+    order = np.argsort(d)
+    n = min(max(guess_factor * nroots, nroots + 4), d.size)
+    cut = d[order[n - 1]] + deg_tol
+    n_max = min(max(4 * nroots, 20, n), d.size)
+    while n < n_max and d[order[n]] <= cut:
+        n += 1
+    return order[:n]
+
+
+def tracked_roots(nroots, npair, extra=None):
+    """nroots + max(2, nroots // 2) (davidson_shared.py:146-154)."""
+    if extra is None:
+        extra = max(2, nroots // 2)
+    return min(nroots + int(extra), npair)
+
+
+def jacobi_davidson_correction(r, u, diag, theta, floor=1e-8):
+    """t = K^-1 r - alpha K^-1 u, alpha = u.K^-1 r / u.K^-1 u, |diag - theta|
+    floored (Solvers/davidson.py:43-77)."""
+    den = diag - theta
+    den = np.where(np.abs(den) < floor, floor, den)
+    Kr = r / den
+    Ku = u / den
+    uKu = float(u @ Ku)
+    if abs(uKu) > 1e-300:
+        Kr = Kr - ((u @ Kr) / uKu) * Ku
+    return Kr
+
+
+def davidson_tda(apply_A, diag, nroots, tol=1e-6, max_cycle=200,
+                 max_space=None, precond_diag=None):
+    """The nroots lowest eigenpairs of the symmetric operator apply_A by a block
+    Jacobi-Davidson: unit-vector guess on the lowest pairs, tracked roots
+    above nroots, residual-norm convergence, collapse onto the tracked Ritz
+    vectors when the space is full (the iteration Solvers/davidson.py:179-326
+    drives through its eigensolver; restated here as a plain loop).
+
+    apply_A maps (n, m) columns to (n, m).  Returns (e, X, conv, n_applied).
+    """
+    diag = np.asarray(diag, float).ravel()
+    pdiag = diag if precond_diag is None else np.asarray(precond_diag, float).ravel()
+    n = diag.size
+    track = tracked_roots(nroots, n)
+    if max_space is None:
+        max_space = max(10 * track + 30, 60)
+    max_space = min(max_space, n)
+    cols = guess_indices(diag, track)
+    Vb = np.zeros((n, len(cols)))
+    Vb[cols, np.arange(len(cols))] = 1.0
+    AV = apply_A(Vb)
+    applied = Vb.shape[1]
+    V = Vb
+    conv = np.zeros(track, bool)
+    e = x = None
+    for _ in range(max_cycle):
+        H = V.T @ AV
+        H = 0.5 * (H + H.T)
+        w, y = np.linalg.eigh(H)
+        w, y = w[:track], y[:, :track]
+        x = V @ y
+        ax = AV @ y
+        r = ax - x * w
+        rn = np.linalg.norm(r, axis=0)
+        conv = rn <= tol
+        e = w
+        if conv.all():
+            break
+        new = [jacobi_davidson_correction(r[:, j], x[:, j], pdiag, w[j])
+               for j in np.flatnonzero(~conv)]
+        T = np.column_stack(new)
+        T /= np.linalg.norm(T, axis=0)
+        if V.shape[1] + T.shape[1] > max_space:
+            V, AV = x, ax
+        for _ in range(2):
+            T -= V @ (V.T @ T)
+        keep = []
+        for j in range(T.shape[1]):
+            t = T[:, j]
+            for q in keep:
+                t -= T[:, q] * (T[:, q] @ t)
+            nrm = np.linalg.norm(t)
+            if nrm > 1e-8:
+                T[:, j] = t / nrm
+                keep.append(j)
+        if not keep:
+            break
+        T = T[:, keep]
+        for _ in range(1):
+            T -= V @ (V.T @ T)
+            T /= np.linalg.norm(T, axis=0)
+        AT = apply_A(T)
+        applied += T.shape[1]
+        V = np.hstack([V, T])
+        AV = np.hstack([AV, AT])
+    return e[:nroots], x[:, :nroots], conv[:nroots], applied

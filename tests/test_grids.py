@@ -1,0 +1,91 @@
+"""Host-side grids against the identities they exist to provide (the cases of
+the reference's tests/test_minimax_tau_grid.py:42-146 and tests/test_grids.py).
+CPU only."""
+import numpy as np
+import pytest
+
+from mbptcode_amd import grids
+
+IN_TABLE = [(6, 1.0, 3.0, 1e-6), (8, 1.0, 12.0, 1e-5), (14, 0.3, 30.0, 1e-6),
+            (20, 0.3, 30.0, 1e-9), (30, 1.0, 4000.0, 1e-8)]
+STRETCHED = [(22, 1.0, 100.0, 1e-9), (24, 1.0, 100.0, 1e-9), (28, 1.0, 100.0, 1e-9),
+             (30, 1.0, 100.0, 1e-9), (34, 1.0, 100.0, 1e-9), (34, 0.3, 30.0, 1e-9),
+             (30, 2.0, 60.0, 1e-9)]
+
+
+def relative_laplace_error(ntau, e_min, e_max, nsample=2001):
+    tau, w = grids.minimax_time_grid(ntau, e_min, e_max)
+    assert tau.shape == (ntau,) and w.shape == (ntau,)
+    x = np.geomspace(e_min, e_max, nsample)
+    return float(np.abs(1.0 - x * (np.exp(-np.outer(x, tau)) @ w)).max())
+
+
+@pytest.mark.parametrize('ntau,e_min,e_max,tol', IN_TABLE)
+def test_laplace_identity_on_tabulated_ranges(ntau, e_min, e_max, tol):
+    assert relative_laplace_error(ntau, e_min, e_max) < tol
+
+
+@pytest.mark.parametrize('ntau,e_min,e_max,tol', STRETCHED)
+def test_narrow_range_stretch_keeps_the_identity(ntau, e_min, e_max, tol):
+    assert relative_laplace_error(ntau, e_min, e_max) < tol
+
+
+def test_stretch_multiplies_time_points():
+    """Dividing instead (the frequency grid's rule) collapses the fit."""
+    row, stretch = grids.table_row(30, 1.0, 100.0)
+    assert stretch > 1.5
+    tau, _ = grids.minimax_time_grid(30, 1.0, 100.0)
+    assert np.allclose(tau, row[:30] * stretch)
+
+
+def test_more_points_never_worse():
+    err = {n: relative_laplace_error(n, 1.0, 100.0)
+           for n in grids.minimax_tau_supported_sizes()}
+    assert max(err[n] for n in err if n > 20) < 1e-9
+
+
+def test_gauss_legendre_integrates_lorentzian():
+    for w0 in (0.2, 0.5, 2.0):
+        x, w = grids.gauss_legendre_grid(40, w0)
+        assert abs(np.sum(w / (1 + x ** 2)) - np.pi / 2) < 1e-8
+
+
+def test_cosine_transform_fits_the_model_pair():
+    """sum_j C[k, j] e^{-x tau_j} ~ 2x/(x^2 + w_k^2) over [e_min, e_max]."""
+    e_min, e_max = 0.4, 40.0
+    nu, wt = grids.gauss_legendre_grid(10, 0.2)
+    for n, tol in ((14, 2e-3), (20, 1e-4)):
+        g = grids.TimeFrequencyGrid.minimax_split(n, e_min, e_max, nu, wt)
+        x = np.geomspace(e_min, e_max, 500)
+        model = np.exp(-np.outer(x, g.tau_points)) @ g.cosft_wt.T
+        ref = 2 * x[:, None] / (x[:, None] ** 2 + nu[None, :] ** 2)
+        assert np.abs(model - ref).max() / np.abs(ref).max() < tol, n
+
+
+def test_split_grid_halves_the_laplace_points():
+    """The time-frequency pair takes the Laplace grid scaled by 1/(2 e_min)
+    (time_frequency.py:524-536, GreenX minimax_grids.F90 `2 e_min`), and its
+    omega = 0 cosine row is then a Laplace fit of 2/x on [e_min, e_max]."""
+    e_min, e_max = 0.5, 50.0
+    g = grids.TimeFrequencyGrid.minimax_split(14, e_min, e_max, [0.0], [1.0])
+    tau, w = grids.minimax_time_grid(14, e_min, e_max)
+    assert np.allclose(g.tau_points, 0.5 * tau) and np.allclose(g.tau_weights, 0.5 * w)
+    x = np.geomspace(e_min, e_max, 1000)
+    fit = np.exp(-np.outer(x, g.tau_points)) @ g.cosft_wt[0]
+    assert np.abs(fit * x / 2 - 1).max() < 1e-3
+
+
+def test_auto_point_count_meets_target():
+    n, err = grids.minimax_points_for_accuracy(0.5, 5.0, 1e-8)
+    assert err <= 1e-8 and n >= 6
+    n2, _ = grids.minimax_points_for_accuracy(0.5, 50.0, 1e-8)
+    assert n2 >= n
+
+
+def test_refuses_bad_windows():
+    with pytest.raises(ValueError):
+        grids.minimax_time_grid(8, 0.0, 1.0)
+    with pytest.raises(ValueError):
+        grids.minimax_time_grid(7, 0.1, 1.0)
+    with pytest.raises(ValueError):
+        grids.gauss_legendre_grid(10, -1.0)
