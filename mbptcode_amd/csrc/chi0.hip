@@ -98,7 +98,7 @@ __global__ __launch_bounds__(256) void k_chi0(int M, int no, int nv,
     for (int fn = 0; fn < 2; ++fn) {
       const int col = tile_col(n0, fn);
       for (int r = 0; r < 4; ++r) {
-        const int row = tile_row(m0, fm, r);
+        const int row = tile_row<T>(m0, fm, r);
         if (row < M && col < M) {
           const double v = (double)chi[fm][fn][r];
           const long o = (long)row * M + col;

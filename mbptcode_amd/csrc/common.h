@@ -42,8 +42,10 @@ inline int launch_status(const char *what) {
 }
 
 // ---- MFMA traits -------------------------------------------------------------
-// v_mfma_{f32,f64}_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16];
-// the 16x16 result leaves D[4*(l/16)+r][l%16] in element r of lane l.
+// v_mfma_{f32,f64}_16x16x4: lane l supplies A[l%16][l/16] and B[l/16][l%16].
+// The 16x16 result's row for element r of lane l differs by type (measured on
+// gfx950): f32 keeps D[4*(l/16)+r][l%16], f64 interleaves D[l/16+4*r][l%16].
+// orow(l/16, r) gives that row.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef double f64x4 __attribute__((ext_vector_type(4)));
 
@@ -53,12 +55,14 @@ template <> struct MF<float> {
   static __device__ __forceinline__ acc mma(float a, float b, acc c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int orow(int lq, int r) { return 4 * lq + r; }
 };
 template <> struct MF<double> {
   typedef f64x4 acc;
   static __device__ __forceinline__ acc mma(double a, double b, acc c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int orow(int lq, int r) { return lq + 4 * r; }
 };
 
 template <typename A> __device__ __forceinline__ void zero(A &a) {
@@ -129,9 +133,9 @@ __device__ __forceinline__ void gemm64(const LA &la, const LB &lb, int m0, int n
 }
 
 // Output coordinates of fragment (fm, fn), element r, for this lane.
-__device__ __forceinline__ int tile_row(int m0, int fm, int r) {
+template <typename T> __device__ __forceinline__ int tile_row(int m0, int fm, int r) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  return m0 + 32 * (wave >> 1) + 16 * fm + 4 * (lane >> 4) + r;
+  return m0 + 32 * (wave >> 1) + 16 * fm + MF<T>::orow(lane >> 4, r);
 }
 __device__ __forceinline__ int tile_col(int n0, int fn) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;

@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_head(int rows, int M, int nv,
     for (int fn = 0; fn < 2; ++fn) {
       const int col = tile_col(tn * kTile, fn);
       for (int r = 0; r < 4; ++r) {
-        const int row = tile_row(tm * kTile, fm, r);
+        const int row = tile_row<T>(tm * kTile, fm, r);
         if (row < rows && col < M) zXv[(long)row * M + col] = acc[fm][fn][r];
       }
     }
@@ -179,9 +179,9 @@ __global__ __launch_bounds__(256) void k_hartree_u(int M, int k, const T *__rest
 //   P (16 mu x 16 nu)   = X_o[mu, :] zXv[b][:, nu]       NOPAD/4 MFMAs, K = n_o
 //   S_r = P_r * W[mu_r, nu]                              in registers
 //   T[:, nu] += X_o[mu, :]^T S                           NOPAD/16 x 4 MFMAs
-// The P result layout (row 4*(l/16)+r of lane l) is used directly as the B
-// operand of step r of the second product, whose k index is that same row,
-// so no shuffle is needed between the two MFMAs.
+// The P result layout (row orow(l/16, r) of lane l) is used directly as the B
+// operand of step r of the second product, whose k index is taken to be that
+// same row, so no shuffle is needed between the two MFMAs.
 template <typename T, int NOPAD>
 __global__ __launch_bounds__(256) void k_exchange(
     int M, int no, int k, const T *__restrict__ Xo, const T *__restrict__ W,
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(256) void k_exchange(
       T S[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int mu = mg + r0 + 4 * lq + r;
+        const int mu = mg + r0 + MF<T>::orow(lq, r);
         const T w = (mu < mu_end && colok) ? W[(long)mu * M + col] : T(0);
         S[r] = P[r] * w;
       }
@@ -237,8 +237,8 @@ __global__ __launch_bounds__(256) void k_exchange(
       for (int it = 0; it < NI; ++it)
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          acc[it] = MF<T>::mma(xs[(r0 + 4 * lq + r) * LD + 16 * it + lr], S[r],
-                               acc[it]);
+          acc[it] = MF<T>::mma(xs[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr],
+                               S[r], acc[it]);
     }
   }
   if (colok) {
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_exchange(
     for (int it = 0; it < NI; ++it)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        Tpart[(((long)s * k + b) * NOPAD + 16 * it + 4 * lq + r) * M + col] =
+        Tpart[(((long)s * k + b) * NOPAD + 16 * it + MF<T>::orow(lq, r)) * M + col] =
             acc[it][r];
   }
 }
@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_tail(int rows, int no, int nv, int M,
     for (int fn = 0; fn < 2; ++fn) {
       const int a = tile_col(tn * kTile, fn);
       for (int r = 0; r < 4; ++r) {
-        const int m = tile_row(tm * kTile, fm, r);
+        const int m = tile_row<T>(tm * kTile, fm, r);
         if (m < rows && a < nv) {
           const int i = m % no;
           const long o = (long)m * nv + a;
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_wsq(int M, int nv, const T *__restrict_
     for (int fn = 0; fn < 2; ++fn) {
       const int a = tile_col(tn * kTile, fn);
       for (int r = 0; r < 4; ++r) {
-        const int mu = tile_row(tm * kTile, fm, r);
+        const int mu = tile_row<T>(tm * kTile, fm, r);
         if (mu < M && a < nv) Q[(long)mu * nv + a] = acc[fm][fn][r];
       }
     }
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void k_sdiag(int M, int no, int nv,
     for (int fn = 0; fn < 2; ++fn) {
       const int a = tile_col(tn * kTile, fn);
       for (int r = 0; r < 4; ++r) {
-        const int i = tile_row(tm * kTile, fm, r);
+        const int i = tile_row<T>(tm * kTile, fm, r);
         if (i < no && a < nv)
           out[(long)i * nv + a] = d[(long)i * nv + a] - (double)acc[fm][fn][r];
       }
