@@ -156,11 +156,17 @@ class SigmaOperator:
                 'launches': int(n.value), 'exchange_flops': float(fl.value)}
 
     def screened_diagonal_device(self):
+        """d - (ii|W|aa), (nocc, nvir) FP64 on the device; formed once per
+        operator (W and X do not change) and kept."""
+        cached = getattr(self, '_sdiag', None)
+        if cached is not None:
+            return cached
         out = torch.empty((self.nocc, self.nvir), dtype=torch.float64,
                           device=self.device)
         _lib.check(self._lib.mbpt_sigma_screened_diag(self._h, _lib.dptr(out),
                                                        _lib.stream_ptr()),
                    'mbpt_sigma_screened_diag')
+        self._sdiag = out
         return out
 
 

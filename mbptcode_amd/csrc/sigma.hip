@@ -87,23 +87,6 @@ template <typename T> struct LdAsq {  // A(m, k) = P[m][k]^2
     return x * x;
   }
 };
-// A(m = (b, i), k = mu) = sum_s T_s[b][i][mu] - kappa X_o^T[i][mu] u[b][mu]
-template <typename T> struct LdTail {
-  const T *Tp, *XoT, *u;
-  T kappa;
-  int no, rows, M, nopad, nsplit;
-  long sstride;
-  __device__ T operator()(int m, int kk) const {
-    if (m >= rows || kk >= M) return T(0);
-    const int b = m / no, i = m - b * no;
-    const long off = ((long)b * nopad + i) * M + kk;
-    T s = Tp[off];
-    for (int q = 1; q < nsplit; ++q) s += Tp[off + q * sstride];
-    if (u) s -= kappa * XoT[(long)i * M + kk] * u[(long)b * M + kk];
-    return s;
-  }
-};
-
 // ---- head: zXv = z X_v^T ---------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_head(int rows, int M, int nv,
@@ -143,29 +126,30 @@ __global__ void k_hartree_p(int M, int no, int k, const T *__restrict__ XoT,
   p[idx] = s;
 }
 
-// u[b][mu] = sum_nu V[mu][nu] p[b][nu]: one wave per grid row, 8 vectors at a
-// time, lanes striding the row (V is read once per 8 vectors, coalesced).
+// u[b][mu] = sum_nu V[mu][nu] p[b][nu]: one wave per grid row, 16 vectors at a
+// time, lanes striding the row (V is read once per 16 vectors, coalesced).
 template <typename T>
 __global__ __launch_bounds__(256) void k_hartree_u(int M, int k, const T *__restrict__ V,
                                                    const T *__restrict__ p,
                                                    T *__restrict__ u) {
+  constexpr int NB = 16;
   const int lane = threadIdx.x & 63;
   const int mu = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (mu >= M) return;
   const T *row = V + (long)mu * M;
-  for (int b0 = 0; b0 < k; b0 += 8) {
-    const int nb = min(8, k - b0);
-    T acc[8];
+  for (int b0 = 0; b0 < k; b0 += NB) {
+    const int nb = min(NB, k - b0);
+    T acc[NB];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = 0;
+    for (int q = 0; q < NB; ++q) acc[q] = 0;
     for (int nu = lane; nu < M; nu += 64) {
       const T v = row[nu];
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
+      for (int q = 0; q < NB; ++q)
         if (q < nb) acc[q] += v * p[(long)(b0 + q) * M + nu];
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < NB; ++q) {
       T s = acc[q];
       for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
       if (lane == 0 && q < nb) u[(long)(b0 + q) * M + mu] = s;
@@ -251,35 +235,69 @@ __global__ __launch_bounds__(256) void k_exchange(
   }
 }
 
-// ---- tail: Az = d o z - T' X_v ------------------------------------------------------
+// ---- tail: Az = d o z - T' X_v, T' = sum_s T_s - kappa X_o^T o u -------------------
+// prep: the split partials summed and the Hartree term folded in, one coalesced
+// pass writing T' as (k nocc, M);  tail: T' X_v with the grid index split in
+// nq chunks (enough workgroups for the skinny output);  finish: d o z minus
+// the chunk partials, in FP64.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tail(int rows, int no, int nv, int M,
-                                              LdTail<T> la, const T *__restrict__ Xv,
-                                              const double *__restrict__ d,
-                                              const double *__restrict__ z,
-                                              double *__restrict__ Az, int ntm,
-                                              int ntn, int nblk) {
+__global__ void k_tail_prep(int k, int no, int nopad, int M, int nsplit,
+                            const T *__restrict__ Tp, const T *__restrict__ XoT,
+                            const T *__restrict__ u, T kappa, T *__restrict__ Tc) {
+  const long n = (long)k * no * M;
+  const long sstride = (long)k * nopad * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int m = (int)(e / M), mu = (int)(e % M);
+    const int b = m / no, i = m - b * no;
+    const long off = ((long)b * nopad + i) * M + mu;
+    T s = Tp[off];
+    for (int q = 1; q < nsplit; ++q) s += Tp[off + q * sstride];
+    if (u) s -= kappa * XoT[(long)i * M + mu] * u[(long)b * M + mu];
+    Tc[e] = s;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_tail(int rows, int nv, int M, int kchunk,
+                                              const T *__restrict__ Tc,
+                                              const T *__restrict__ Xv,
+                                              T *__restrict__ Ep, int ntm, int ntn,
+                                              int nq, int nblk) {
   const int lid = xcd_logical(blockIdx.x, nblk);
-  if (lid >= ntm * ntn) return;
-  const int tm = lid / ntn, tn = lid % ntn;
+  if (lid >= ntm * ntn * nq) return;
+  const int tn = lid % ntn, tm = (lid / ntn) % ntm, q = lid / (ntn * ntm);
+  const int k0 = q * kchunk, kc = min(kchunk, M - k0);
   __shared__ TileSmem<T> sm;
   typename MF<T>::acc acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) zero(acc[i][j]);
-  gemm64<T, true, false>(la, LdBkn<T>{Xv, nv, M, nv}, tm * kTile, tn * kTile, M,
-                         acc, sm);
+  gemm64<T, true, false>(LdArm<T>{Tc + k0, M, rows, kc},
+                         LdBkn<T>{Xv + (long)k0 * nv, nv, kc, nv}, tm * kTile,
+                         tn * kTile, kc, acc, sm);
+  T *out = Ep + (long)q * rows * nv;
   for (int fm = 0; fm < 2; ++fm)
     for (int fn = 0; fn < 2; ++fn) {
       const int a = tile_col(tn * kTile, fn);
       for (int r = 0; r < 4; ++r) {
         const int m = tile_row<T>(tm * kTile, fm, r);
-        if (m < rows && a < nv) {
-          const int i = m % no;
-          const long o = (long)m * nv + a;
-          Az[o] = d[(long)i * nv + a] * z[o] - (double)acc[fm][fn][r];
-        }
+        if (m < rows && a < nv) out[(long)m * nv + a] = acc[fm][fn][r];
       }
     }
+}
+
+template <typename T>
+__global__ void k_tail_finish(int rows, int no, int nv, int nq, const T *__restrict__ Ep,
+                              const double *__restrict__ d, const double *__restrict__ z,
+                              double *__restrict__ Az) {
+  const long n = (long)rows * nv;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int m = (int)(e / nv), a = (int)(e % nv), i = m % no;
+    T s = Ep[e];
+    for (int q = 1; q < nq; ++q) s += Ep[(long)q * n + e];
+    Az[e] = d[(long)i * nv + a] * z[e] - (double)s;
+  }
 }
 
 // ---- screened diagonal: d - (X_o o X_o)^T W (X_v o X_v) -------------------------------
@@ -347,8 +365,9 @@ struct mbpt_sigma {
   size_t esz;              // bytes per working-precision element
   void *Xo, *Xv, *XoT, *W, *V;
   double *d;
-  void *zXv, *p, *u, *Tpart;
-  long cap_k, cap_tp;      // vectors / (split x vector) slots the workspace holds
+  // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
+  void *ws[6];
+  size_t ws_cap[6];        // bytes
   size_t held;
   int prof;                // stage timing on
   hipEvent_t ev[5];
@@ -363,10 +382,14 @@ constexpr int kTargetBlocks = 2048;
 
 int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
 
+enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_N };
+
 void free_ws(mbpt_sigma *h) {
-  (void)hipFree(h->zXv); (void)hipFree(h->p); (void)hipFree(h->u); (void)hipFree(h->Tpart);
-  h->zXv = h->p = h->u = h->Tpart = nullptr;
-  h->cap_k = h->cap_tp = 0;
+  for (int q = 0; q < WS_N; ++q) {
+    (void)hipFree(h->ws[q]);
+    h->ws[q] = nullptr;
+    h->ws_cap[q] = 0;
+  }
 }
 
 int nsplit_for(const mbpt_sigma *h, int k) {
@@ -376,19 +399,33 @@ int nsplit_for(const mbpt_sigma *h, int k) {
   return ns;
 }
 
-int ensure_ws(mbpt_sigma *h, int k, int nsplit) {
-  const long need_tp = (long)k * nsplit;
-  if (k <= h->cap_k && need_tp <= h->cap_tp) return MBPT_OK;
-  const long ck = std::max<long>(h->cap_k, ((long)k + 7) / 8 * 8);
-  const long ctp = std::max<long>(h->cap_tp, std::max<long>(need_tp, ck));
-  free_ws(h);
-  const size_t e = h->esz;
-  MBPT_HIP_TRY(hipMalloc(&h->zXv, e * ck * h->no * h->M));
-  MBPT_HIP_TRY(hipMalloc(&h->p, e * ck * h->M));
-  MBPT_HIP_TRY(hipMalloc(&h->u, e * ck * h->M));
-  MBPT_HIP_TRY(hipMalloc(&h->Tpart, e * ctp * h->nopad * h->M));
-  h->cap_k = ck;
-  h->cap_tp = ctp;
+// grid-index chunks of the tail GEMM: enough workgroups to fill the chip
+int nq_for(int rows, int nv, int M) {
+  const int tiles = ((rows + 63) / 64) * ((nv + 63) / 64);
+  int nq = (512 + tiles - 1) / tiles;
+  return std::max(1, std::min(nq, (M + 63) / 64));
+}
+
+int ensure_buf(mbpt_sigma *h, int q, size_t bytes) {
+  if (bytes <= h->ws_cap[q]) return MBPT_OK;
+  (void)hipFree(h->ws[q]);
+  h->ws[q] = nullptr;
+  h->ws_cap[q] = 0;
+  const size_t grow = std::max(bytes, h->ws_cap[q] + h->ws_cap[q] / 4);
+  MBPT_HIP_TRY(hipMalloc(&h->ws[q], grow));
+  h->ws_cap[q] = grow;
+  return MBPT_OK;
+}
+
+int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq) {
+  const size_t e = h->esz, M = h->M, no = h->no, nv = h->nv;
+  const size_t sizes[WS_N] = {e * k * no * M, e * k * M, e * k * M,
+                              e * (size_t)nsplit * k * h->nopad * M, e * k * no * M,
+                              e * (size_t)nq * k * no * nv};
+  for (int q = 0; q < WS_N; ++q) {
+    int rc = ensure_buf(h, q, sizes[q]);
+    if (rc) return rc;
+  }
   return MBPT_OK;
 }
 
@@ -413,36 +450,38 @@ void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_
   const int nvb = (h->M + 63) / 64;
   const int nblk = pad_grid((long)nvb * k * nsplit);
   k_exchange<T, NOPAD><<<nblk, 256, 0, st>>>(h->M, h->no, k, (const T *)h->Xo,
-                                             (const T *)h->W, (const T *)h->zXv,
-                                             (T *)h->Tpart, nsplit, rps, nvb, nblk);
+                                             (const T *)h->W, (const T *)h->ws[WS_ZXV],
+                                             (T *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
 
 template <typename T>
 int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st) {
   const int M = h->M, no = h->no, nv = h->nv;
   const int nsplit = nsplit_for(h, k);
-  int rc = ensure_ws(h, k, nsplit);
+  const int rows = k * no;
+  const int nq = nq_for(rows, nv, M);
+  int rc = ensure_ws(h, k, nsplit, nq);
   if (rc) return rc;
+  T *zXv = (T *)h->ws[WS_ZXV], *p = (T *)h->ws[WS_P], *u = (T *)h->ws[WS_U];
+  T *Tpart = (T *)h->ws[WS_TPART], *Tc = (T *)h->ws[WS_TC], *Ep = (T *)h->ws[WS_EP];
   // rows per split, a multiple of the 64-row LDS stage
   const int rps = (((M + nsplit - 1) / nsplit) + 63) / 64 * 64;
   const int nsplit_eff = (M + rps - 1) / rps;
-  const int rows = k * no;
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[0], st));
 
   {  // head
     const int ntm = (rows + 63) / 64, ntn = (M + 63) / 64;
     const int nblk = pad_grid((long)ntm * ntn);
-    k_head<T><<<nblk, 256, 0, st>>>(rows, M, nv, z, (const T *)h->Xv, (T *)h->zXv,
-                                    ntm, ntn, nblk);
+    k_head<T><<<nblk, 256, 0, st>>>(rows, M, nv, z, (const T *)h->Xv, zXv, ntm, ntn,
+                                    nblk);
     if ((rc = launch_status("k_head"))) return rc;
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[1], st));
   if (h->kappa != 0.0) {  // hartree
     k_hartree_p<T><<<grid1((long)k * M), 256, 0, st>>>(M, no, k, (const T *)h->XoT,
-                                                        (const T *)h->zXv, (T *)h->p);
+                                                        zXv, p);
     if ((rc = launch_status("k_hartree_p"))) return rc;
-    k_hartree_u<T><<<(M + 3) / 4, 256, 0, st>>>(M, k, (const T *)h->V,
-                                                (const T *)h->p, (T *)h->u);
+    k_hartree_u<T><<<(M + 3) / 4, 256, 0, st>>>(M, k, (const T *)h->V, p, u);
     if ((rc = launch_status("k_hartree_u"))) return rc;
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
@@ -460,14 +499,20 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   if ((rc = launch_status("k_exchange"))) return rc;
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
   {  // tail
-    LdTail<T> la{(const T *)h->Tpart, (const T *)h->XoT,
-                 h->kappa != 0.0 ? (const T *)h->u : nullptr, (T)h->kappa, no, rows,
-                 M, h->nopad, nsplit_eff, (long)k * h->nopad * M};
+    k_tail_prep<T><<<grid1((long)rows * M), 256, 0, st>>>(
+        k, no, h->nopad, M, nsplit_eff, Tpart, (const T *)h->XoT,
+        h->kappa != 0.0 ? u : nullptr, (T)h->kappa, Tc);
+    if ((rc = launch_status("k_tail_prep"))) return rc;
     const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
-    const int nblk = pad_grid((long)ntm * ntn);
-    k_tail<T><<<nblk, 256, 0, st>>>(rows, no, nv, M, la, (const T *)h->Xv, h->d, z, Az,
-                                    ntm, ntn, nblk);
+    const int kchunk = ((M + nq - 1) / nq + 15) / 16 * 16;
+    const int nq_eff = (M + kchunk - 1) / kchunk;
+    const int nblk = pad_grid((long)ntm * ntn * nq_eff);
+    k_tail<T><<<nblk, 256, 0, st>>>(rows, nv, M, kchunk, Tc, (const T *)h->Xv, Ep, ntm,
+                                    ntn, nq_eff, nblk);
     if ((rc = launch_status("k_tail"))) return rc;
+    k_tail_finish<T><<<grid1((long)rows * nv), 256, 0, st>>>(rows, no, nv, nq_eff, Ep,
+                                                              h->d, z, Az);
+    if ((rc = launch_status("k_tail_finish"))) return rc;
   }
   if (h->prof) {
     MBPT_HIP_TRY(hipEventRecord(h->ev[4], st));
@@ -591,8 +636,9 @@ int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream) {
 
 size_t mbpt_sigma_held_bytes(const mbpt_sigma *h) {
   if (!h) return 0;
-  return h->held + h->esz * ((size_t)h->cap_k * (h->no + 2) * h->M +
-                             (size_t)h->cap_tp * h->nopad * h->M);
+  size_t ws = 0;
+  for (int q = 0; q < WS_N; ++q) ws += h->ws_cap[q];
+  return h->held + ws;
 }
 
 int mbpt_sigma_profile(mbpt_sigma *h, int enable) {

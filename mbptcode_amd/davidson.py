@@ -120,27 +120,24 @@ def jacobi_davidson_correction(R, U, diag, theta, floor=DAVIDSON_DENOMINATOR_FLO
 # ---------------------------------------------------------------------------
 
 def _orthonormal_new(T, V, drop):
-    """T projected off V (twice) and orthonormalised among itself through the
-    eigendecomposition of its Gram matrix; directions whose norm after the
-    projection is below `drop` (relative to their norm before it) are
-    discarded.  FP64 throughout."""
+    """T projected off the orthonormal columns V (twice) and orthonormalised
+    among itself by the eigendecomposition of its Gram matrix (solved on the
+    host: the matrix is at most a few dozen wide).  Columns whose norm after
+    the projection is below `drop` relative to before are discarded.  FP64."""
     T = T / T.norm(dim=0).clamp_min(1e-300)
     if V is not None and V.shape[1]:
         T = T - V @ (V.T @ T)
         T = T - V @ (V.T @ T)
-    nrm = T.norm(dim=0)
-    keep = nrm > drop
-    if not bool(keep.any()):
+    G = (T.T @ T).cpu().numpy()
+    keep = np.flatnonzero(np.sqrt(np.clip(np.diag(G), 0, None)) > drop)
+    if keep.size == 0:
         return T[:, :0]
-    T = T[:, keep] / nrm[keep]
-    G = T.T @ T
-    s, U = torch.linalg.eigh(G)
+    G = G[np.ix_(keep, keep)]
+    s, U = np.linalg.eigh(G)
     good = s > 1e-10 * s.max()
-    T = T @ (U[:, good] / s[good].sqrt())
-    if V is not None and V.shape[1]:
-        T = T - V @ (V.T @ T)
-    T = T / T.norm(dim=0)
-    return T
+    C = np.zeros((T.shape[1], int(good.sum())))
+    C[keep] = U[:, good] / np.sqrt(s[good])
+    return T @ torch.as_tensor(C, device=T.device)
 
 
 class DavidsonResult:
@@ -164,67 +161,126 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     `max_space` it collapses onto the current Ritz vectors.  `on_stall` is
     called when the largest unconverged residual has not fallen by 10x over
     `stall_cycles` cycles (the learned preconditioner's fallback hook).
+
+    The trial space V and its image AV live in preallocated device buffers;
+    the Rayleigh matrix H = V^T A V and G = (AV)^T AV grow block by block on
+    the device, and one (m, 2m) transfer per cycle gives the host both the
+    Ritz pairs (eigh of H) and the residual norms,
+    |A x - theta x|^2 = y^T G y - theta^2 for x = V y, |x| = 1.
+    The returned residual norms are recomputed explicitly from A X - X theta.
     """
+    dev = diag.device
     n = diag.numel()
     nroots = max(1, min(int(nroots), n))
     if max_space is None:
         max_space = max(10 * nroots + 30, 60)
-    max_space = int(min(max_space, n))
+    max_space = int(min(max(max_space, 2 * nroots), n))
     lindep = lindep_for(tol) if lindep is None else lindep
     drop = float(np.sqrt(lindep))
     if precond is None:
         def precond(R, U, theta, cycle):
             return jacobi_davidson_correction(R, U, diag, theta)
     t0 = time.time()
-    V = _orthonormal_new(x0, None, 1e-12)
-    AV = matmat(V)
-    vectors = V.shape[1]
+    T0 = _orthonormal_new(x0, None, 1e-12)
+    cap = max_space + max(T0.shape[1], nroots)
+    Vb = torch.empty((n, cap), dtype=torch.float64, device=dev)
+    AVb = torch.empty_like(Vb)
+    Hb = torch.zeros((cap, cap), dtype=torch.float64, device=dev)
+    Gb = torch.zeros_like(Hb)
+    m = 0
+
+    def append(T, AT):
+        nonlocal m
+        p = T.shape[1]
+        Vb[:, m:m + p] = T
+        AVb[:, m:m + p] = AT
+        blkH = Vb[:, :m + p].T @ AT
+        blkG = AVb[:, :m + p].T @ AT
+        Hb[:m + p, m:m + p] = blkH
+        Hb[m:m + p, :m] = blkH[:m].T
+        Gb[:m + p, m:m + p] = blkG
+        Gb[m:m + p, :m] = blkG[:m].T
+        m += p
+
+    append(T0, matmat(T0))
+    vectors = m
     trace = []
     conv = np.zeros(nroots, bool)
     w = rn = None
-    X = AX = None
-    best_hist = []
+    hist = []
     stalled = False
     cycles = 0
+    y = None
     for cycle in range(max_cycle):
         cycles = cycle + 1
-        H = V.T @ AV
-        H = 0.5 * (H + H.T)
-        Hh = H.cpu().numpy()
-        evals, evecs = np.linalg.eigh(Hh)
-        k = min(nroots, len(evals))
-        y = torch.as_tensor(evecs[:, :k], device=V.device)
-        w = evals[:k]
-        X = V @ y
-        AX = AV @ y
-        wt = torch.as_tensor(w, device=V.device)
-        R = AX - X * wt[None, :]
-        rn = R.norm(dim=0).cpu().numpy()
+        HG = torch.cat([Hb[:m, :m], Gb[:m, :m]], dim=1).cpu().numpy()
+        H = 0.5 * (HG[:, :m] + HG[:, :m].T)
+        G = 0.5 * (HG[:, m:] + HG[:, m:].T)
+        evals, evecs = np.linalg.eigh(H)
+        k = min(nroots, m)
+        w, y = evals[:k], evecs[:, :k]
+        y_rows = m
+        sq = np.einsum('ik,ij,jk->k', y, G, y)
+        rn = np.sqrt(np.clip(sq - w ** 2, 0.0, None))
+        # y^T G y - theta^2 cancels: below ~sqrt(1e3 eps) |theta| it is noise,
+        # so residuals that small (and a tolerance that asks for them) are
+        # recomputed explicitly from A x - theta x.
+        floor = np.sqrt(1e3 * np.finfo(float).eps * np.maximum(sq, w ** 2))
+        fuzzy = np.flatnonzero(rn < 10 * floor)
+        if fuzzy.size and tol < 10 * floor.max():
+            yf = torch.as_tensor(y[:, fuzzy], device=dev)
+            wf = torch.as_tensor(w[fuzzy], device=dev)
+            rn[fuzzy] = (AVb[:, :m] @ yf - (Vb[:, :m] @ yf) * wf[None, :]).norm(dim=0).cpu().numpy()
         conv = rn <= tol
-        trace.append({'cycle': cycle, 'space': int(V.shape[1]),
-                      'max_residual': float(rn.max()), 'converged': int(conv.sum())})
+        trace.append({'cycle': cycle, 'space': int(m), 'max_residual': float(rn.max()),
+                      'converged': int(conv.sum())})
         if conv.all():
             break
         worst = float(rn[~conv].max())
-        best_hist.append(worst)
-        if stall_cycles and len(best_hist) > stall_cycles:
-            if worst > 0.1 * best_hist[-1 - stall_cycles] and on_stall is not None:
+        hist.append(worst)
+        if stall_cycles and on_stall is not None and len(hist) > stall_cycles:
+            if worst > 0.1 * hist[-1 - stall_cycles]:
                 stalled = True
                 new_pre = on_stall(cycle)
                 if new_pre is not None:
                     precond = new_pre
-                best_hist = []
-        idx = torch.as_tensor(np.flatnonzero(~conv), device=V.device)
-        T = precond(R[:, idx], X[:, idx], wt[idx], cycle)
-        if V.shape[1] + T.shape[1] > max_space:
-            V, AV = X, AX
-        T = _orthonormal_new(T, V, drop)
+                hist = []
+        idx = np.flatnonzero(~conv)
+        yu = torch.as_tensor(y[:, idx], device=dev)
+        Xu = Vb[:, :m] @ yu
+        AXu = AVb[:, :m] @ yu
+        wu = torch.as_tensor(w[idx], device=dev)
+        T = precond(AXu - Xu * wu[None, :], Xu, wu, cycle)
+        T = _orthonormal_new(T, Vb[:, :m], drop)
         if T.shape[1] == 0:
             break
+        if m + T.shape[1] > max_space:
+            # collapse onto the tracked Ritz vectors (T is already orthogonal to them)
+            yt = torch.as_tensor(y, device=dev)
+            X = Vb[:, :m] @ yt
+            AX = AVb[:, :m] @ yt
+            Hc = torch.as_tensor(y.T @ H @ y, device=dev)
+            Gc = torch.as_tensor(y.T @ G @ y, device=dev)
+            Vb[:, :k] = X
+            AVb[:, :k] = AX
+            Hb[:k, :k] = Hc
+            Gb[:k, :k] = Gc
+            m = k
         AT = matmat(T)
         vectors += T.shape[1]
-        V = torch.cat([V, T], dim=1)
-        AV = torch.cat([AV, AT], dim=1)
+        append(T, AT)
+    if y_rows != m:
+        # the space grew after the last Rayleigh-Ritz step (cycle budget ran out)
+        H = Hb[:m, :m].cpu().numpy()
+        evals, evecs = np.linalg.eigh(0.5 * (H + H.T))
+        k = min(nroots, m)
+        w, y = evals[:k], evecs[:, :k]
+    yt = torch.as_tensor(y, device=dev)
+    X = Vb[:, :m] @ yt
+    AX = AVb[:, :m] @ yt
+    wt = torch.as_tensor(w, device=dev)
+    rn = (AX - X * wt[None, :]).norm(dim=0).cpu().numpy()
+    conv = rn <= tol
     return DavidsonResult(e=np.asarray(w), X=X, conv=conv, rnorm=rn, cycles=cycles,
                           vectors=vectors, seconds=time.time() - t0, trace=trace,
                           stalled=stalled)
