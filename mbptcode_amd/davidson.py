@@ -194,10 +194,12 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         p = T.shape[1]
         Vb[:, m:m + p] = T
         AVb[:, m:m + p] = AT
-        blkH = Vb[:, :m + p].T @ AT
+        # Hb holds V^T (A V) whole: a lower-precision operator is not exactly
+        # symmetric, and x^T A x must be the one the residual sees.
+        Hb[:m + p, m:m + p] = Vb[:, :m + p].T @ AT
+        if m:
+            Hb[m:m + p, :m] = T.T @ AVb[:, :m]
         blkG = AVb[:, :m + p].T @ AT
-        Hb[:m + p, m:m + p] = blkH
-        Hb[m:m + p, :m] = blkH[:m].T
         Gb[:m + p, m:m + p] = blkG
         Gb[m:m + p, :m] = blkG[:m].T
         m += p
@@ -214,14 +216,18 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     for cycle in range(max_cycle):
         cycles = cycle + 1
         HG = torch.cat([Hb[:m, :m], Gb[:m, :m]], dim=1).cpu().numpy()
-        H = 0.5 * (HG[:, :m] + HG[:, :m].T)
+        Hns = HG[:, :m]
+        H = 0.5 * (Hns + Hns.T)
         G = 0.5 * (HG[:, m:] + HG[:, m:].T)
         evals, evecs = np.linalg.eigh(H)
         k = min(nroots, m)
         w, y = evals[:k], evecs[:, :k]
         y_rows = m
         sq = np.einsum('ik,ij,jk->k', y, G, y)
-        rn = np.sqrt(np.clip(sq - w ** 2, 0.0, None))
+        xax = np.einsum('ik,ij,jk->k', y, Hns, y)
+        # |A x - w x|^2 = |A x|^2 - 2 w x^T A x + w^2 (|x| = 1), exact for a
+        # non-symmetric A as well
+        rn = np.sqrt(np.clip(sq - 2.0 * w * xax + w ** 2, 0.0, None))
         # y^T G y - theta^2 cancels: below ~sqrt(1e3 eps) |theta| it is noise,
         # so residuals that small (and a tolerance that asks for them) are
         # recomputed explicitly from A x - theta x.
@@ -259,7 +265,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             yt = torch.as_tensor(y, device=dev)
             X = Vb[:, :m] @ yt
             AX = AVb[:, :m] @ yt
-            Hc = torch.as_tensor(y.T @ H @ y, device=dev)
+            Hc = torch.as_tensor(y.T @ Hns @ y, device=dev)
             Gc = torch.as_tensor(y.T @ G @ y, device=dev)
             Vb[:, :k] = X
             AVb[:, :k] = AX
