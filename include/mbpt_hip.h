@@ -128,6 +128,21 @@ int mbpt_chi0_profile_read(const mbpt_chi0 *h, double *ms, long *launches,
 
 int mbpt_chi0_destroy(mbpt_chi0 *h);
 
+/* ------------------------------------------------------------------------
+ * Davidson subspace reductions (FP64)
+ *
+ * C (a, b) = A^T B for A (n, a) and B (n, b) with row strides lda, ldb (so
+ * column slices of a wider trial-space buffer pass directly), C row stride
+ * ldc.  The products of the Rayleigh-Ritz step and the orthogonalisation of
+ * the trial space, V^T (A V), (A V)^T (A V), V^T T (the reference runs them
+ * inside pyscf's lib.davidson1, called at src/Solvers/davidson.py:277).
+ * work: mbpt_gemm_tn_f64_work(n, a, b) doubles of device scratch.  n > 0.
+ * ------------------------------------------------------------------------ */
+long mbpt_gemm_tn_f64_work(int n, int a, int b);
+int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda,
+                     const double *B, int ldb, double *C, int ldc,
+                     double *work, long work_elems, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,7 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_sigma_held_bytes', 'mbpt_sigma_profile',
            'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
            'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
-           'mbpt_chi0_destroy')
+           'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64')
 
 _lib = None
 
@@ -46,6 +46,9 @@ _SIGNATURES = {
     'mbpt_chi0_profile': (_i, [_vp, _i]),
     'mbpt_chi0_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
     'mbpt_chi0_destroy': (_i, [_vp]),
+    'mbpt_gemm_tn_f64_work': (ctypes.c_long, [_i, _i, _i]),
+    'mbpt_gemm_tn_f64': (_i, [_i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, ctypes.c_long,
+                              _vp]),
 }
 
 

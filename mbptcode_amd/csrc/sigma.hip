@@ -126,37 +126,6 @@ __global__ void k_hartree_p(int M, int no, int k, const T *__restrict__ XoT,
   p[idx] = s;
 }
 
-// u[b][mu] = sum_nu V[mu][nu] p[b][nu]: one wave per grid row, 16 vectors at a
-// time, lanes striding the row (V is read once per 16 vectors, coalesced).
-template <typename T>
-__global__ __launch_bounds__(256) void k_hartree_u(int M, int k, const T *__restrict__ V,
-                                                   const T *__restrict__ p,
-                                                   T *__restrict__ u) {
-  constexpr int NB = 16;
-  const int lane = threadIdx.x & 63;
-  const int mu = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (mu >= M) return;
-  const T *row = V + (long)mu * M;
-  for (int b0 = 0; b0 < k; b0 += NB) {
-    const int nb = min(NB, k - b0);
-    T acc[NB];
-#pragma unroll
-    for (int q = 0; q < NB; ++q) acc[q] = 0;
-    for (int nu = lane; nu < M; nu += 64) {
-      const T v = row[nu];
-#pragma unroll
-      for (int q = 0; q < NB; ++q)
-        if (q < nb) acc[q] += v * p[(long)(b0 + q) * M + nu];
-    }
-#pragma unroll
-    for (int q = 0; q < NB; ++q) {
-      T s = acc[q];
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-      if (lane == 0 && q < nb) u[(long)(b0 + q) * M + mu] = s;
-    }
-  }
-}
-
 // ---- exchange: T[b] = X_o^T (W o (X_o zXv[b])) ------------------------------------
 // Workgroup = (64 grid columns nu, one vector b, one split s of the grid rows mu).
 // Wave w owns columns nu0 + 16w .. +16.  Per 16 rows of mu:
@@ -243,7 +212,8 @@ __global__ __launch_bounds__(256) void k_exchange(
 template <typename T>
 __global__ void k_tail_prep(int k, int no, int nopad, int M, int nsplit,
                             const T *__restrict__ Tp, const T *__restrict__ XoT,
-                            const T *__restrict__ u, T kappa, T *__restrict__ Tc) {
+                            const T *__restrict__ u, int nqu, T kappa,
+                            T *__restrict__ Tc) {
   const long n = (long)k * no * M;
   const long sstride = (long)k * nopad * M;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
@@ -253,35 +223,42 @@ __global__ void k_tail_prep(int k, int no, int nopad, int M, int nsplit,
     const long off = ((long)b * nopad + i) * M + mu;
     T s = Tp[off];
     for (int q = 1; q < nsplit; ++q) s += Tp[off + q * sstride];
-    if (u) s -= kappa * XoT[(long)i * M + mu] * u[(long)b * M + mu];
+    if (u) {
+      T ub = u[(long)b * M + mu];
+      for (int q = 1; q < nqu; ++q) ub += u[((long)q * k + b) * M + mu];
+      s -= kappa * XoT[(long)i * M + mu] * ub;
+    }
     Tc[e] = s;
   }
 }
 
+// C_q (rows x cols) = A[:, K-chunk q] B[K-chunk q, :] for q < nq, A row-major
+// (lda), B row-major (ldb): the skinny products of the tail (T' X_v, K = M)
+// and of the Hartree term (p V, K = M) with the contraction split over enough
+// workgroups to fill the chip; the chunks are summed by the consumer.
 template <typename T>
-__global__ __launch_bounds__(256) void k_tail(int rows, int nv, int M, int kchunk,
-                                              const T *__restrict__ Tc,
-                                              const T *__restrict__ Xv,
-                                              T *__restrict__ Ep, int ntm, int ntn,
-                                              int nq, int nblk) {
+__global__ __launch_bounds__(256) void k_gemm_nn_splitk(
+    int rows, int cols, int K, int kchunk, const T *__restrict__ A, int lda,
+    const T *__restrict__ B, int ldb, T *__restrict__ C, int ntm, int ntn, int nq,
+    int nblk) {
   const int lid = xcd_logical(blockIdx.x, nblk);
   if (lid >= ntm * ntn * nq) return;
   const int tn = lid % ntn, tm = (lid / ntn) % ntm, q = lid / (ntn * ntm);
-  const int k0 = q * kchunk, kc = min(kchunk, M - k0);
+  const int k0 = q * kchunk, kc = min(kchunk, K - k0);
   __shared__ TileSmem<T> sm;
   typename MF<T>::acc acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) zero(acc[i][j]);
-  gemm64<T, true, false>(LdArm<T>{Tc + k0, M, rows, kc},
-                         LdBkn<T>{Xv + (long)k0 * nv, nv, kc, nv}, tm * kTile,
+  gemm64<T, true, false>(LdArm<T>{A + k0, lda, rows, kc},
+                         LdBkn<T>{B + (long)k0 * ldb, ldb, kc, cols}, tm * kTile,
                          tn * kTile, kc, acc, sm);
-  T *out = Ep + (long)q * rows * nv;
+  T *out = C + (long)q * rows * cols;
   for (int fm = 0; fm < 2; ++fm)
     for (int fn = 0; fn < 2; ++fn) {
-      const int a = tile_col(tn * kTile, fn);
+      const int c = tile_col(tn * kTile, fn);
       for (int r = 0; r < 4; ++r) {
         const int m = tile_row<T>(tm * kTile, fm, r);
-        if (m < rows && a < nv) out[(long)m * nv + a] = acc[fm][fn][r];
+        if (m < rows && c < cols) out[(long)m * cols + c] = acc[fm][fn][r];
       }
     }
 }
@@ -417,9 +394,9 @@ int ensure_buf(mbpt_sigma *h, int q, size_t bytes) {
   return MBPT_OK;
 }
 
-int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq) {
+int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu) {
   const size_t e = h->esz, M = h->M, no = h->no, nv = h->nv;
-  const size_t sizes[WS_N] = {e * k * no * M, e * k * M, e * k * M,
+  const size_t sizes[WS_N] = {e * k * no * M, e * k * M, e * (size_t)nqu * k * M,
                               e * (size_t)nsplit * k * h->nopad * M, e * k * no * M,
                               e * (size_t)nq * k * no * nv};
   for (int q = 0; q < WS_N; ++q) {
@@ -460,7 +437,10 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   const int nsplit = nsplit_for(h, k);
   const int rows = k * no;
   const int nq = nq_for(rows, nv, M);
-  int rc = ensure_ws(h, k, nsplit, nq);
+  // Hartree u = p V: K = M split in nqu chunks of kchu (multiples of 16)
+  const int kchu = ((M + nq_for(k, M, M) - 1) / nq_for(k, M, M) + 15) / 16 * 16;
+  const int nqu = (M + kchu - 1) / kchu;
+  int rc = ensure_ws(h, k, nsplit, nq, nqu);
   if (rc) return rc;
   T *zXv = (T *)h->ws[WS_ZXV], *p = (T *)h->ws[WS_P], *u = (T *)h->ws[WS_U];
   T *Tpart = (T *)h->ws[WS_TPART], *Tc = (T *)h->ws[WS_TC], *Ep = (T *)h->ws[WS_EP];
@@ -481,8 +461,11 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     k_hartree_p<T><<<grid1((long)k * M), 256, 0, st>>>(M, no, k, (const T *)h->XoT,
                                                         zXv, p);
     if ((rc = launch_status("k_hartree_p"))) return rc;
-    k_hartree_u<T><<<(M + 3) / 4, 256, 0, st>>>(M, k, (const T *)h->V, p, u);
-    if ((rc = launch_status("k_hartree_u"))) return rc;
+    const int ntm = (k + 63) / 64, ntn = (M + 63) / 64;
+    const int nblk = pad_grid((long)ntm * ntn * nqu);
+    k_gemm_nn_splitk<T><<<nblk, 256, 0, st>>>(k, M, M, kchu, p, M, (const T *)h->V, M,
+                                              u, ntm, ntn, nqu, nblk);
+    if ((rc = launch_status("k_gemm_nn_splitk(hartree)"))) return rc;
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
   switch (h->nopad) {  // exchange
@@ -501,15 +484,16 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   {  // tail
     k_tail_prep<T><<<grid1((long)rows * M), 256, 0, st>>>(
         k, no, h->nopad, M, nsplit_eff, Tpart, (const T *)h->XoT,
-        h->kappa != 0.0 ? u : nullptr, (T)h->kappa, Tc);
+        h->kappa != 0.0 ? u : nullptr, nqu, (T)h->kappa, Tc);
     if ((rc = launch_status("k_tail_prep"))) return rc;
     const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
     const int kchunk = ((M + nq - 1) / nq + 15) / 16 * 16;
     const int nq_eff = (M + kchunk - 1) / kchunk;
     const int nblk = pad_grid((long)ntm * ntn * nq_eff);
-    k_tail<T><<<nblk, 256, 0, st>>>(rows, nv, M, kchunk, Tc, (const T *)h->Xv, Ep, ntm,
-                                    ntn, nq_eff, nblk);
-    if ((rc = launch_status("k_tail"))) return rc;
+    k_gemm_nn_splitk<T><<<nblk, 256, 0, st>>>(rows, nv, M, kchunk, Tc, M,
+                                              (const T *)h->Xv, nv, Ep, ntm, ntn,
+                                              nq_eff, nblk);
+    if ((rc = launch_status("k_gemm_nn_splitk(tail)"))) return rc;
     k_tail_finish<T><<<grid1((long)rows * nv), 256, 0, st>>>(rows, no, nv, nq_eff, Ep,
                                                               h->d, z, Az);
     if ((rc = launch_status("k_tail_finish"))) return rc;

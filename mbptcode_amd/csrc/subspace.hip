@@ -1,0 +1,109 @@
+// FP64 reductions over the pair index for the Davidson subspace on gfx950:
+// C (a x b) = A^T B with A (n x a) and B (n x b), n the pair count (10^3 to
+// 10^5) and a, b the trial-space widths (tens to a few hundred).  A library
+// GEMM tiles the tiny output and runs the long contraction in one or two
+// workgroups; here the contraction is cut into row chunks, each chunk's
+// partial product is an MFMA tile (v_mfma_f64_16x16x4f64), and a second pass
+// sums the partials in chunk order (deterministic).
+#include "common.h"
+
+#include <algorithm>
+
+namespace mbpt {
+
+struct LdAt {  // A(m = i, k = r) = A[r][i]
+  const double *p;
+  int ld, rows, K;
+  __device__ double operator()(int m, int k) const {
+    return (m < rows && k < K) ? p[(long)k * ld + m] : 0.0;
+  }
+};
+struct LdBr {  // B(k = r, n = j) = B[r][j]
+  const double *p;
+  int ld, K, cols;
+  __device__ double operator()(int k, int n) const {
+    return (k < K && n < cols) ? p[(long)k * ld + n] : 0.0;
+  }
+};
+
+__global__ __launch_bounds__(256) void k_tn_partial(int n, int a, int b, int chunk,
+                                                    const double *__restrict__ A, int lda,
+                                                    const double *__restrict__ B, int ldb,
+                                                    double *__restrict__ P, int nta,
+                                                    int ntb, int nc, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= nta * ntb * nc) return;
+  const int tb = lid % ntb, ta = (lid / ntb) % nta, c = lid / (ntb * nta);
+  const int r0 = c * chunk, rc = min(chunk, n - r0);
+  __shared__ TileSmem<double> sm;
+  MF<double>::acc acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) zero(acc[i][j]);
+  gemm64<double, false, false>(LdAt{A + (long)r0 * lda, lda, a, rc},
+                               LdBr{B + (long)r0 * ldb, ldb, rc, b}, ta * kTile,
+                               tb * kTile, rc, acc, sm);
+  double *out = P + (long)c * a * b;
+  for (int fm = 0; fm < 2; ++fm)
+    for (int fn = 0; fn < 2; ++fn) {
+      const int j = tile_col(tb * kTile, fn);
+      for (int r = 0; r < 4; ++r) {
+        const int i = tile_row<double>(ta * kTile, fm, r);
+        if (i < a && j < b) out[(long)i * b + j] = acc[fm][fn][r];
+      }
+    }
+}
+
+__global__ void k_tn_sum(int a, int b, int nc, const double *__restrict__ P,
+                         double *__restrict__ C, int ldc) {
+  const long count = (long)a * b;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count;
+       e += (long)gridDim.x * blockDim.x) {
+    double s = P[e];
+    for (int c = 1; c < nc; ++c) s += P[(long)c * count + e];
+    C[(e / b) * ldc + (e % b)] = s;
+  }
+}
+
+}  // namespace mbpt
+
+using namespace mbpt;
+
+namespace {
+int tn_chunk(int n) {
+  const int c = std::max(256, (n + 63) / 64);
+  return (c + 15) / 16 * 16;
+}
+}  // namespace
+
+extern "C" {
+
+long mbpt_gemm_tn_f64_work(int n, int a, int b) {
+  if (n <= 0 || a <= 0 || b <= 0) return 0;
+  const int chunk = tn_chunk(n);
+  return (long)((n + chunk - 1) / chunk) * a * b;
+}
+
+int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda, const double *B,
+                     int ldb, double *C, int ldc, double *work, long work_elems,
+                     void *stream) {
+  MBPT_REQUIRE(n > 0 && a >= 0 && b >= 0, "mbpt_gemm_tn_f64: n must be positive, a and b non-negative");
+  if (a == 0 || b == 0) return MBPT_OK;
+  MBPT_REQUIRE(A && B && C && work, "mbpt_gemm_tn_f64: NULL argument");
+  MBPT_REQUIRE(lda >= a && ldb >= b && ldc >= b, "mbpt_gemm_tn_f64: leading dimension too small");
+  hipStream_t st = (hipStream_t)stream;
+  const int chunk = tn_chunk(n);
+  const int nc = (n + chunk - 1) / chunk;
+  MBPT_REQUIRE(work_elems >= (long)nc * a * b, "mbpt_gemm_tn_f64: work buffer too small");
+  const int nta = (a + kTile - 1) / kTile, ntb = (b + kTile - 1) / kTile;
+  const int nblk = pad_grid((long)nta * ntb * nc);
+  k_tn_partial<<<nblk, 256, 0, st>>>(n, a, b, chunk, A, lda, B, ldb, work, nta, ntb, nc,
+                                     nblk);
+  int rc = launch_status("k_tn_partial");
+  if (rc) return rc;
+  const long count = (long)a * b;
+  k_tn_sum<<<(int)std::min<long>((count + 255) / 256, 4096L), 256, 0, st>>>(a, b, nc, work,
+                                                                            C, ldc);
+  return launch_status("k_tn_sum");
+}
+
+}  // extern "C"

@@ -28,6 +28,7 @@ import warnings
 import numpy as np
 import torch
 
+from .subspace import tn
 from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
                         DAVIDSON_GUESS_DEG_TOL, DAVIDSON_GUESS_FACTOR,
                         DAVIDSON_LINDEP_CEILING, DAVIDSON_LINDEP_FLOOR,
@@ -119,6 +120,14 @@ def jacobi_davidson_correction(R, U, diag, theta, floor=DAVIDSON_DENOMINATOR_FLO
 # the iteration
 # ---------------------------------------------------------------------------
 
+def _tn(A, B):
+    """A^T B: on the device through the split-contraction FP64 kernel
+    (the subspace reductions over the pair index), else plain torch."""
+    if A.is_cuda:
+        return tn(A, B)
+    return A.T @ B
+
+
 def _orthonormal_new(T, V, drop):
     """T projected off the orthonormal columns V (twice) and orthonormalised
     among itself by the eigendecomposition of its Gram matrix (solved on the
@@ -126,9 +135,9 @@ def _orthonormal_new(T, V, drop):
     the projection is below `drop` relative to before are discarded.  FP64."""
     T = T / T.norm(dim=0).clamp_min(1e-300)
     if V is not None and V.shape[1]:
-        T = T - V @ (V.T @ T)
-        T = T - V @ (V.T @ T)
-    G = (T.T @ T).cpu().numpy()
+        T = T - V @ _tn(V, T)
+        T = T - V @ _tn(V, T)
+    G = _tn(T, T).cpu().numpy()
     keep = np.flatnonzero(np.sqrt(np.clip(np.diag(G), 0, None)) > drop)
     if keep.size == 0:
         return T[:, :0]
@@ -196,10 +205,10 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         AVb[:, m:m + p] = AT
         # Hb holds V^T (A V) whole: a lower-precision operator is not exactly
         # symmetric, and x^T A x must be the one the residual sees.
-        Hb[:m + p, m:m + p] = Vb[:, :m + p].T @ AT
+        Hb[:m + p, m:m + p] = _tn(Vb[:, :m + p], AT)
         if m:
-            Hb[m:m + p, :m] = T.T @ AVb[:, :m]
-        blkG = AVb[:, :m + p].T @ AT
+            Hb[m:m + p, :m] = _tn(T, AVb[:, :m])
+        blkG = _tn(AVb[:, :m + p], AT)
         Gb[:m + p, m:m + p] = blkG
         Gb[m:m + p, :m] = blkG[:m].T
         m += p
