@@ -44,6 +44,10 @@ def tn(A, B, out=None):
         out = torch.empty((a, b), dtype=torch.float64, device=A.device)
     if a == 0 or b == 0:
         return out
+    if a > 1 and A.stride(1) != 1:
+        A = A.contiguous()
+    if b > 1 and B.stride(1) != 1:
+        B = B.contiguous()
     lib = _lib.lib()
     pa, lda = _rowmajor(A)
     pb, ldb = _rowmajor(B)
