@@ -14,6 +14,7 @@
 // and per grid-row tile; here all k vectors go in every launch, so the W tile
 // a workgroup reads is shared by the k workgroups of one XCD through its L2.
 #include "common.h"
+#include "exchange_bf16.h"
 
 #include <algorithm>
 #include <vector>
@@ -341,6 +342,7 @@ struct mbpt_sigma {
   double kappa;
   size_t esz;              // bytes per working-precision element
   void *Xo, *Xv, *XoT, *W, *V;
+  __bf16 *xh, *xl, *xth, *xtl;   // split X_o (MBPT_BF16X3): [M][nopad], [nopad][M]
   double *d;
   // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
   void *ws[6];
@@ -431,6 +433,16 @@ void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_
                                              (T *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
 
+template <int NP>
+void launch_exchange_bf16(const mbpt_sigma *h, int k, int nsplit, int rps,
+                          hipStream_t st) {
+  const int nvb = (h->M + 63) / 64;
+  const int nblk = pad_grid((long)nvb * k * nsplit);
+  k_exchange_bf16<NP><<<nblk, 256, 0, st>>>(
+      h->M, h->no, k, h->xh, h->xl, h->xth, h->xtl, (const float *)h->W,
+      (const float *)h->ws[WS_ZXV], (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
+}
+
 template <typename T>
 int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st) {
   const int M = h->M, no = h->no, nv = h->nv;
@@ -468,7 +480,15 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     if ((rc = launch_status("k_gemm_nn_splitk(hartree)"))) return rc;
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
-  switch (h->nopad) {  // exchange
+  if (h->prec == MBPT_BF16X3) {  // exchange, split BF16
+    switch (h->nopad) {
+      case 32: launch_exchange_bf16<32>(h, k, nsplit_eff, rps, st); break;
+      case 64: launch_exchange_bf16<64>(h, k, nsplit_eff, rps, st); break;
+      case 96: launch_exchange_bf16<96>(h, k, nsplit_eff, rps, st); break;
+      case 128: launch_exchange_bf16<128>(h, k, nsplit_eff, rps, st); break;
+      default: set_error("internal: nopad"); return MBPT_ERR_VALUE;
+    }
+  } else switch (h->nopad) {  // exchange, FP32 / FP64
     case 16: launch_exchange<T, 16>(h, k, nsplit_eff, rps, st); break;
     case 32: launch_exchange<T, 32>(h, k, nsplit_eff, rps, st); break;
     case 48: launch_exchange<T, 48>(h, k, nsplit_eff, rps, st); break;
@@ -553,14 +573,16 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
   MBPT_REQUIRE(nocc <= kMaxNocc, "mbpt_sigma_create: nocc > 128 is not supported by the exchange kernel");
   MBPT_REQUIRE(X_dev && W_dev && eps_host, "mbpt_sigma_create: NULL X, W or eps");
   MBPT_REQUIRE(kappa == 0.0 || V_dev, "mbpt_sigma_create: kappa != 0 needs V");
-  if (precision != MBPT_FP64 && precision != MBPT_FP32) {
+  if (precision != MBPT_FP64 && precision != MBPT_FP32 && precision != MBPT_BF16X3) {
     set_error("mbpt_sigma_create: precision " + std::to_string(precision) +
-              " is not built (MBPT_FP64 or MBPT_FP32)");
+              " is not built (MBPT_FP64, MBPT_FP32 or MBPT_BF16X3)");
     return MBPT_ERR_UNSUPPORTED;
   }
   hipStream_t st = (hipStream_t)stream;
   mbpt_sigma *h = new mbpt_sigma();
-  h->M = M; h->no = nocc; h->nv = nvir; h->nopad = (nocc + 15) / 16 * 16;
+  h->M = M; h->no = nocc; h->nv = nvir;
+  // exchange K width: 16-multiples for the 16x16x4 MFMAs, 32 for 16x16x32 BF16
+  h->nopad = precision == MBPT_BF16X3 ? (nocc + 31) / 32 * 32 : (nocc + 15) / 16 * 16;
   h->prec = precision; h->kappa = kappa;
   h->esz = precision == MBPT_FP64 ? 8 : 4;
   const size_t e = h->esz;
@@ -595,6 +617,21 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
   int rc = precision == MBPT_FP64 ? create_impl<double>(h, X_dev, W_dev, V_dev, st)
                                   : create_impl<float>(h, X_dev, W_dev, V_dev, st);
   if (rc) return fail(rc);
+  if (precision == MBPT_BF16X3) {
+    const size_t nb = sizeof(__bf16) * (size_t)M * h->nopad;
+    __bf16 **bufs[4] = {&h->xh, &h->xl, &h->xth, &h->xtl};
+    for (__bf16 **b : bufs) {
+      he = hipMalloc((void **)b, nb);
+      if (he != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
+        return fail(MBPT_ERR_HIP);
+      }
+    }
+    h->held += 4 * nb;
+    k_split_bf16_xo<<<grid1((long)M * h->nopad), 256, 0, st>>>(
+        M, nocc, h->nopad, (const float *)h->Xo, h->xh, h->xl, h->xth, h->xtl);
+    if ((rc = launch_status("k_split_bf16_xo"))) return fail(rc);
+  }
   *out = h;
   return MBPT_OK;
 }
@@ -656,6 +693,7 @@ int mbpt_sigma_destroy(mbpt_sigma *h) {
     for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
   free_ws(h);
   (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
+  (void)hipFree(h->xh); (void)hipFree(h->xl); (void)hipFree(h->xth); (void)hipFree(h->xtl);
   (void)hipFree(h->d);
   delete h;
   return MBPT_OK;

@@ -69,8 +69,14 @@ __global__ void k_tn_sum(int a, int b, int nc, const double *__restrict__ P,
 using namespace mbpt;
 
 namespace {
-int tn_chunk(int n) {
-  const int c = std::max(256, (n + 63) / 64);
+// Rows per chunk: short chunks (each workgroup runs a few K steps, the
+// partial sums are latency-bound otherwise), about a thousand workgroups in
+// all, at least 64 rows.
+int tn_chunk(int n, int a, int b) {
+  const long tiles = (long)((a + kTile - 1) / kTile) * ((b + kTile - 1) / kTile);
+  long nc = std::max<long>(1, 1024 / tiles);
+  nc = std::min<long>(nc, (n + 63) / 64);
+  const int c = (int)((n + nc - 1) / nc);
   return (c + 15) / 16 * 16;
 }
 }  // namespace
@@ -79,7 +85,7 @@ extern "C" {
 
 long mbpt_gemm_tn_f64_work(int n, int a, int b) {
   if (n <= 0 || a <= 0 || b <= 0) return 0;
-  const int chunk = tn_chunk(n);
+  const int chunk = tn_chunk(n, a, b);
   return (long)((n + chunk - 1) / chunk) * a * b;
 }
 
@@ -91,7 +97,7 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda, const double
   MBPT_REQUIRE(A && B && C && work, "mbpt_gemm_tn_f64: NULL argument");
   MBPT_REQUIRE(lda >= a && ldb >= b && ldc >= b, "mbpt_gemm_tn_f64: leading dimension too small");
   hipStream_t st = (hipStream_t)stream;
-  const int chunk = tn_chunk(n);
+  const int chunk = tn_chunk(n, a, b);
   const int nc = (n + chunk - 1) / chunk;
   MBPT_REQUIRE(work_elems >= (long)nc * a * b, "mbpt_gemm_tn_f64: work buffer too small");
   const int nta = (a + kTile - 1) / kTile, ntb = (b + kTile - 1) / kTile;

@@ -182,3 +182,39 @@ def test_reference_signature_entry():
         ref = O.isdf_block_action(inst.eps, 6, inst.X, D, W, kappa=kappa)[0](z)
         assert rel(Az, ref) < 1e-11
         assert np.allclose(d, O.pair_diagonal(inst.eps, 6))
+
+
+# Split BF16: the error is measured on the kernel part K z = A z - d o z
+# (d o z is exact in every mode).  hi+lo carries ~16 bits per operand, so
+# |dKz| / |Kz| ~ 1e-5; plain BF16 (8 bits) would sit near 4e-3.
+KTOL = {'fp64': 1e-12, 'fp32': 2e-6, 'bf16x3': 6e-5}
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+@pytest.mark.parametrize('shape', [(1, 3, 10), (5, 19, 120), (21, 93, 1200),
+                                   (33, 20, 65), (60, 70, 257), (100, 40, 200),
+                                   (128, 10, 70)])
+def test_kernel_part_relative_error(shape, precision):
+    nocc, nvir, M = shape
+    print(f'sigma kernel part {shape} {precision}', flush=True)
+    inst = synth.make_instance(nocc, nvir, M, seed=[7, nocc, M])
+    W_grid, D, W_aux = grid_inputs(inst, screened=False)
+    z = np.random.default_rng(nocc + 1).normal(size=(6, nocc, nvir))
+    ref = O.isdf_block_action(inst.eps, nocc, inst.X, D, W_aux)[0](z)
+    dz = O.pair_diagonal(inst.eps, nocc)[None] * z
+    op = device_action(inst, W_grid, precision)
+    got = apply_np(op, z)
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref - dz)
+    assert err < KTOL[precision], err
+
+
+def test_bf16x3_golden_and_screened_diagonal():
+    gold = np.load(GOLDEN)
+    inst = synth.config(0)
+    op = device_action(inst, gold['W_grid'], 'bf16x3')
+    got = apply_np(op, gold['z'])
+    dz = O.pair_diagonal(inst.eps, inst.nocc)[None] * gold['z']
+    assert np.linalg.norm(got - gold['Az']) / np.linalg.norm(gold['Az'] - dz) < KTOL['bf16x3']
+    sd = op.screened_diagonal_device().cpu().numpy()
+    d = O.pair_diagonal(inst.eps, inst.nocc)
+    assert np.abs(sd - gold['sdiag']).max() < 1e-6 * np.abs(gold['sdiag'] - d).max() + 1e-12
