@@ -143,6 +143,41 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda,
                      const double *B, int ldb, double *C, int ldc,
                      double *work, long work_elems, void *stream);
 
+/* ------------------------------------------------------------------------
+ * Learned Davidson preconditioner: a 3 -> 256 -> 256 -> 1 SiLU network, BF16
+ * weights and activations, FP32 accumulation.  No counterpart in the
+ * reference: it stands in for the diagonal correction r / (d - theta)
+ * (LR/davidson_shared.py:157-163, Solvers/davidson.py:43-77).
+ *
+ * Parameters (FP32, host order): W1 (3, 256), b1 (256), W2 (256, 256),
+ * b2 (256), w3 (256), b3 (1); mbpt_mlp_nparams() of them.
+ * ------------------------------------------------------------------------ */
+typedef struct mbpt_mlp mbpt_mlp;
+
+int mbpt_mlp_nparams(void);
+int mbpt_mlp_create(mbpt_mlp **out, const float *params_host, void *stream);
+int mbpt_mlp_get_params(mbpt_mlp *h, float *params_host, void *stream);
+/* y (N) = net(X (N, 3)) */
+int mbpt_mlp_forward(mbpt_mlp *h, long N, const float *X_dev, float *y_dev,
+                     void *stream);
+/* mean-squared-error loss against T (N) and its gradient (nparams, device;
+ * NULL to skip the copy), loss on the host (NULL to skip) */
+int mbpt_mlp_gradient(mbpt_mlp *h, long N, const float *X_dev,
+                      const float *T_dev, float *grad_dev, float *loss_host,
+                      void *stream);
+/* one Adam step (beta 0.9 / 0.999) on the mean-squared error */
+int mbpt_mlp_train_step(mbpt_mlp *h, long N, const float *X_dev,
+                        const float *T_dev, float lr, float *loss_host,
+                        void *stream);
+/* Davidson correction for k residual columns: R (n, k) and T (n, k) FP64
+ * with row strides ldr, ldt; d (n) pair energies, theta (k) Ritz values.
+ * T[i][j] = rms_j * net(R[i][j] / rms_j, d_i - theta_j, theta_j), rms_j the
+ * root-mean-square of column j of R. */
+int mbpt_mlp_correction(mbpt_mlp *h, int n, int k, const double *R_dev,
+                        int ldr, const double *d_dev, const double *theta_dev,
+                        double *T_dev, int ldt, void *stream);
+int mbpt_mlp_destroy(mbpt_mlp *h);
+
 #ifdef __cplusplus
 }
 #endif

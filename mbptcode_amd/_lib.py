@@ -24,11 +24,15 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_sigma_held_bytes', 'mbpt_sigma_profile',
            'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
            'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
-           'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64')
+           'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
+           'mbpt_mlp_nparams', 'mbpt_mlp_create', 'mbpt_mlp_get_params',
+           'mbpt_mlp_forward', 'mbpt_mlp_gradient', 'mbpt_mlp_train_step',
+           'mbpt_mlp_correction', 'mbpt_mlp_destroy')
 
 _lib = None
 
 _vp, _i, _d, _dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _SIGNATURES = {
     'mbpt_last_error': (ctypes.c_char_p, []),
     'mbpt_abi_version': (_i, []),
@@ -49,6 +53,14 @@ _SIGNATURES = {
     'mbpt_gemm_tn_f64_work': (ctypes.c_long, [_i, _i, _i]),
     'mbpt_gemm_tn_f64': (_i, [_i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, ctypes.c_long,
                               _vp]),
+    'mbpt_mlp_nparams': (_i, []),
+    'mbpt_mlp_create': (_i, [ctypes.POINTER(_vp), _fp, _vp]),
+    'mbpt_mlp_get_params': (_i, [_vp, _fp, _vp]),
+    'mbpt_mlp_forward': (_i, [_vp, ctypes.c_long, _vp, _vp, _vp]),
+    'mbpt_mlp_gradient': (_i, [_vp, ctypes.c_long, _vp, _vp, _vp, _fp, _vp]),
+    'mbpt_mlp_train_step': (_i, [_vp, ctypes.c_long, _vp, _vp, ctypes.c_float, _fp, _vp]),
+    'mbpt_mlp_correction': (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp]),
+    'mbpt_mlp_destroy': (_i, [_vp]),
 }
 
 

@@ -265,8 +265,15 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         Xu = Vb[:, :m] @ yu
         AXu = AVb[:, :m] @ yu
         wu = torch.as_tensor(w[idx], device=dev)
-        T = precond(AXu - Xu * wu[None, :], Xu, wu, cycle)
-        T = _orthonormal_new(T, Vb[:, :m], drop)
+        Ru = AXu - Xu * wu[None, :]
+        T = _orthonormal_new(precond(Ru, Xu, wu, cycle), Vb[:, :m], drop)
+        if T.shape[1] == 0 and on_stall is not None and not stalled:
+            # the correction added nothing new: the same fallback as a stall
+            stalled = True
+            new_pre = on_stall(cycle)
+            if new_pre is not None:
+                precond = new_pre
+            T = _orthonormal_new(precond(Ru, Xu, wu, cycle), Vb[:, :m], drop)
         if T.shape[1] == 0:
             break
         if m + T.shape[1] > max_space:
