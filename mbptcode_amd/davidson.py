@@ -160,7 +160,7 @@ class DavidsonResult:
 
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
-             lindep=None, stall_cycles=None, on_stall=None):
+             lindep=None, stall_cycles=None, on_stall=None, observe=None):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -266,6 +266,8 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         AXu = AVb[:, :m] @ yu
         wu = torch.as_tensor(w[idx], device=dev)
         Ru = AXu - Xu * wu[None, :]
+        if observe is not None:
+            observe(cycle, Vb[:, :m], Ru, Xu, wu)
         T = _orthonormal_new(precond(Ru, Xu, wu, cycle), Vb[:, :m], drop)
         if T.shape[1] == 0 and on_stall is not None and not stalled:
             # the correction added nothing new: the same fallback as a stall
@@ -364,7 +366,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  preconditioner=DAVIDSON_PRECONDITIONER, read_roots=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
-                 learned=None, stall_cycles=3):
+                 learned=None, stall_cycles=3, learned_until=100.0):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
 
@@ -419,7 +421,11 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
         state = {'fallback': False}
 
         def precond(R, U, theta, cycle):
-            if state['fallback']:
+            # near convergence the useful (out-of-space) part of a correction
+            # is small next to the network's BF16 output noise: the
+            # Jacobi-Davidson correction takes over below learned_until * tol
+            if (state['fallback'] or learned_until is not None and
+                    float(R.norm(dim=0).max()) < learned_until * conv_tol):
                 return jacobi_davidson_correction(R, U, pdiag, theta)
             return learned.correction(R, U, theta, pdiag)
 

@@ -2,15 +2,21 @@
 
 A 3 -> 256 -> 256 -> 1 SiLU network (BF16 weights, FP32 accumulation, the
 forward, backward and Adam kernels in libmbpt_hip.so, mlp.hip) maps each pair's
-[r_ia / rms(r), d_ia - theta, theta] to a correction t_ia / rms(r).  It stands
-in for the diagonal correction r / (d - theta) (LR/davidson_shared.py:157-163,
-Solvers/davidson.py:43-77); `run_davidson(preconditioner='learned')` falls back
-to the screened Jacobi-Davidson correction when the residual stalls.
+[r_ia / rms(r), d_ia - theta, theta] -- d the pair energy on the Davidson's
+preconditioner diagonal -- to t_ia / rms(r).  It stands in for the diagonal
+correction r / (d - theta) (LR/davidson_shared.py:157-163, Solvers/davidson.py:
+43-77): the correction applied is the Jacobi-Davidson one on that diagonal
+plus rms(r) times the network's output, a learned remainder (the output
+layer starts at zero, so an untrained network is the diagonal correction).
+`run_davidson(preconditioner='learned')` falls back to the screened
+Jacobi-Davidson correction alone when the residual stalls.
 
 Training targets come from plain FP64 Davidson runs: for a Ritz vector u with
 residual r, the exact eigenvector x of the root it approximates (the tracked
 exact root of largest overlap) gives the ideal correction
-t* = x / (u . x) - u, which is orthogonal to u and makes u + t* parallel to x.
+t* = x / (u . x) - u, orthogonal to u and making u + t* parallel to x; the
+network learns (t* - t_JD) / rms(r) on pairs drawn half uniformly and half in
+proportion to |r|.
 """
 import ctypes
 import os
@@ -30,12 +36,16 @@ def nparams():
     return int(_lib.lib().mbpt_mlp_nparams())
 
 
-def init_params(seed=0, hidden=HIDDEN):
-    """He-style initial parameters, flattened FP32 (W1, b1, W2, b2, w3, b3)."""
+def init_params(seed=0, hidden=HIDDEN, zero_output=False):
+    """He-style initial parameters, flattened FP32 (W1, b1, W2, b2, w3, b3);
+    zero_output starts the output layer at 0 (the net adds nothing to the
+    Jacobi-Davidson correction before it is trained)."""
     rng = np.random.default_rng(seed)
+    w3 = (np.zeros(hidden) if zero_output
+          else rng.normal(0, np.sqrt(1.0 / hidden), hidden))
     parts = [rng.normal(0, np.sqrt(2.0 / 3), (3, hidden)), np.zeros(hidden),
              rng.normal(0, np.sqrt(2.0 / hidden), (hidden, hidden)), np.zeros(hidden),
-             rng.normal(0, np.sqrt(1.0 / hidden), hidden), np.zeros(1)]
+             w3, np.zeros(1)]
     return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
 
 
@@ -68,6 +78,9 @@ class LearnedPreconditioner:
         self.d = None if d is None else torch.as_tensor(
             np.asarray(d, float).ravel(), device=self.device)
         self.calls = 0
+        #: the network predicts the remainder t - t_JD on top of the
+        #: Jacobi-Davidson correction on the same diagonal (how it is trained)
+        self.residual_of_jd = True
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -88,12 +101,16 @@ class LearnedPreconditioner:
         return out
 
     def save(self, path=DEFAULT_WEIGHTS, **meta):
-        np.savez(path, params=self.params(), **{k: np.asarray(v) for k, v in meta.items()})
+        np.savez(path, params=self.params(), residual_of_jd=np.asarray(self.residual_of_jd),
+                 **{k: np.asarray(v) for k, v in meta.items()})
 
     @classmethod
     def load(cls, path=DEFAULT_WEIGHTS, d=None):
         with np.load(path, allow_pickle=False) as f:
-            return cls(params=f['params'], d=d)
+            net = cls(params=f['params'], d=d)
+            if 'residual_of_jd' in f:
+                net.residual_of_jd = bool(f['residual_of_jd'])
+        return net
 
     # -- network ------------------------------------------------------------
     def forward(self, X):
@@ -149,19 +166,26 @@ class LearnedPreconditioner:
     # -- the Davidson hook -----------------------------------------------------
     def correction(self, R, U, theta, diag=None):
         """Corrections T (n, k) for residual columns R (n, k) at Ritz values
-        theta (k): rms(r) * net(r / rms(r), d - theta, theta) with d the bare
-        pair energies this preconditioner was built with."""
-        if self.d is None:
+        theta (k): rms(r) * net(r / rms(r), d - theta, theta).  d is the pair
+        energy on the Davidson's preconditioner diagonal (`diag`, the screened
+        d - (ii|W|aa) by default, the features the network was trained on),
+        else the pair energies this object was built with."""
+        dvec = diag if diag is not None else self.d
+        if dvec is None:
             raise ValueError('LearnedPreconditioner needs the pair energies d')
+        dvec = dvec.reshape(-1).to(torch.float64).contiguous()
         n, k = R.shape
         R = R.contiguous()
         T = torch.empty_like(R)
         th = theta.to(torch.float64).contiguous()
         _lib.check(self._lib.mbpt_mlp_correction(self._h, n, k, _lib.dptr(R), k,
-                                                  _lib.dptr(self.d), _lib.dptr(th),
+                                                  _lib.dptr(dvec), _lib.dptr(th),
                                                   _lib.dptr(T), k, _lib.stream_ptr()),
                    'mbpt_mlp_correction')
         self.calls += 1
+        if self.residual_of_jd and U is not None:
+            from .davidson import jacobi_davidson_correction
+            T += jacobi_davidson_correction(R, U, dvec, th)
         return T
 
 
@@ -170,13 +194,17 @@ class LearnedPreconditioner:
 # ---------------------------------------------------------------------------
 
 def harvest(inst, W_grid=None, pairs_per_root=256, seed=0, tol=1e-5, tight=1e-9,
-            max_cycle=200):
+            max_cycle=200, target='jd'):
     """(X (N, 3), T (N,)) float32 device tensors from one instance.
 
     A tight FP64 solve gives the exact tracked eigenvectors; a plain FP64
     solve at `tol` with the screened Jacobi-Davidson correction then records,
     every cycle and every unconverged root, the features of `pairs_per_root`
-    random pairs and the scaled ideal correction t*_ia / rms(r).
+    pairs and the scaled target: target 'jd' the remainder
+    (t* - t_JD) / rms(r) of the ideal correction over the Jacobi-Davidson one;
+    target 'span' the part of the exact vector outside the current trial
+    space, (x - V V^T x) / (u . x) / rms(r), the direction the next
+    expansion should add.
     """
     from .block_actions import grid_block_action
     from .davidson import davidson, guess_indices, tracked_roots, jacobi_davidson_correction
@@ -192,7 +220,6 @@ def harvest(inst, W_grid=None, pairs_per_root=256, seed=0, tol=1e-5, tight=1e-9,
     no, nv = inst.nocc, inst.nvir
     npair = no * nv
     dev = op.device
-    d = torch.as_tensor(dd.ravel(), device=dev)
     pdiag = op.screened_diagonal_device().reshape(-1)
     track = tracked_roots(inst.nroots, npair)
     order = guess_indices(dd, inst.nroots)
@@ -208,26 +235,37 @@ def harvest(inst, W_grid=None, pairs_per_root=256, seed=0, tol=1e-5, tight=1e-9,
     gen = torch.Generator(device=dev).manual_seed(seed)
     feats, targs = [], []
 
-    def record(R, U, theta, cycle):
+    def record(cycle, V, R, U, theta):
         ov = (U.T @ Xe).abs()                           # (k, track)
         j = ov.argmax(dim=1)
         x = Xe[:, j]
         dots = (U * x).sum(0)
         ok = dots.abs() > 0.5
-        tstar = x / dots[None, :] - U
+        if target == 'span':
+            # the part of the exact vector the trial space V still lacks
+            tstar = (x - V @ (V.T @ x)) / dots[None, :]
+        else:
+            tstar = x / dots[None, :] - U
         rms = R.pow(2).mean(0).sqrt().clamp_min(1e-300)
         k = R.shape[1]
-        idx = torch.randint(0, npair, (pairs_per_root, k), device=dev, generator=gen)
+        # half the pairs uniform, half in proportion to |r| (where the
+        # correction matters)
+        half = pairs_per_root // 2
+        idx = torch.cat([
+            torch.randint(0, npair, (half, k), device=dev, generator=gen),
+            torch.multinomial(R.abs().T + 1e-300, pairs_per_root - half,
+                              replacement=True, generator=gen).T], dim=0)
+        jd = jacobi_davidson_correction(R, U, pdiag, theta)
         cols = torch.arange(k, device=dev)[None, :].expand_as(idx)
-        f = torch.stack([R[idx, cols] / rms[None, :], d[idx] - theta[None, :],
+        f = torch.stack([R[idx, cols] / rms[None, :], pdiag[idx] - theta[None, :],
                          theta[None, :].expand_as(idx)], dim=-1)
-        t = tstar[idx, cols] / rms[None, :]
+        base = jd[idx, cols] if target == 'jd' else 0.0
+        t = (tstar[idx, cols] - base) / rms[None, :]
         keep = ok[None, :].expand_as(idx)
         feats.append(f[keep].reshape(-1, 3).float())
         targs.append(t[keep].reshape(-1).float())
-        return jacobi_davidson_correction(R, U, pdiag, theta)
 
-    davidson(mm, pdiag, track, x0, tol=tol, max_cycle=max_cycle, precond=record)
+    davidson(mm, pdiag, track, x0, tol=tol, max_cycle=max_cycle, observe=record)
     if not feats:
         return (torch.empty((0, 3), dtype=torch.float32, device=dev),
                 torch.empty(0, dtype=torch.float32, device=dev))
@@ -235,22 +273,25 @@ def harvest(inst, W_grid=None, pairs_per_root=256, seed=0, tol=1e-5, tight=1e-9,
 
 
 def train_on_instances(instances, steps=2000, batch=65536, lr=2e-3, seed=0, clip=50.0,
-                       log_every=0):
-    """Harvest every instance, fit a fresh network, return (network, info)."""
+                       log_every=0, target='jd'):
+    """Harvest every instance, fit a fresh network, return (network, info).
+    target: 'jd' (the network adds to the Jacobi-Davidson correction) or
+    'span' (the network is the whole correction)."""
     t0 = time.time()
     Xs, Ts = [], []
     for i, inst in enumerate(instances):
-        X, T = harvest(inst, seed=seed + i)
+        X, T = harvest(inst, seed=seed + i, target=target)
         Xs.append(X)
         Ts.append(T)
     X = torch.cat(Xs)
     T = torch.cat(Ts).clamp(-clip, clip)
     t_harvest = time.time() - t0
-    net = LearnedPreconditioner(seed=seed)
+    net = LearnedPreconditioner(params=init_params(seed, zero_output=(target == 'jd')))
+    net.residual_of_jd = target == 'jd'
     t0 = time.time()
     hist = net.fit(X, T, steps=steps, batch=batch, lr=lr, lr_final=lr / 20, seed=seed,
                    log_every=log_every)
     torch.cuda.synchronize()
     return net, {'rows': int(X.shape[0]), 'harvest_s': t_harvest,
                  'train_s': time.time() - t0, 'loss_first': hist[0], 'loss_last': hist[-1],
-                 'target_var': float(T.var())}
+                 'target_var': float(T.var()), 'target': target}

@@ -110,6 +110,7 @@ def test_davidson_with_learned_correction_and_fallback():
     apply_AB, d = grid_block_action(inst.eps, inst.nocc, inst.X, gold['W_grid'], inst.V,
                                     precision='fp64')
     net = LearnedPreconditioner(seed=11, d=d)
+    net.residual_of_jd = False          # the bare (random) network alone
     tims = {}
     w, X, _ = run_davidson(apply_AB, d, inst.nroots, 1e-7, 200, tda=True,
                            preconditioner='learned', learned=net, stall_cycles=2,
