@@ -366,7 +366,8 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  preconditioner=DAVIDSON_PRECONDITIONER, read_roots=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
-                 learned=None, stall_cycles=3, learned_until=100.0):
+                 learned=None, stall_cycles=3, learned_until=100.0,
+                 learned_mode='augment'):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
 
@@ -424,10 +425,14 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
             # near convergence the useful (out-of-space) part of a correction
             # is small next to the network's BF16 output noise: the
             # Jacobi-Davidson correction takes over below learned_until * tol
+            jd = jacobi_davidson_correction(R, U, pdiag, theta)
             if (state['fallback'] or learned_until is not None and
                     float(R.norm(dim=0).max()) < learned_until * conv_tol):
-                return jacobi_davidson_correction(R, U, pdiag, theta)
-            return learned.correction(R, U, theta, pdiag)
+                return jd
+            t = learned.correction(R, U, theta, pdiag)
+            # 'augment': both directions enter the space (the diagonal one is
+            # never lost); 'replace': the learned one alone
+            return torch.cat([jd, t], dim=1) if learned_mode == 'augment' else t
 
         def on_stall(cycle):
             state['fallback'] = True

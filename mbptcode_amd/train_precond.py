@@ -43,8 +43,9 @@ def training_instances(count, seed=2):
 def evaluate(net_params, instances, precision='fp64', residual_of_jd=True):
     """Per preconditioner: total cycles, sigma-vectors and seconds over the
     instances, and the worst root error (meV) against the screened solve."""
+    modes = ('bare', 'screened', 'learned', 'learned_replace')
     out = {p: {'cycles': 0, 'vectors': 0, 'seconds': 0.0, 'max_err_meV': 0.0,
-               'fallbacks': 0} for p in ('bare', 'screened', 'learned')}
+               'fallbacks': 0} for p in modes}
     for inst in instances:
         g = static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
         W, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g,
@@ -55,12 +56,15 @@ def evaluate(net_params, instances, precision='fp64', residual_of_jd=True):
         net = LearnedPreconditioner(params=net_params, d=d)
         net.residual_of_jd = residual_of_jd
         ref = None
-        for pre in ('screened', 'bare', 'learned'):
+        for pre in ('screened', 'bare', 'learned', 'learned_replace'):
             tims = {}
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             w, _, _ = run_davidson(act, d, inst.nroots, inst.tol, 200, tda=True,
-                                   preconditioner=pre, learned=net, timings=tims)
+                                   preconditioner=pre.split('_')[0], learned=net,
+                                   timings=tims,
+                                   learned_mode='replace' if pre.endswith('replace')
+                                   else 'augment')
             torch.cuda.synchronize()
             dt = time.perf_counter() - t0
             if ref is None:
