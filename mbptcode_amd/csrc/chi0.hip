@@ -10,6 +10,7 @@
 // multiplied in registers and folded into the tile's running sum.  The output
 // is written once; off-diagonal tiles are mirrored into the lower triangle.
 #include "common.h"
+#include "chi0_f32.h"
 
 #include <algorithm>
 #include <cmath>
@@ -138,6 +139,8 @@ struct mbpt_chi0 {
   int prof;
   hipEvent_t e0, e1;
   double prof_ms, prof_flops;
+  float *Yt, *part;              // FP32 path: scaled branches, per-chunk sums
+  size_t cap_yt, cap_part;       // bytes
   long prof_launches;
 };
 
@@ -172,6 +175,72 @@ int build_impl(mbpt_chi0 *h, int ntau, const double *tau, const double *w, doubl
     MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->e0, h->e1));
     h->prof_ms += ms;
     h->prof_flops += (double)ntau * (double)h->M * (h->M + 1.0) * (norb + 1.0);
+    h->prof_launches += 1;
+  }
+  return MBPT_OK;
+}
+
+int grow(float *&buf, size_t &cap, size_t bytes) {
+  if (bytes <= cap) return MBPT_OK;
+  (void)hipFree(buf);
+  buf = nullptr;
+  cap = 0;
+  MBPT_HIP_TRY(hipMalloc(&buf, bytes));
+  cap = bytes;
+  return MBPT_OK;
+}
+
+// time-point chunks per tile: the split whose item count best fills whole
+// rounds of one 512-thread workgroup per CU (131 VGPRs: 3 waves per SIMD fit,
+// not the 4 a second workgroup needs)
+int chi0_chunks(int ntiles, int ntau) {
+  const int slots = 256;
+  int best = 1;
+  double best_eff = 0.0;
+  for (int c = 1; c <= std::min(ntau, 8); ++c) {
+    const long items = (long)ntiles * c;
+    const double eff = (double)items / (double)(((items + slots - 1) / slots) * slots);
+    if (eff > best_eff + 0.02) { best_eff = eff; best = c; }
+  }
+  return best;
+}
+
+int build_f32(mbpt_chi0 *h, int ntau, const double *tau, const double *w, double *out,
+              int accumulate, hipStream_t st) {
+  const int M = h->M, Mp = (M + kCT - 1) / kCT * kCT;
+  const int ko = (h->no + 15) / 16 * 16, kv = (h->nv + 15) / 16 * 16;
+  const int nt = Mp / kCT, ntiles = nt * (nt + 1) / 2;
+  const int nchunk = chi0_chunks(ntiles, ntau);
+  const int tpc = (ntau + nchunk - 1) / nchunk;
+  int rc = grow(h->Yt, h->cap_yt, sizeof(float) * (size_t)ntau * (ko + kv) * Mp);
+  if (rc) return rc;
+  rc = grow(h->part, h->cap_part, sizeof(float) * (size_t)nchunk * Mp * Mp);
+  if (rc) return rc;
+  TauW tw;
+  tw.n = ntau;
+  for (int t = 0; t < ntau; ++t) {
+    tw.tau_half[t] = 0.5 * tau[t];
+    tw.w[t] = (float)w[t];
+  }
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->e0, st));
+  const long nprep = (long)ntau * (ko + kv) * Mp;
+  k_chi0_prep<<<(int)std::min<long>((nprep + 255) / 256, 16384L), 256, 0, st>>>(
+      M, Mp, h->no, h->nv, ko, kv, (const float *)h->Xo, (const float *)h->Xv, h->eo,
+      h->ev, tw, h->Yt);
+  if ((rc = launch_status("k_chi0_prep"))) return rc;
+  const int nblk = pad_grid((long)ntiles * nchunk);
+  k_chi0_f32<<<nblk, 512, 0, st>>>(Mp, ko, kv, nt, nchunk, tpc, h->Yt, tw, h->part, nblk);
+  if ((rc = launch_status("k_chi0_f32"))) return rc;
+  k_chi0_reduce<<<(int)std::min<long>(((long)M * M + 255) / 256, 16384L), 256, 0, st>>>(
+      M, Mp, nchunk, h->part, out, accumulate);
+  if ((rc = launch_status("k_chi0_reduce"))) return rc;
+  if (h->prof) {
+    MBPT_HIP_TRY(hipEventRecord(h->e1, st));
+    MBPT_HIP_TRY(hipEventSynchronize(h->e1));
+    float ms = 0.f;
+    MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->e0, h->e1));
+    h->prof_ms += ms;
+    h->prof_flops += (double)ntau * (double)M * (M + 1.0) * (h->no + h->nv + 1.0);
     h->prof_launches += 1;
   }
   return MBPT_OK;
@@ -250,7 +319,7 @@ int mbpt_chi0_build(mbpt_chi0 *h, int ntau, const double *tau_host,
   hipStream_t st = (hipStream_t)stream;
   return h->prec == MBPT_FP64
              ? build_impl<double>(h, ntau, tau_host, weight_host, out_dev, accumulate, st)
-             : build_impl<float>(h, ntau, tau_host, weight_host, out_dev, accumulate, st);
+             : build_f32(h, ntau, tau_host, weight_host, out_dev, accumulate, st);
 }
 
 int mbpt_chi0_profile(mbpt_chi0 *h, int enable) {
@@ -286,6 +355,7 @@ int mbpt_chi0_destroy(mbpt_chi0 *h) {
   }
   (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->so); (void)hipFree(h->sv);
   (void)hipFree(h->eo); (void)hipFree(h->ev);
+  (void)hipFree(h->Yt); (void)hipFree(h->part);
   delete h;
   return MBPT_OK;
 }

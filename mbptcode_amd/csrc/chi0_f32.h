@@ -1,0 +1,179 @@
+// FP32 space-time chi0 kernels, gfx950 (v_mfma_f32_16x16x4f32).
+//
+//   out += sum_t w_t (-2) Go(t) o Gv(t),  Go(t) = Yo(t) Yo(t)^T,  Gv(t) = Yv(t) Yv(t)^T
+//
+// with the exponentials split symmetrically, Yo(t) = X_o e^{e_o tau_t / 2} and
+// Yv(t) = X_v e^{-e_v tau_t / 2}, so both GEMM operands of a Green's function
+// tile are rows of one matrix.  A prep pass writes Y for every time point in
+// a K-major, zero-padded layout Yt[t][c][mu] (c over the padded occupied then
+// virtual columns, mu over M padded to the tile), so the main loop needs no
+// bounds checks and every global load is a coalesced float4.
+//
+// Main kernel: one 128 x 128 upper-triangular output tile and one chunk of the
+// time points per workgroup of 8 waves (2 x 4, 64 x 32 each: 4 x 2 MFMA
+// fragments per Green's function and per running sum).  K is staged 16 deep
+// through double-buffered LDS (row stride 144 floats: the two 16-lane halves of
+// a read land in disjoint banks).  Per time point the occupied and the virtual
+// products are accumulated in registers, multiplied, and folded into the
+// tile's running sum, written once per chunk to a partial buffer; a last pass
+// sums the chunks into the FP64 output and mirrors the lower triangle.
+#pragma once
+
+#include "common.h"
+
+namespace mbpt {
+
+constexpr int kCT = 128;      // output tile edge
+constexpr int kCBK = 16;      // K per stage
+constexpr int kCLD = 144;     // LDS row stride (floats)
+
+struct TauW {
+  int n;
+  double tau_half[MBPT_CHI0_MAX_TAU];
+  float w[MBPT_CHI0_MAX_TAU];
+};
+
+// Yt[t][c][mu] for c < kp (occupied, padded to ko) then virtual (padded to kv)
+__global__ void k_chi0_prep(int M, int Mp, int no, int nv, int ko, int kv,
+                            const float *__restrict__ Xo, const float *__restrict__ Xv,
+                            const double *__restrict__ eo, const double *__restrict__ ev,
+                            TauW tw, float *__restrict__ Yt) {
+  const int K = ko + kv;
+  const long per_t = (long)K * Mp;
+  const long n = per_t * tw.n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int t = (int)(e / per_t);
+    const long r = e % per_t;
+    const int c = (int)(r / Mp), mu = (int)(r % Mp);
+    float y = 0.f;
+    if (mu < M) {
+      if (c < no)
+        y = Xo[(long)mu * no + c] * (float)exp(eo[c] * tw.tau_half[t]);
+      else if (c >= ko && c - ko < nv)
+        y = Xv[(long)mu * nv + (c - ko)] * (float)exp(-ev[c - ko] * tw.tau_half[t]);
+    }
+    Yt[e] = y;
+  }
+}
+
+__device__ __forceinline__ void tri_decode2(int lid, int nt, int &ti, int &tj) {
+  int r = 0, base = 0;
+  while (lid >= base + (nt - r)) { base += nt - r; ++r; }
+  ti = r;
+  tj = r + (lid - base);
+}
+
+// acc[fm][fn] += A[m-rows of this wave] B[n-rows]^T over K rows of Yt
+// (Yt rows are K-major: Y^T), staged through double-buffered LDS.
+__device__ __forceinline__ void chi0_gemm(const float *__restrict__ Yk, int Mp, int m0,
+                                          int n0, int K, f32x4 (&acc)[4][2],
+                                          float (*sa)[kCBK][kCLD],
+                                          float (*sb)[kCBK][kCLD]) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 2, wn = wave & 3;      // 2 x 4 waves: 64 x 32 each
+  const int lr = lane & 15, lq = lane >> 4;
+  // 512 threads stage 16 x 128 of A and of B: one float4 each per operand
+  const int srow = tid >> 5, scol = (tid & 31) * 4;
+  const float *ga = Yk + (long)srow * Mp + m0 + scol;
+  const float *gb = Yk + (long)srow * Mp + n0 + scol;
+  float4 ra = *(const float4 *)ga, rb = *(const float4 *)gb;
+  int buf = 0;
+  *(float4 *)&sa[0][srow][scol] = ra;
+  *(float4 *)&sb[0][srow][scol] = rb;
+  __syncthreads();
+  for (int k0 = 0; k0 < K; k0 += kCBK) {
+    const bool more = k0 + kCBK < K;
+    if (more) {
+      ra = *(const float4 *)(ga + (long)(k0 + kCBK) * Mp);
+      rb = *(const float4 *)(gb + (long)(k0 + kCBK) * Mp);
+    }
+#pragma unroll
+    for (int kk = 0; kk < kCBK / 4; ++kk) {
+      float a[4], b[2];
+#pragma unroll
+      for (int fm = 0; fm < 4; ++fm) a[fm] = sa[buf][4 * kk + lq][64 * wm + 16 * fm + lr];
+#pragma unroll
+      for (int fn = 0; fn < 2; ++fn) b[fn] = sb[buf][4 * kk + lq][32 * wn + 16 * fn + lr];
+#pragma unroll
+      for (int fm = 0; fm < 4; ++fm)
+#pragma unroll
+        for (int fn = 0; fn < 2; ++fn)
+          acc[fm][fn] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[fm], b[fn], acc[fm][fn], 0, 0, 0);
+    }
+    if (more) {
+      *(float4 *)&sa[buf ^ 1][srow][scol] = ra;
+      *(float4 *)&sb[buf ^ 1][srow][scol] = rb;
+    }
+    __syncthreads();
+    buf ^= 1;
+  }
+}
+
+__global__ __launch_bounds__(512) void k_chi0_f32(int Mp, int ko, int kv, int nt,
+                                                  int nchunk, int tpc,
+                                                  const float *__restrict__ Yt, TauW tw,
+                                                  float *__restrict__ part, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  const int ntiles = nt * (nt + 1) / 2;
+  if (lid >= ntiles * nchunk) return;
+  const int chunk = lid / ntiles, tl = lid % ntiles;
+  int ti, tj;
+  tri_decode2(tl, nt, ti, tj);
+  const int m0 = ti * kCT, n0 = tj * kCT;
+  __shared__ __attribute__((aligned(16))) float sa[2][kCBK][kCLD];
+  __shared__ __attribute__((aligned(16))) float sb[2][kCBK][kCLD];
+  f32x4 chi[4][2], go[4][2], gv[4][2];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) zero(chi[i][j]);
+  const int K = ko + kv;
+  const int t0 = chunk * tpc, t1 = min(tw.n, t0 + tpc);
+  for (int t = t0; t < t1; ++t) {
+    const float *Y = Yt + (long)t * K * Mp;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) { zero(go[i][j]); zero(gv[i][j]); }
+    chi0_gemm(Y, Mp, m0, n0, ko, go, sa, sb);
+    chi0_gemm(Y + (long)ko * Mp, Mp, m0, n0, kv, gv, sa, sb);
+    const float c = -2.f * tw.w[t];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) chi[i][j][r] += c * go[i][j][r] * gv[i][j][r];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave >> 2, wn = wave & 3, lr = lane & 15, lq = lane >> 4;
+  float *o = part + (long)chunk * Mp * Mp;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + 64 * wm + 16 * i + 4 * lq + r;
+        const int col = n0 + 32 * wn + 16 * j + lr;
+        o[(long)row * Mp + col] = chi[i][j][r];
+      }
+}
+
+// out[mu][nu] = [acc ? out : 0] + sum_chunks part[.][min tile][max tile]
+__global__ void k_chi0_reduce(int M, int Mp, int nchunk, const float *__restrict__ part,
+                              double *__restrict__ out, int accumulate) {
+  const long n = (long)M * M;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int mu = (int)(e / M), nu = (int)(e % M);
+    const bool upper = (mu / kCT) <= (nu / kCT);
+    const long src = upper ? (long)mu * Mp + nu : (long)nu * Mp + mu;
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(long)c * Mp * Mp + src];
+    out[e] = accumulate ? out[e] + (double)s : (double)s;
+  }
+}
+
+}  // namespace mbpt
