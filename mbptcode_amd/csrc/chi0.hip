@@ -139,7 +139,7 @@ struct mbpt_chi0 {
   int prof;
   hipEvent_t e0, e1;
   double prof_ms, prof_flops;
-  float *Yt, *part;              // FP32 path: scaled branches, per-chunk sums
+  float *Yt, *part, *XT;         // FP32 path: scaled branches, per-chunk sums, X K-major
   size_t cap_yt, cap_part;       // bytes
   long prof_launches;
 };
@@ -223,16 +223,14 @@ int build_f32(mbpt_chi0 *h, int ntau, const double *tau, const double *w, double
     tw.w[t] = (float)w[t];
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->e0, st));
-  const long nprep = (long)ntau * (ko + kv) * Mp;
-  k_chi0_prep<<<(int)std::min<long>((nprep + 255) / 256, 16384L), 256, 0, st>>>(
-      M, Mp, h->no, h->nv, ko, kv, (const float *)h->Xo, (const float *)h->Xv, h->eo,
-      h->ev, tw, h->Yt);
+  k_chi0_prep<<<ntau * (ko + kv), 256, 0, st>>>(M, Mp, h->no, h->nv, ko, kv, h->XT,
+                                                  h->eo, h->ev, tw, h->Yt);
   if ((rc = launch_status("k_chi0_prep"))) return rc;
   const int nblk = pad_grid((long)ntiles * nchunk);
   k_chi0_f32<<<nblk, 512, 0, st>>>(Mp, ko, kv, nt, nchunk, tpc, h->Yt, tw, h->part, nblk);
   if ((rc = launch_status("k_chi0_f32"))) return rc;
-  k_chi0_reduce<<<(int)std::min<long>(((long)M * M + 255) / 256, 16384L), 256, 0, st>>>(
-      M, Mp, nchunk, h->part, out, accumulate);
+  const int nto = (M + 63) / 64;
+  k_chi0_reduce<<<nto * nto, 256, 0, st>>>(M, Mp, nchunk, nto, h->part, out, accumulate);
   if ((rc = launch_status("k_chi0_reduce"))) return rc;
   if (h->prof) {
     MBPT_HIP_TRY(hipEventRecord(h->e1, st));
@@ -300,6 +298,19 @@ int mbpt_chi0_create(mbpt_chi0 **out, int M, int nocc, int nvir, int precision,
   }
   int rc = precision == MBPT_FP64 ? split_impl<double>(h, X_dev, st)
                                   : split_impl<float>(h, X_dev, st);
+  if (!rc && precision == MBPT_FP32) {
+    const int ko = (nocc + 15) / 16 * 16, kv = (nvir + 15) / 16 * 16;
+    he = hipMalloc(&h->XT, sizeof(float) * (size_t)(ko + kv) * M);
+    if (he != hipSuccess) {
+      set_error(std::string("mbpt_chi0_create: ") + hipGetErrorString(he));
+      rc = MBPT_ERR_HIP;
+    } else {
+      const long n = (long)(ko + kv) * M;
+      k_chi0_xt<<<(int)std::min<long>((n + 255) / 256, 16384L), 256, 0, st>>>(
+          M, nocc, nvir, ko, kv, X_dev, h->XT);
+      rc = launch_status("k_chi0_xt");
+    }
+  }
   if (rc) {
     mbpt_chi0_destroy(h);
     return rc;
@@ -355,7 +366,7 @@ int mbpt_chi0_destroy(mbpt_chi0 *h) {
   }
   (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->so); (void)hipFree(h->sv);
   (void)hipFree(h->eo); (void)hipFree(h->ev);
-  (void)hipFree(h->Yt); (void)hipFree(h->part);
+  (void)hipFree(h->Yt); (void)hipFree(h->part); (void)hipFree(h->XT);
   delete h;
   return MBPT_OK;
 }

@@ -33,27 +33,37 @@ struct TauW {
   float w[MBPT_CHI0_MAX_TAU];
 };
 
-// Yt[t][c][mu] for c < kp (occupied, padded to ko) then virtual (padded to kv)
-__global__ void k_chi0_prep(int M, int Mp, int no, int nv, int ko, int kv,
-                            const float *__restrict__ Xo, const float *__restrict__ Xv,
-                            const double *__restrict__ eo, const double *__restrict__ ev,
-                            TauW tw, float *__restrict__ Yt) {
+// Yt[t][c][mu] (c < ko occupied, padded; then virtual, padded to kv) from the
+// K-major collocation XT[c][mu] (same column layout, M columns): one block per
+// (t, c) row, the exponential computed once per row.
+__global__ __launch_bounds__(256) void k_chi0_prep(int M, int Mp, int no, int nv, int ko,
+                                                   int kv, const float *__restrict__ XT,
+                                                   const double *__restrict__ eo,
+                                                   const double *__restrict__ ev, TauW tw,
+                                                   float *__restrict__ Yt) {
   const int K = ko + kv;
-  const long per_t = (long)K * Mp;
-  const long n = per_t * tw.n;
+  const int t = blockIdx.x / K, c = blockIdx.x % K;
+  float f = 0.f;
+  if (c < no)
+    f = (float)exp(eo[c] * tw.tau_half[t]);
+  else if (c >= ko && c - ko < nv)
+    f = (float)exp(-ev[c - ko] * tw.tau_half[t]);
+  const float *x = XT + (long)c * M;
+  float *y = Yt + ((long)t * K + c) * Mp;
+  for (int mu = threadIdx.x; mu < Mp; mu += 256) y[mu] = (mu < M && f != 0.f) ? x[mu] * f : 0.f;
+}
+
+// XT[c][mu]: the collocation K-major with the chi0 column padding
+__global__ void k_chi0_xt(int M, int no, int nv, int ko, int kv,
+                          const double *__restrict__ X, float *__restrict__ XT) {
+  const long n = (long)(ko + kv) * M;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
        e += (long)gridDim.x * blockDim.x) {
-    const int t = (int)(e / per_t);
-    const long r = e % per_t;
-    const int c = (int)(r / Mp), mu = (int)(r % Mp);
-    float y = 0.f;
-    if (mu < M) {
-      if (c < no)
-        y = Xo[(long)mu * no + c] * (float)exp(eo[c] * tw.tau_half[t]);
-      else if (c >= ko && c - ko < nv)
-        y = Xv[(long)mu * nv + (c - ko)] * (float)exp(-ev[c - ko] * tw.tau_half[t]);
-    }
-    Yt[e] = y;
+    const int c = (int)(e / M), mu = (int)(e % M);
+    float v = 0.f;
+    if (c < no) v = (float)X[(long)mu * (no + nv) + c];
+    else if (c >= ko && c - ko < nv) v = (float)X[(long)mu * (no + nv) + no + (c - ko)];
+    XT[e] = v;
   }
 }
 
@@ -161,18 +171,32 @@ __global__ __launch_bounds__(512) void k_chi0_f32(int Mp, int ko, int kv, int nt
       }
 }
 
-// out[mu][nu] = [acc ? out : 0] + sum_chunks part[.][min tile][max tile]
-__global__ void k_chi0_reduce(int M, int Mp, int nchunk, const float *__restrict__ part,
-                              double *__restrict__ out, int accumulate) {
-  const long n = (long)M * M;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
-       e += (long)gridDim.x * blockDim.x) {
-    const int mu = (int)(e / M), nu = (int)(e % M);
-    const bool upper = (mu / kCT) <= (nu / kCT);
-    const long src = upper ? (long)mu * Mp + nu : (long)nu * Mp + mu;
+// out = [acc ? out : 0] + sum_chunks part, one 64 x 64 output tile per block:
+// tiles below the diagonal read the mirrored upper tile and transpose it in
+// LDS, so loads and stores stay coalesced.
+__global__ __launch_bounds__(256) void k_chi0_reduce(int M, int Mp, int nchunk, int nto,
+                                                     const float *__restrict__ part,
+                                                     double *__restrict__ out,
+                                                     int accumulate) {
+  const int bi = blockIdx.x / nto, bj = blockIdx.x % nto;
+  const bool upper = (bi * 64) / kCT <= (bj * 64) / kCT;
+  const int sr0 = upper ? bi * 64 : bj * 64, sc0 = upper ? bj * 64 : bi * 64;
+  __shared__ float t[64][65];
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const long src = (long)(sr0 + r) * Mp + sc0 + c;
     float s = 0.f;
-    for (int c = 0; c < nchunk; ++c) s += part[(long)c * Mp * Mp + src];
-    out[e] = accumulate ? out[e] + (double)s : (double)s;
+    for (int q = 0; q < nchunk; ++q) s += part[(long)q * Mp * Mp + src];
+    t[r][c] = s;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const int row = bi * 64 + r, col = bj * 64 + c;
+    if (row >= M || col >= M) continue;
+    const double v = upper ? t[r][c] : t[c][r];
+    const long o = (long)row * M + col;
+    out[o] = accumulate ? out[o] + v : v;
   }
 }
 
