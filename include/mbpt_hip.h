@@ -144,6 +144,70 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda,
                      double *work, long work_elems, void *stream);
 
 /* ------------------------------------------------------------------------
+ * Batched small symmetric eigensolver (FP64): the Rayleigh-Ritz step of many
+ * Davidson runs at once, the batched counterpart of the m x m eigh inside
+ * pyscf's lib.davidson1 (called at src/Solvers/davidson.py:277).
+ *
+ * For b < nbatch: the symmetric part of A_b = A_dev + b strideA (m x m, row
+ * stride lda) is diagonalised; w_dev + b strideW receives the eigenvalues
+ * ascending, V_dev + b strideV (row stride ldv) the eigenvectors as columns.
+ * 1 <= m <= 64; cyclic Jacobi, at most max_sweeps sweeps (10-20 suffice).
+ * ------------------------------------------------------------------------ */
+int mbpt_syev_batched_f64(int nbatch, int m, const double *A_dev, int lda,
+                          long strideA, double *w_dev, long strideW,
+                          double *V_dev, int ldv, long strideV, int max_sweeps,
+                          void *stream);
+
+/* ------------------------------------------------------------------------
+ * Batched sigma-vectors of n independent ISDF instances of uneven size, the
+ * TDA `apply_AB` (block_actions.py:563-661) of every instance in one pass.
+ * The reference has no batched path: it would run n separate closures.
+ *
+ * Each stage (head, Hartree, exchange per 16-wide occupied bucket, tail) is
+ * one launch over all instances driven by a device work table; the launch
+ * sequence is fixed per handle, so it can be captured in a hipGraph.
+ * ------------------------------------------------------------------------ */
+typedef struct mbpt_batch mbpt_batch;
+
+/* n instances: M[b], nocc[b] (<= 64), nvir[b]; X_dev[b] (M, nocc + nvir),
+ * W_dev[b] and V_dev[b] (M, M) device FP64 (V unread when kappa == 0);
+ * eps_host the n orbital-energy vectors concatenated; precision MBPT_FP32 or
+ * MBPT_FP64; kmax trial vectors per instance per apply.  The operands are
+ * copied.  A default plan is made (see mbpt_batch_plan). */
+int mbpt_batch_create(mbpt_batch **out, int n, const int *M, const int *nocc,
+                      const int *nvir, int precision, double kappa,
+                      const double *const *X_dev, const double *const *W_dev,
+                      const double *const *V_dev, const double *eps_host,
+                      int kmax, void *stream);
+
+/* Work decomposition (the tuning knobs; results do not depend on them
+ * beyond rounding): grid rows per exchange work item (>= 64), K chunk of
+ * the tail product T' X_v and of the Hartree product p V (>= 16), and the
+ * persistent grid size of every launch (0: one workgroup per work item).
+ * Synchronises the stream; reallocates the workspace, so a graph captured
+ * before must be captured again. */
+int mbpt_batch_plan(mbpt_batch *h, int rows_per_split, int tail_kchunk,
+                    int hartree_kchunk, int grid_cap, void *stream);
+
+/* out_dev (n, npair_max) FP64: row b, first nocc[b] nvir[b] entries, gets
+ * instance b's screened diagonal d - (ii|W|aa) (`screened_diagonal()`,
+ * block_actions.py:663-689); the padding is not written.  Synchronises. */
+int mbpt_batch_screened_diag(mbpt_batch *h, double *out_dev, void *stream);
+
+/* Largest nocc * nvir over the instances: the pair stride of the vectors. */
+int mbpt_batch_npair_max(const mbpt_batch *h);
+size_t mbpt_batch_held_bytes(const mbpt_batch *h);
+
+/* Az = A z for every instance: z_dev and Az_dev (n, kmax, npair_max) FP64,
+ * instance b's (nocc, nvir) pairs first in each row; kact_dev (n) int32 on
+ * the device (NULL: all kmax) gives the live vectors of each instance, the
+ * leading kact[b] rows -- rows after them are neither read nor written, nor
+ * is the pair padding. */
+int mbpt_batch_apply(mbpt_batch *h, const double *z_dev, double *Az_dev,
+                     const int *kact_dev, void *stream);
+int mbpt_batch_destroy(mbpt_batch *h);
+
+/* ------------------------------------------------------------------------
  * Learned Davidson preconditioner: a 3 -> 256 -> 256 -> 1 SiLU network, BF16
  * weights and activations, FP32 accumulation.  No counterpart in the
  * reference: it stands in for the diagonal correction r / (d - theta)
@@ -176,6 +240,17 @@ int mbpt_mlp_train_step(mbpt_mlp *h, long N, const float *X_dev,
 int mbpt_mlp_correction(mbpt_mlp *h, int n, int k, const double *R_dev,
                         int ldr, const double *d_dev, const double *theta_dev,
                         double *T_dev, int ldt, void *stream);
+/* The same for a batch of Davidson runs: nbatch instances, each a block of P
+ * rows of R (row stride ldr, k columns), d and T (row stride ldt), of which
+ * the first npair[b] are real; off_dev (nbatch + 1) int32 on the device is
+ * the prefix sum of npair, ntot = off[nbatch].  theta_dev (nbatch, k); rms
+ * per instance and column over its real rows.  Rows past npair[b] of T are
+ * not written. */
+int mbpt_mlp_correction_batched(mbpt_mlp *h, int nbatch, int P, int k,
+                                const int *off_dev, long ntot,
+                                const double *R_dev, int ldr,
+                                const double *d_dev, const double *theta_dev,
+                                double *T_dev, int ldt, void *stream);
 int mbpt_mlp_destroy(mbpt_mlp *h);
 
 #ifdef __cplusplus

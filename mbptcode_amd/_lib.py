@@ -27,12 +27,17 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
            'mbpt_mlp_nparams', 'mbpt_mlp_create', 'mbpt_mlp_get_params',
            'mbpt_mlp_forward', 'mbpt_mlp_gradient', 'mbpt_mlp_train_step',
-           'mbpt_mlp_correction', 'mbpt_mlp_destroy')
+           'mbpt_mlp_correction', 'mbpt_mlp_correction_batched', 'mbpt_mlp_destroy',
+           'mbpt_syev_batched_f64',
+           'mbpt_batch_create', 'mbpt_batch_plan', 'mbpt_batch_screened_diag',
+           'mbpt_batch_npair_max',
+           'mbpt_batch_held_bytes', 'mbpt_batch_apply', 'mbpt_batch_destroy')
 
 _lib = None
 
 _vp, _i, _d, _dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double)
 _fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
 _SIGNATURES = {
     'mbpt_last_error': (ctypes.c_char_p, []),
     'mbpt_abi_version': (_i, []),
@@ -60,7 +65,20 @@ _SIGNATURES = {
     'mbpt_mlp_gradient': (_i, [_vp, ctypes.c_long, _vp, _vp, _vp, _fp, _vp]),
     'mbpt_mlp_train_step': (_i, [_vp, ctypes.c_long, _vp, _vp, ctypes.c_float, _fp, _vp]),
     'mbpt_mlp_correction': (_i, [_vp, _i, _i, _vp, _i, _vp, _vp, _vp, _i, _vp]),
+    'mbpt_mlp_correction_batched': (_i, [_vp, _i, _i, _i, _vp, ctypes.c_long, _vp, _i, _vp,
+                                         _vp, _vp, _i, _vp]),
     'mbpt_mlp_destroy': (_i, [_vp]),
+    'mbpt_syev_batched_f64': (_i, [_i, _i, _vp, _i, ctypes.c_long, _vp, ctypes.c_long,
+                                   _vp, _i, ctypes.c_long, _i, _vp]),
+    'mbpt_batch_create': (_i, [ctypes.POINTER(_vp), _i, _ip, _ip, _ip, _i, _d,
+                               ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                               ctypes.POINTER(_vp), _dp, _i, _vp]),
+    'mbpt_batch_plan': (_i, [_vp, _i, _i, _i, _i, _vp]),
+    'mbpt_batch_screened_diag': (_i, [_vp, _vp, _vp]),
+    'mbpt_batch_npair_max': (_i, [_vp]),
+    'mbpt_batch_held_bytes': (ctypes.c_size_t, [_vp]),
+    'mbpt_batch_apply': (_i, [_vp, _vp, _vp, _vp, _vp]),
+    'mbpt_batch_destroy': (_i, [_vp]),
 }
 
 

@@ -63,14 +63,43 @@ __global__ void k_mlp_pack(const float *__restrict__ P, bf16x8 *__restrict__ fwd
 }
 
 // Inputs of one row: explicit features, or a Davidson residual block.
+// Batched residual rows (mode 2): nb instances whose pair blocks are P rows
+// apart in R, d and T, with off[b] the prefix sum of their real pair counts;
+// network row = j ntot + g over real pairs g only, instance b = the block
+// holding g.  theta and rms are (nb, k).
+struct Batch {
+  const int *off;
+  long ntot;
+  int nb, P, k;
+  __device__ void locate(long row, long &pos, int &col) const {
+    col = (int)(row / ntot);
+    const int g = (int)(row % ntot);
+    int lo = 0, hi = nb;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (off[mid] <= g) lo = mid; else hi = mid;
+    }
+    pos = (long)lo * P + (g - off[lo]);
+    col += lo * k;   // index into theta / rms; the column itself is col % k
+  }
+};
+
 struct Feat {
   const float *X;                  // mode 0: (N, 3)
   const double *R;                 // mode 1: residuals (n, k), row stride ldr
   const double *d, *theta, *rms;   // (n), (k), (k)
   int n, ldr;
+  Batch bt;                        // mode 2 when bt.off is set
   __device__ void get(long row, float (&x)[3]) const {
     if (X) {
       x[0] = X[row * 3]; x[1] = X[row * 3 + 1]; x[2] = X[row * 3 + 2];
+    } else if (bt.off) {
+      long pos;
+      int c;
+      bt.locate(row, pos, c);
+      x[0] = (float)(R[pos * ldr + c % bt.k] / rms[c]);
+      x[1] = (float)(d[pos] - theta[c]);
+      x[2] = (float)theta[c];
     } else {
       const int j = (int)(row / n), i = (int)(row % n);
       x[0] = (float)(R[(long)i * ldr + j] / rms[j]);
@@ -85,9 +114,15 @@ struct Out {
   double *T;                       // mode 1: corrections (n, k), row stride ldt
   const double *rms;
   int n, ldt;
+  Batch bt;                        // mode 2 when bt.off is set
   __device__ void put(long row, float v) const {
     if (y) {
       y[row] = v;
+    } else if (bt.off) {
+      long pos;
+      int c;
+      bt.locate(row, pos, c);
+      T[pos * ldt + c % bt.k] = (double)v * rms[c];
     } else {
       const int j = (int)(row / n), i = (int)(row % n);
       T[(long)i * ldt + j] = (double)v * rms[j];
@@ -384,6 +419,24 @@ __global__ void k_col_rms(int n, int k, const double *__restrict__ R, int ldr,
   if (threadIdx.x == 0) rms[j] = sqrt((ws[0] + ws[1] + ws[2] + ws[3]) / n) + 1e-300;
 }
 
+// rms per (instance, column) over the instance's real pairs, mode-2 layout
+__global__ void k_col_rms_batched(Batch bt, const double *__restrict__ R, int ldr,
+                                  double *__restrict__ rms) {
+  const int b = blockIdx.x / bt.k, j = blockIdx.x % bt.k;
+  const int n = bt.off[b + 1] - bt.off[b];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double v = R[((long)b * bt.P + i) * ldr + j];
+    s += v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+  __shared__ double ws[4];
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    rms[blockIdx.x] = sqrt((ws[0] + ws[1] + ws[2] + ws[3]) / (n > 0 ? n : 1)) + 1e-300;
+}
+
 constexpr size_t kFwdSmem = sizeof(bf16x8) * KB * NT * 64 + sizeof(float) * 6 * H;
 constexpr size_t kBwdSmem = sizeof(bf16x8) * KB * NT * 64 + sizeof(float) * (5 * H + NPART);
 constexpr int kRowBlocks = 256;   // persistent workgroups for the row kernels
@@ -568,6 +621,35 @@ int mbpt_mlp_correction(mbpt_mlp *h, int n, int k, const double *R_dev, int ldr,
   const long N = (long)n * k;
   Feat f{nullptr, R_dev, d_dev, theta_dev, h->rms, n, ldr};
   Out o{nullptr, T_dev, h->rms, n, ldt};
+  k_mlp_fwd<<<grid_rows(N), 256, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o, nullptr, nullptr);
+  return launch_status("k_mlp_fwd");
+}
+
+int mbpt_mlp_correction_batched(mbpt_mlp *h, int nbatch, int P, int k, const int *off_dev,
+                                long ntot, const double *R_dev, int ldr, const double *d_dev,
+                                const double *theta_dev, double *T_dev, int ldt,
+                                void *stream) {
+  MBPT_REQUIRE(h && nbatch > 0 && P > 0 && k >= 0 && ntot >= 0,
+               "mbpt_mlp_correction_batched: bad arguments");
+  if (k == 0 || ntot == 0) return MBPT_OK;
+  MBPT_REQUIRE(off_dev && R_dev && d_dev && theta_dev && T_dev && ldr >= k && ldt >= k,
+               "mbpt_mlp_correction_batched: bad arguments");
+  MBPT_REQUIRE(ntot * k < (1L << 40), "mbpt_mlp_correction_batched: too many rows");
+  hipStream_t st = (hipStream_t)stream;
+  const int nk = nbatch * k;
+  if (nk > h->cap_k) {
+    (void)hipFree(h->rms);
+    h->rms = nullptr;
+    MBPT_HIP_TRY(hipMalloc(&h->rms, sizeof(double) * nk));
+    h->cap_k = nk;
+  }
+  const Batch bt{off_dev, ntot, nbatch, P, k};
+  k_col_rms_batched<<<nk, 256, 0, st>>>(bt, R_dev, ldr, h->rms);
+  int rc = launch_status("k_col_rms_batched");
+  if (rc) return rc;
+  const long N = ntot * k;
+  Feat f{nullptr, R_dev, d_dev, theta_dev, h->rms, 0, ldr, bt};
+  Out o{nullptr, T_dev, h->rms, 0, ldt, bt};
   k_mlp_fwd<<<grid_rows(N), 256, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o, nullptr, nullptr);
   return launch_status("k_mlp_fwd");
 }

@@ -188,6 +188,35 @@ class LearnedPreconditioner:
             T += jacobi_davidson_correction(R, U, dvec, th)
         return T
 
+    def correction_batched(self, R, U, theta, diag, npair, jd=None):
+        """Corrections (B, P, k) for a batch of Davidson runs: R, U (B, P, k),
+        theta (B, k), diag (B, P), npair (B,) real pairs per instance (the
+        rest of each P block is padding and stays zero).  With the network
+        trained as a remainder over Jacobi-Davidson, `jd` (the JD corrections,
+        computed when not given) is added."""
+        B, P, k = R.shape
+        R = R.contiguous()
+        T = torch.zeros_like(R)
+        key = (R.device, tuple(int(n) for n in npair))
+        if getattr(self, '_off_key', None) != key:
+            off = np.concatenate([[0], np.cumsum(np.asarray(npair, np.int64))])
+            self._off = torch.as_tensor(off.astype(np.int32), device=R.device)
+            self._ntot = int(off[-1])
+            self._off_key = key
+        ntot = self._ntot
+        d = diag.to(torch.float64).contiguous()
+        th = theta.to(torch.float64).contiguous()
+        _lib.check(self._lib.mbpt_mlp_correction_batched(
+            self._h, B, P, k, _lib.dptr(self._off), ntot, _lib.dptr(R), k, _lib.dptr(d),
+            _lib.dptr(th), _lib.dptr(T), k, _lib.stream_ptr()), 'mbpt_mlp_correction_batched')
+        self.calls += 1
+        if self.residual_of_jd and U is not None:
+            if jd is None:
+                from .batched import jd_correction_batched
+                jd = jd_correction_batched(R, U, d, th)
+            T += jd
+        return T
+
 
 # ---------------------------------------------------------------------------
 # training data from plain FP64 Davidson runs
