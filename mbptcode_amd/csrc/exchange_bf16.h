@@ -11,7 +11,7 @@
 // A[l%16][8(l/16)+s] and B[8(l/16)+s][l%16] in slot s = 0..7 of its bf16x8
 // operands; the 16x16 FP32 result leaves D[4(l/16)+r][l%16] in element r.
 //
-// Per wave: 16 grid columns nu.  Per 32 grid rows mu (two 16-row P tiles):
+// Per 16 grid columns nu and 32 grid rows mu (two 16-row P tiles):
 //   P_t (16 mu x 16 nu) = X_o[mu, :] zXv[b][:, nu]          K = NP, 3 MFMAs/32
 //   S = P o W                                               FP32, in registers
 //   T[:, nu] += X_o[mu, :]^T S                              K = 32 mu, 3 MFMAs
@@ -49,125 +49,188 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &
   }
 }
 
-// X_o split once per handle: A-side [mu][NP] and transposed [NP][M] copies
-__global__ void k_split_bf16_xo(int M, int no, int np, const float *__restrict__ Xo,
-                                __bf16 *__restrict__ xh, __bf16 *__restrict__ xl,
-                                __bf16 *__restrict__ xth, __bf16 *__restrict__ xtl) {
-  const long n = (long)M * np;
+// ---- X_o in MFMA fragment order -------------------------------------------------
+// The two products read X_o as A operands in two shapes: X_o[mu][j] for P
+// (K = j) and X_o^T[i][mu] for T (K = mu, in the slot order pi above).  Both
+// are packed once per handle, split into hi and lo, so that every operand a
+// lane needs is one 16-byte slot and a wave's fragment is 1 KB contiguous.
+// Per 32-row chunk of the grid, in 16-byte slots:
+//   P part  [tt < 2][kb < NP/32][hl < 2][lane]  rows 16 tt + l%16, j = 32 kb + 8 (l/16) + s
+//   T part  [it < NP/16][hl < 2][lane]          rows pi(l/16, s),  i = 16 it + l%16
+// rows >= M and columns >= nocc are zero.
+__host__ __device__ constexpr int bf16_chunk_slots(int np) { return 64 * (4 * (np / 32) + 2 * (np / 16)); }
+
+__global__ void k_pack_bf16_xo(int M, int no, int np, const float *__restrict__ Xo,
+                               bf16x8 *__restrict__ frag) {
+  const int nkb = np / 32, spc = bf16_chunk_slots(np), toff = 64 * 4 * nkb;
+  const long n = (long)((M + 31) / 32) * spc;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
        e += (long)gridDim.x * blockDim.x) {
-    const int mu = (int)(e / np), j = (int)(e % np);
-    const float x = j < no ? Xo[(long)mu * no + j] : 0.f;
-    const __bf16 h = (__bf16)x;
-    const __bf16 l = (__bf16)(x - (float)h);
-    xh[e] = h;
-    xl[e] = l;
-    xth[(long)j * M + mu] = h;
-    xtl[(long)j * M + mu] = l;
+    const int chunk = (int)(e / spc), rem = (int)(e % spc);
+    int lane, hl, rowbase, jbase, mode;
+    if (rem < toff) {
+      lane = rem & 63;
+      const int g = rem >> 6;
+      hl = g & 1;
+      const int kb = (g >> 1) % nkb, tt = (g >> 1) / nkb;
+      rowbase = 32 * chunk + 16 * tt + (lane & 15);
+      jbase = 32 * kb + 8 * (lane >> 4);
+      mode = 0;
+    } else {
+      const int r2 = rem - toff;
+      lane = r2 & 63;
+      const int g = r2 >> 6;
+      hl = g & 1;
+      rowbase = 32 * chunk;
+      jbase = 16 * (g >> 1) + (lane & 15);
+      mode = 1;
+    }
+    bf16x8 out;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int q = lane >> 4;
+      const int row = mode == 0 ? rowbase : rowbase + (s < 4 ? 4 * q + s : 16 + 4 * q + (s - 4));
+      const int j = mode == 0 ? jbase + s : jbase;
+      const float x = (row < M && j < no) ? Xo[(long)row * no + j] : 0.f;
+      const __bf16 h = (__bf16)x;
+      out[s] = hl == 0 ? h : (__bf16)(x - (float)h);
+    }
+    frag[e] = out;
   }
 }
 
+// ---- the exchange ----------------------------------------------------------------
+// Workgroup = 4 waves x 32 grid columns (two 16-column tiles per wave, so each
+// X_o fragment read from LDS feeds two MFMA chains), one vector b, one split
+// of the grid rows.  The packed X_o chunks stream through a double-buffered
+// LDS stage (plain 16-byte copies), and the next chunk's W values are loaded
+// into registers while the current chunk computes.
 template <int NP>
 __global__ __launch_bounds__(256) void k_exchange_bf16(
-    int M, int no, int k, const __bf16 *__restrict__ xh, const __bf16 *__restrict__ xl,
-    const __bf16 *__restrict__ xth, const __bf16 *__restrict__ xtl,
-    const float *__restrict__ W, const float *__restrict__ zXv,
-    float *__restrict__ Tpart, int nsplit, int rows_per_split, int nvb, int nblk) {
+    int M, int no, int k, const bf16x8 *__restrict__ frag, const float *__restrict__ W,
+    const float *__restrict__ zXv, float *__restrict__ Tpart, int nsplit,
+    int rows_per_split, int nvb, int nblk) {
+  constexpr int NKB = NP / 32, NI = NP / 16, C = 2;
+  constexpr int SPC = bf16_chunk_slots(NP), TOFF = 64 * 4 * NKB, LPT = SPC / 256;
+  __shared__ bf16x8 buf[2][SPC];
   const int lid = xcd_logical(blockIdx.x, nblk);
   if (lid >= nvb * k * nsplit) return;
   const int b = lid % k, t = lid / k, vb = t % nvb, s = t / nvb;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lr = lane & 15, lq = lane >> 4;
-  const int col = vb * 64 + wave * 16 + lr;
-  const bool colok = col < M;
-  constexpr int NKB = NP / 32, NI = NP / 16, ROWS = 64;
-  constexpr int LDA = NP + 8;     // bf16 per staged X_o row (16 B pad)
-  constexpr int LDT = ROWS + 4;   // bf16 per staged X_o^T row (8 B pad)
-  __shared__ __bf16 sah[ROWS * LDA], sal[ROWS * LDA];
-  __shared__ __bf16 sth[NP * LDT], stl[NP * LDT];
+  const int cbase = vb * 128 + wave * 32;
 
   // zXv[b][:, col] as the B operand of the P products, split once
-  bf16x8 bzh[NKB], bzl[NKB];
+  bf16x8 bzh[C][NKB], bzl[C][NKB];
 #pragma unroll
-  for (int kb = 0; kb < NKB; ++kb) {
-    float v[8];
+  for (int cc = 0; cc < C; ++cc) {
+    const int col = cbase + 16 * cc + lr;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int j = 32 * kb + 8 * lq + q;
-      v[q] = (j < no && colok) ? zXv[((long)b * no + j) * M + col] : 0.f;
+    for (int kb = 0; kb < NKB; ++kb) {
+      float v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = 32 * kb + 8 * lq + q;
+        v[q] = (j < no && col < M) ? zXv[((long)b * no + j) * M + col] : 0.f;
+      }
+      split8(v, bzh[cc][kb], bzl[cc][kb]);
     }
-    split8(v, bzh[kb], bzl[kb]);
   }
-  f32x4 acc[NI];
+  f32x4 acc[NI][C];
 #pragma unroll
-  for (int it = 0; it < NI; ++it) zero(acc[it]);
+  for (int it = 0; it < NI; ++it)
+#pragma unroll
+    for (int cc = 0; cc < C; ++cc) zero(acc[it][cc]);
 
-  const int mu_beg = s * rows_per_split;
-  const int mu_end = min(M, mu_beg + rows_per_split);
-  for (int mg = mu_beg; mg < mu_end; mg += ROWS) {
-    __syncthreads();
-    // stage 64 rows of X_o (hi, lo) in both layouts; rows past mu_end are 0
-    for (int e = tid; e < ROWS * NP; e += 256) {
-      const int r = e / NP, j = e % NP, mu = mg + r;
-      const bool ok = mu < mu_end;
-      sah[r * LDA + j] = ok ? xh[(long)mu * NP + j] : (__bf16)0.f;
-      sal[r * LDA + j] = ok ? xl[(long)mu * NP + j] : (__bf16)0.f;
-    }
-    for (int e = tid; e < NP * ROWS; e += 256) {
-      const int j = e / ROWS, r = e % ROWS, mu = mg + r;
-      const bool ok = mu < mu_end;
-      sth[j * LDT + r] = ok ? xth[(long)j * M + mu] : (__bf16)0.f;
-      stl[j * LDT + r] = ok ? xtl[(long)j * M + mu] : (__bf16)0.f;
-    }
-    __syncthreads();
+  const int c_beg = (s * rows_per_split) / 32;
+  const int c_end = min((M + 31) / 32, ((s + 1) * rows_per_split) / 32);
+  auto load_w = [&](int c, float (&w)[2][C][4]) {
 #pragma unroll
-    for (int c = 0; c < ROWS / 32; ++c) {
-      const int r0 = 32 * c;
-      if (mg + r0 >= mu_end) break;
-      float sv[8];
+    for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-      for (int tt = 0; tt < 2; ++tt) {
-        f32x4 P;
-        zero(P);
-        const int row = r0 + 16 * tt + lr;
-#pragma unroll
-        for (int kb = 0; kb < NKB; ++kb) {
-          const bf16x8 ah = *(const bf16x8 *)&sah[row * LDA + 32 * kb + 8 * lq];
-          const bf16x8 al = *(const bf16x8 *)&sal[row * LDA + 32 * kb + 8 * lq];
-          P = mma3(ah, al, bzh[kb], bzl[kb], P);
-        }
+      for (int cc = 0; cc < C; ++cc) {
+        const int col = cbase + 16 * cc + lr;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int mu = mg + r0 + 16 * tt + 4 * lq + r;
-          const float w = (mu < mu_end && colok) ? W[(long)mu * M + col] : 0.f;
-          sv[4 * tt + r] = P[r] * w;
+          const int mu = 32 * c + 16 * tt + 4 * lq + r;
+          w[tt][cc][r] = (mu < M && col < M) ? W[(long)mu * M + col] : 0.f;
         }
       }
-      bf16x8 sh, sl;
-      split8(sv, sh, sl);
+  };
+  if (c_beg < c_end) {
+    const bf16x8 *src = frag + (long)c_beg * SPC;
 #pragma unroll
-      for (int it = 0; it < NI; ++it) {
-        const int row = (16 * it + lr) * LDT + r0 + 4 * lq;
-        bf16x8 ah, al;
-        const bf16x4 h0 = *(const bf16x4 *)&sth[row];
-        const bf16x4 h1 = *(const bf16x4 *)&sth[row + 16];
-        const bf16x4 l0 = *(const bf16x4 *)&stl[row];
-        const bf16x4 l1 = *(const bf16x4 *)&stl[row + 16];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          ah[q] = h0[q]; ah[4 + q] = h1[q];
-          al[q] = l0[q]; al[4 + q] = l1[q];
-        }
-        acc[it] = mma3(ah, al, sh, sl, acc[it]);
-      }
-    }
+    for (int q = 0; q < LPT; ++q) buf[0][tid + 256 * q] = src[tid + 256 * q];
   }
-  if (colok) {
+  float w[2][C][4];
+  load_w(c_beg, w);
+  __syncthreads();
+  int cur = 0;
+  for (int c = c_beg; c < c_end; ++c) {
+    const bool more = c + 1 < c_end;
+    bf16x8 stage[LPT];
+    float wn[2][C][4];
+    if (more) {
+      const bf16x8 *src = frag + (long)(c + 1) * SPC;
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) stage[q] = src[tid + 256 * q];
+      load_w(c + 1, wn);
+    }
+    const bf16x8 *fb = buf[cur];
+    f32x4 P[2][C];
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc) zero(P[tt][cc]);
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int kb = 0; kb < NKB; ++kb) {
+        const bf16x8 ah = fb[((tt * NKB + kb) * 2 + 0) * 64 + lane];
+        const bf16x8 al = fb[((tt * NKB + kb) * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc)
+          P[tt][cc] = mma3(ah, al, bzh[cc][kb], bzl[cc][kb], P[tt][cc]);
+      }
+    bf16x8 sh[C], sl[C];
+#pragma unroll
+    for (int cc = 0; cc < C; ++cc) {
+      float sv[8];
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) sv[4 * tt + r] = P[tt][cc][r] * w[tt][cc][r];
+      split8(sv, sh[cc], sl[cc]);
+    }
+#pragma unroll
+    for (int it = 0; it < NI; ++it) {
+      const bf16x8 ah = fb[TOFF + (it * 2 + 0) * 64 + lane];
+      const bf16x8 al = fb[TOFF + (it * 2 + 1) * 64 + lane];
+#pragma unroll
+      for (int cc = 0; cc < C; ++cc) acc[it][cc] = mma3(ah, al, sh[cc], sl[cc], acc[it][cc]);
+    }
+    if (more) {
+#pragma unroll
+      for (int q = 0; q < LPT; ++q) buf[cur ^ 1][tid + 256 * q] = stage[q];
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int cc = 0; cc < C; ++cc)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w[tt][cc][r] = wn[tt][cc][r];
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int cc = 0; cc < C; ++cc) {
+    const int col = cbase + 16 * cc + lr;
+    if (col >= M) continue;
 #pragma unroll
     for (int it = 0; it < NI; ++it)
 #pragma unroll
       for (int r = 0; r < 4; ++r)
-        Tpart[(((long)s * k + b) * NP + 16 * it + 4 * lq + r) * M + col] = acc[it][r];
+        Tpart[(((long)s * k + b) * NP + 16 * it + 4 * lq + r) * M + col] = acc[it][cc][r];
   }
 }
 

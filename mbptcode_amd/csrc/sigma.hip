@@ -342,7 +342,7 @@ struct mbpt_sigma {
   double kappa;
   size_t esz;              // bytes per working-precision element
   void *Xo, *Xv, *XoT, *W, *V;
-  __bf16 *xh, *xl, *xth, *xtl;   // split X_o (MBPT_BF16X3): [M][nopad], [nopad][M]
+  bf16x8 *frag;            // X_o split and packed in MFMA fragment order (MBPT_BF16X3)
   double *d;
   // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
   void *ws[6];
@@ -436,11 +436,11 @@ void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_
 template <int NP>
 void launch_exchange_bf16(const mbpt_sigma *h, int k, int nsplit, int rps,
                           hipStream_t st) {
-  const int nvb = (h->M + 63) / 64;
+  const int nvb = (h->M + 127) / 128;
   const int nblk = pad_grid((long)nvb * k * nsplit);
   k_exchange_bf16<NP><<<nblk, 256, 0, st>>>(
-      h->M, h->no, k, h->xh, h->xl, h->xth, h->xtl, (const float *)h->W,
-      (const float *)h->ws[WS_ZXV], (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
+      h->M, h->no, k, h->frag, (const float *)h->W, (const float *)h->ws[WS_ZXV],
+      (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
 
 template <typename T>
@@ -618,19 +618,16 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
                                   : create_impl<float>(h, X_dev, W_dev, V_dev, st);
   if (rc) return fail(rc);
   if (precision == MBPT_BF16X3) {
-    const size_t nb = sizeof(__bf16) * (size_t)M * h->nopad;
-    __bf16 **bufs[4] = {&h->xh, &h->xl, &h->xth, &h->xtl};
-    for (__bf16 **b : bufs) {
-      he = hipMalloc((void **)b, nb);
-      if (he != hipSuccess) {
-        set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
-        return fail(MBPT_ERR_HIP);
-      }
+    const long slots = (long)((M + 31) / 32) * bf16_chunk_slots(h->nopad);
+    he = hipMalloc((void **)&h->frag, sizeof(bf16x8) * slots);
+    if (he != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
+      return fail(MBPT_ERR_HIP);
     }
-    h->held += 4 * nb;
-    k_split_bf16_xo<<<grid1((long)M * h->nopad), 256, 0, st>>>(
-        M, nocc, h->nopad, (const float *)h->Xo, h->xh, h->xl, h->xth, h->xtl);
-    if ((rc = launch_status("k_split_bf16_xo"))) return fail(rc);
+    h->held += sizeof(bf16x8) * slots;
+    k_pack_bf16_xo<<<grid1(slots), 256, 0, st>>>(M, nocc, h->nopad, (const float *)h->Xo,
+                                                  h->frag);
+    if ((rc = launch_status("k_pack_bf16_xo"))) return fail(rc);
   }
   *out = h;
   return MBPT_OK;
@@ -693,7 +690,7 @@ int mbpt_sigma_destroy(mbpt_sigma *h) {
     for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
   free_ws(h);
   (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
-  (void)hipFree(h->xh); (void)hipFree(h->xl); (void)hipFree(h->xth); (void)hipFree(h->xtl);
+  (void)hipFree(h->frag);
   (void)hipFree(h->d);
   delete h;
   return MBPT_OK;

@@ -261,8 +261,15 @@ def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
     C, info = torch.linalg.cholesky_ex(A)
     if int(info) != 0:
         raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
-    # W = L A^-1 L^T = (L C^-T)(L C^-T)^T
-    Y = torch.linalg.solve_triangular(C, L.T, upper=False).T   # L C^-T
+    # W = L A^-1 L^T = (L C^-T)(L C^-T)^T.  The triangular solve runs on
+    # column blocks of L^T: rocBLAS's trsm workspace under torch is bounded
+    # and a single solve with thousands of right-hand sides fails to get it.
+    Lt = L.T.contiguous()
+    Yt = torch.empty_like(Lt)
+    for j in range(0, n, 1024):
+        Yt[:, j:j + 1024] = torch.linalg.solve_triangular(C, Lt[:, j:j + 1024].contiguous(),
+                                                          upper=False)
+    Y = Yt.T                                                   # L C^-T
     W = Y @ Y.T
     return 0.5 * (W + W.T), chi0
 
