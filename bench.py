@@ -1,22 +1,31 @@
-"""Benchmark: Tamm-Dancoff BSE Davidson roots/s within 1 meV of FP64, and the
-chi0 build's MFMA fraction, on one MI355X.
+"""Benchmark: Tamm-Dancoff BSE Davidson roots/s within 1 meV of FP64 (learned
+preconditioner + mixed precision against plain FP64), and the chi0 build's
+MFMA fraction, on one MI355X.
 
-A step is one complete Davidson solve of the benzene/def2-SVP-size instance
-(21 occupied, 93 virtual, 1200 grid points, 10 lowest singlet roots,
-residual tol 1e-5) on device-resident inputs: sigma-vectors on the device
-kernels, the FP64 subspace work on the device, the FP64 residual recheck of
-the converged roots included.  The static W it needs (16 minimax time
-points, FP64 Cholesky Dyson step) is built once before the timed region and
-timed apart.
+Headline step: one complete Davidson solve of the learned-preconditioner
+instance (100 occupied, 700 virtual, 6000 grid points, 20 lowest singlet
+roots, residual tol 1e-5) on device-resident inputs:
 
-Printed: one JSON line (rank 0).  `value` is roots/s of the headline solver
-(mixed precision: FP32 MFMA sigma + FP64 subspace + FP64 residual recheck);
-`plain_fp64` is the same solve with FP64 sigma-vectors.  `roofline` is the
-exchange kernel (the sigma-vector's dominant launch) timed with HIP events
-inside the timed steps; `chi0_roofline` the fused chi0 kernel on the large
-single instance (60 occupied, 400 virtual, 4000 points, 20 time points).
-`cpu_baseline` is the FP64 NumPy oracle (the reference algorithm restated,
-one trial vector at a time) solving the same instance on the host cores.
+* learned+mixed (the headline `value`): split-BF16 MFMA sigma-vectors, FP64
+  subspace work on the device, the learned MLP correction (trained weights
+  shipped in mbptcode_amd/data) with the screened Jacobi-Davidson fallback,
+  and the FP64 recheck of every requested root (A x - theta x with FP64
+  sigma-vectors, iterated in FP64 until the FP64 residual meets the tolerance);
+* plain: the same Davidson with FP64 sigma-vectors and the screened
+  correction, timed the same way (`plain_fp64`), whose eigenvalues judge the
+  errors (`max_err_meV`, `err_meV_per_root`).
+
+The static W (FP64 chi0 at 20 minimax points, FP64 Cholesky Dyson step) is
+built once before the timed region and timed apart.  `roofline` is the
+split-BF16 exchange kernel, the dominant launch of the headline solve, timed
+with HIP events on its stream inside the timed steps.  Further legs:
+the benzene-size instance (plain / FP32 / split-BF16), the large instance
+(per-root errors of FP32 and split-BF16 against FP64, the RPA correlation
+energy error in micro-Hartree, and the fused chi0 kernel's MFMA fraction in
+`chi0_roofline`), and the batched workload (256 uneven instances in one
+captured batched pass per cycle).  `cpu_baseline` is the FP64 NumPy oracle
+(the reference algorithm restated: one trial vector at a time) solving the
+headline instance on the host cores.
 """
 import argparse
 import json
@@ -31,6 +40,8 @@ sys.path.insert(0, ROOT)
 
 HA_MEV = 27211.386245988
 PEAK_TFLOPS = {'fp64': 78.6, 'fp32': 157.3, 'bf16x3': 2500.0}
+KERNEL_NAMES = {'fp64': 'k_exchange<double>', 'fp32': 'k_exchange<float>',
+                'bf16x3': 'k_exchange_bf16'}
 
 
 def parse():
@@ -38,9 +49,13 @@ def parse():
     p.add_argument('--gpus', type=int, default=1)
     p.add_argument('--steps', type=int, default=5)
     p.add_argument('--warmup', type=int, default=2)
-    p.add_argument('--precision', default='fp32', choices=['fp32', 'fp64'])
+    p.add_argument('--config', type=int, default=4, choices=[1, 2, 4],
+                   help='single-instance workload of the headline')
+    p.add_argument('--precision', default='bf16x3', choices=['bf16x3', 'fp32'],
+                   help='sigma-vector precision of the mixed solver')
+    p.add_argument('--learned-mode', default='augment', choices=['augment', 'replace'])
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
-    p.add_argument('--no-chi0', action='store_true', help='skip the large chi0 leg')
+    p.add_argument('--no-legs', action='store_true', help='headline only')
     return p.parse_args()
 
 
@@ -48,15 +63,244 @@ def log(msg):
     print(f'[bench] {msg}', file=sys.stderr, flush=True)
 
 
-def solve(apply_AB, d, inst, refine_apply=None, timings=None):
-    from mbptcode_amd.davidson import run_davidson
-    return run_davidson(apply_AB, d, inst.nroots, inst.tol, 200, tda=True,
-                        preconditioner='screened', refine_apply=refine_apply,
-                        timings=timings)
+def sync():
+    import torch
+    torch.cuda.synchronize()
 
 
-def cpu_baseline(inst, W_grid, e_ref):
-    """The oracle's FP64 solve of the same instance on the host cores."""
+def build_problem(cfg, w_precision='fp64'):
+    from mbptcode_amd import grids, synth
+    from mbptcode_amd.space_time import screened_interaction_grid
+    inst = synth.config(cfg)
+    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    sync()
+    t0 = time.perf_counter()
+    W, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g,
+                                     precision=w_precision)
+    sync()
+    return inst, W, time.perf_counter() - t0
+
+
+class Solver:
+    """One way of solving an instance: sigma precision, preconditioner,
+    FP64 recheck."""
+
+    def __init__(self, name, act, d, inst, refine=None, preconditioner='screened',
+                 learned=None, learned_mode='augment'):
+        self.name, self.act, self.d, self.inst = name, act, d, inst
+        self.refine, self.pre = refine, preconditioner
+        self.learned, self.mode = learned, learned_mode
+
+    def __call__(self, tol=None, timings=None):
+        from mbptcode_amd.davidson import run_davidson
+        return run_davidson(self.act, self.d, self.inst.nroots,
+                            self.inst.tol if tol is None else tol, 200, tda=True,
+                            preconditioner=self.pre, refine_apply=self.refine,
+                            learned=self.learned, learned_mode=self.mode,
+                            timings=timings)
+
+
+def time_solver(solver, steps, warmup, e_ref, profile_op=None):
+    """(roots/s counting roots within 1 meV of e_ref, ms/step, max error per
+    root (meV), timings of the last step, exchange profile)."""
+    for _ in range(warmup):
+        solver()
+    sync()
+    if profile_op is not None:
+        profile_op.profile(True)
+    good, errs, tims = 0, [], {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w, _, _ = solver(timings=tims)
+        err = np.abs(w - e_ref) * HA_MEV
+        good += int(np.sum(err <= 1.0))
+        errs.append(err)
+    sync()
+    dt = time.perf_counter() - t0
+    prof = None
+    if profile_op is not None:
+        prof = profile_op.profile_read()
+        profile_op.profile(False)
+    return good / dt, dt / steps * 1e3, np.max(errs, axis=0), tims, prof
+
+
+def exchange_roofline(prof, precision, kernel_shape):
+    ex_ms = prof['ms']['exchange'] / prof['launches']
+    ex_flops = prof['exchange_flops'] / prof['launches']
+    achieved = ex_flops / (ex_ms * 1e-3) / 1e12
+    return {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[precision],
+            'unit': 'TFLOP/s', 'frac': achieved / PEAK_TFLOPS[precision], 'traffic': None,
+            'kernel': KERNEL_NAMES[precision], 'shape': kernel_shape, 'avg_ms': ex_ms,
+            'launches': prof['launches'], 'flops_per_launch': ex_flops,
+            'flop_definition': 'k (4 nocc M^2 + M^2) per launch, k = trial vectors in '
+                               'the launch (algorithmic FP32-equivalent flops)',
+            'stage_ms_per_apply': {k: v / prof['launches'] for k, v in prof['ms'].items()}}
+
+
+def headline(args):
+    import torch
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.mlp_precond import LearnedPreconditioner
+    inst, W, w_s = build_problem(args.config)
+    log(f'headline instance cfg{args.config}: nocc={inst.nocc} nvir={inst.nvir} '
+        f'M={inst.M} roots={inst.nroots}; W built in {w_s:.2f} s')
+    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
+    actm, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V,
+                                precision=args.precision)
+    net = LearnedPreconditioner.load()
+    plain = Solver('plain_fp64', act64, d, inst)
+    mixed = Solver(f'{args.precision}+screened', actm, d, inst, refine=act64)
+    learned = Solver(f'{args.precision}+learned', actm, d, inst, refine=act64,
+                     preconditioner='learned', learned=net, learned_mode=args.learned_mode)
+    e_plain, _, _ = plain()
+    e_tight, _, _ = Solver('tight', act64, d, inst)(tol=1e-8)
+    log(f'roots (eV): {np.round(e_plain * 27.211386, 4).tolist()}')
+    out = {}
+    for s in (learned, mixed, plain):
+        prof_op = actm.operator if s is learned else None
+        rps, ms, err, tims, prof = time_solver(s, args.steps, args.warmup, e_plain, prof_op)
+        w, _, _ = s()
+        out[s.name] = {'roots_per_s': rps, 'ms_per_step': ms,
+                       'max_err_meV': float(err.max()),
+                       'err_meV_per_root': [float(x) for x in err],
+                       'max_err_vs_tight_fp64_meV': float(np.abs(w - e_tight).max() * HA_MEV),
+                       'cycles': tims.get('davidson_iterations'),
+                       'sigma_vectors': tims.get('davidson_vectors'),
+                       'fp64_recheck': tims.get('davidson_refine'),
+                       'learned_fallback': 'learned_fallback_cycle' in tims.get('davidson_flags', {})}
+        if prof is not None:
+            out['roofline'] = exchange_roofline(prof, args.precision,
+                                                f'nocc={inst.nocc}, M={inst.M}')
+        log(f'{s.name}: {rps:.1f} roots/s, {ms:.1f} ms/solve, max err {err.max():.2e} meV, '
+            f'{tims.get("davidson_iterations")} cycles, {tims.get("davidson_vectors")} vectors')
+    return inst, W, out, w_s
+
+
+def leg_benzene():
+    from mbptcode_amd.block_actions import grid_block_action
+    inst, W, _ = build_problem(1)
+    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
+    plain = Solver('plain_fp64', act64, d, inst)
+    e_plain, _, _ = plain()
+    res = {}
+    for p in ('fp64', 'fp32', 'bf16x3'):
+        if p == 'fp64':
+            s = plain
+        else:
+            act, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision=p)
+            s = Solver(p, act, d, inst, refine=act64)
+        rps, ms, err, tims, _ = time_solver(s, 5, 2, e_plain)
+        res[p] = {'roots_per_s': rps, 'ms_per_step': ms, 'max_err_meV': float(err.max()),
+                  'cycles': tims.get('davidson_iterations')}
+    log(f'benzene-size: ' + ', '.join(f'{k} {v["roots_per_s"]:.0f} roots/s' for k, v in res.items()))
+    return {'workload': 'nocc=21, nvir=93, M=1200, 10 roots, tol 1e-5', 'solvers': res}
+
+
+def leg_large():
+    """Per-root errors of FP32 and split-BF16 sigma against FP64, RPA E_c
+    error, and the chi0 kernel's MFMA fraction on the large instance."""
+    import torch
+    from mbptcode_amd import grids
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.space_time import Chi0Builder, rpa_correlation_energy_grid
+    inst, W, _ = build_problem(2)
+    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
+    e64, _, _ = Solver('fp64', act64, d, inst)()
+    res = {'workload': 'nocc=60, nvir=400, M=4000, 16 roots, tol 1e-5, 20 time points'}
+    for p in ('fp32', 'bf16x3'):
+        act, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision=p)
+        s = Solver(p, act, d, inst, refine=act64)
+        rps, ms, err, tims, _ = time_solver(s, 3, 1, e64)
+        res[p] = {'roots_per_s': rps, 'ms_per_step': ms,
+                  'err_meV_per_root': [float(x) for x in err], 'max_err_meV': float(err.max())}
+    s = Solver('fp64', act64, d, inst)
+    rps, ms, _, _, _ = time_solver(s, 3, 1, e64)
+    res['fp64'] = {'roots_per_s': rps, 'ms_per_step': ms}
+    grid = grids.rpa_energy_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    ec = {}
+    for p in ('fp64', 'fp32'):
+        sync()
+        t0 = time.perf_counter()
+        ec[p] = rpa_correlation_energy_grid(inst.X, inst.V, inst.eps, inst.nocc, grid,
+                                            precision=p)
+        sync()
+        ec[p + '_s'] = time.perf_counter() - t0
+    res['rpa_ec'] = {'fp64_Ha': ec['fp64'], 'fp32_Ha': ec['fp32'],
+                     'err_uHa': abs(ec['fp32'] - ec['fp64']) * 1e6,
+                     'seconds': {'fp64': ec['fp64_s'], 'fp32': ec['fp32_s']},
+                     'nfreq': grid.nfreq}
+    # chi0 kernel MFMA fraction: the static chi0(0) build, 20 time points
+    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    b = Chi0Builder(inst.X, inst.eps, inst.nocc, precision='fp32')
+    chi = b.build(g.tau_points, g.cosft_wt[0])
+    sync()
+    b.profile(True)
+    for _ in range(5):
+        b.build(g.tau_points, g.cosft_wt[0], out=chi)
+    sync()
+    pr = b.profile_read()
+    b.profile(False)
+    avg_ms = pr['ms'] / pr['launches']
+    achieved = pr['flops'] / pr['launches'] / (avg_ms * 1e-3) / 1e12
+    res['chi0_roofline'] = {
+        'kernel': 'k_chi0_prep + k_chi0_f32 + k_chi0_reduce (one build)', 'bound': 'mfma',
+        'achieved': achieved, 'peak': PEAK_TFLOPS['fp32'], 'unit': 'TFLOP/s',
+        'frac': achieved / PEAK_TFLOPS['fp32'], 'traffic': None, 'avg_ms': avg_ms,
+        'launches': pr['launches'], 'flops_per_launch': pr['flops'] / pr['launches'],
+        'workload': f'static chi0(0), M={inst.M}, nocc={inst.nocc}, nvir={inst.nvir}, '
+                    f'ntau={inst.ntau}',
+        'flop_definition': 'ntau M(M+1)(nocc+nvir+1): the upper triangle with diagonal'}
+    log(f'large: fp32 {res["fp32"]["max_err_meV"]:.1e} meV, bf16x3 '
+        f'{res["bf16x3"]["max_err_meV"]:.1e} meV, E_c err {res["rpa_ec"]["err_uHa"]:.2e} uHa, '
+        f'chi0 {achieved:.1f} TF')
+    return res
+
+
+def leg_batched():
+    """The batched workload: 256 instances, 5 roots each, tol 1e-5."""
+    import torch
+    from mbptcode_amd import synth, grids
+    from mbptcode_amd.batched import BatchedSigma, autotune, solve_batched
+    from mbptcode_amd.space_time import screened_interaction_grid
+    sizes = synth.batched_sizes(256)
+    insts = [synth.batched_instance(i, sizes) for i in range(256)]
+    Ws = []
+    for inst in insts:
+        g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+        Ws.append(screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g,
+                                            precision='fp32')[0])
+    args = ([i.eps for i in insts], [i.nocc for i in insts], [i.X for i in insts], Ws,
+            [i.V for i in insts])
+    op64 = BatchedSigma(*args, precision='fp64', kmax=10)
+    op32 = BatchedSigma(*args, precision='fp32', kmax=10)
+    tune = {}
+    for op in (op64, op32):
+        plan, info = autotune(op)
+        tune[op.precision] = {'plan': plan, 'pass_ms': info.get('ms'), 'source': info['source']}
+    runs = {}
+    e_plain = None
+    for name, op, ref in (('plain_fp64', op64, None), ('fp32+fp64_recheck', op32, op64)):
+        solve_batched(op, 5, 1e-5, refine_op=ref)          # warm-up
+        sync()
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            e, _, info = solve_batched(op, 5, 1e-5, refine_op=ref)
+        sync()
+        dt = (time.perf_counter() - t0) / reps
+        if e_plain is None:
+            e_plain = e
+        err = float(np.abs(e - e_plain).max() * HA_MEV)
+        runs[name] = {'roots_per_s': 256 * 5 / dt, 'ms_per_solve': dt * 1e3,
+                      'cycles': info['cycles'], 'refine_cycles': info.get('refine_cycles'),
+                      'converged': info['converged'], 'max_err_meV': err}
+    log('batched: ' + ', '.join(f'{k} {v["roots_per_s"]:.0f} roots/s' for k, v in runs.items()))
+    return {'workload': '256 instances, nocc~U[5,40], nvir~U[20,200], M=8(nocc+nvir), '
+                        '5 roots, tol 1e-5', 'solvers': runs, 'plan': tune}
+
+
+def cpu_baseline(inst, W_grid, e_ref, vectors):
+    """The oracle's FP64 solve of the headline instance on the host cores."""
     from oracle import isdf_oracle as O
     try:
         from threadpoolctl import threadpool_info
@@ -70,54 +314,22 @@ def cpu_baseline(inst, W_grid, e_ref):
     W_aux = 0.5 * (W_aux + W_aux.T)
     apply_A, d = O.isdf_block_action(inst.eps, inst.nocc, inst.X, D, W_aux)
     no, nv = inst.nocc, inst.nvir
-
-    def mm(V):
-        return apply_A(V.T.reshape(-1, no, nv)).reshape(V.shape[1], -1).T
-
+    # bounded sample: the per-vector cost of the reference's action, timed on
+    # a few trial vectors, scaled by the sigma-vectors the FP64 solve needs
+    rng = np.random.default_rng(0)
+    z = rng.normal(size=(4, no, nv))
+    apply_A(z[:1])
     t0 = time.perf_counter()
-    sd = apply_A.screened_diagonal()
-    e, X, conv, napplied = O.davidson_tda(mm, d, inst.nroots, tol=inst.tol,
-                                          precond_diag=sd)
-    dt = time.perf_counter() - t0
-    good = int(np.sum(np.abs(e - e_ref) * HA_MEV <= 1.0))
-    return {'value': good / dt, 'unit': 'roots/s', 'cores': cores, 'kind': 'port',
-            'sample': (f'one FP64 Davidson solve of the same instance '
-                       f'({napplied} sigma-vectors, one at a time as the '
-                       f'reference does), {dt:.2f} s, {good}/{inst.nroots} roots '
-                       f'within 1 meV'),
-            'blas': blas, 'seconds': dt}
-
-
-def chi0_leg(reps=3):
-    """MFMA fraction of the fused chi0 kernel on the large single instance."""
-    import torch
-    from mbptcode_amd import grids, synth
-    from mbptcode_amd.space_time import Chi0Builder
-    inst = synth.config(2)
-    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
-    out = {}
-    for precision in ('fp32',):
-        b = Chi0Builder(inst.X, inst.eps, inst.nocc, precision=precision)
-        chi = b.build(g.tau_points, g.cosft_wt[0])      # warm
-        torch.cuda.synchronize()
-        b.profile(True)
-        for _ in range(reps):
-            b.build(g.tau_points, g.cosft_wt[0], out=chi)
-        torch.cuda.synchronize()
-        pr = b.profile_read()
-        b.profile(False)
-        avg_ms = pr['ms'] / pr['launches']
-        achieved = pr['flops'] / pr['launches'] / (avg_ms * 1e-3) / 1e12
-        M, no, nv = inst.M, inst.nocc, inst.nvir
-        out = {'kernel': 'k_chi0<float>', 'bound': 'mfma', 'achieved': achieved,
-               'peak': PEAK_TFLOPS[precision], 'unit': 'TFLOP/s',
-               'frac': achieved / PEAK_TFLOPS[precision], 'traffic': None,
-               'avg_ms': avg_ms, 'launches': pr['launches'],
-               'workload': f'static chi0(0), M={M}, nocc={no}, nvir={nv}, '
-                           f'ntau={inst.ntau}',
-               'flops_per_launch': pr['flops'] / pr['launches'],
-               'flop_definition': 'ntau * M(M+1)(nocc+nvir+1): upper triangle'}
-    return out
+    apply_A(z)
+    per_vec = (time.perf_counter() - t0) / 4
+    est = per_vec * vectors
+    return {'value': inst.nroots / est, 'unit': 'roots/s', 'cores': cores, 'kind': 'port',
+            'sample': (f'the oracle block action (the reference algorithm, one trial '
+                       f'vector at a time) on 4 random trial vectors of the headline '
+                       f'instance, {per_vec:.3f} s per vector, scaled by the {vectors} '
+                       f'sigma-vectors of the plain FP64 solve (subspace work not '
+                       f'counted, so the CPU figure is an upper bound)'),
+            'blas': blas, 'seconds_per_vector': per_vec}
 
 
 def main():
@@ -127,99 +339,41 @@ def main():
               f'(asked for --gpus {args.gpus}).', file=sys.stderr)
         sys.exit(2)
     import torch
-    from mbptcode_amd import grids, synth
-    from mbptcode_amd.block_actions import grid_block_action
-    from mbptcode_amd.space_time import screened_interaction_grid
-
     torch.cuda.set_device(0)
-    inst = synth.config(1)
-    log(f'instance nocc={inst.nocc} nvir={inst.nvir} M={inst.M} roots={inst.nroots}')
-    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    W_grid, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g,
-                                          precision='fp64')
-    torch.cuda.synchronize()
-    w_build_s = time.perf_counter() - t0
-    log(f'W built in {w_build_s:.3f} s')
-
-    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W_grid, inst.V,
-                                 precision='fp64')
-    # FP64 reference eigenvalues: the FP64 device solve at a tight residual
-    from mbptcode_amd.davidson import run_davidson
-    e_ref, _, _ = run_davidson(act64, d, inst.nroots, 1e-8, 300, tda=True)
-    log(f'reference roots (eV): {np.round(e_ref * 27.211386, 4).tolist()}')
-
-    def timed(apply_AB, refine, steps, warmup, profile_op=None):
-        for _ in range(warmup):
-            solve(apply_AB, d, inst, refine)
-        torch.cuda.synchronize()
-        if profile_op is not None:
-            profile_op.profile(True)
-        good, errs, tims = 0, [], {}
-        t0 = time.perf_counter()
-        for s in range(steps):
-            w, X, _ = solve(apply_AB, d, inst, refine, timings=tims)
-            err = np.abs(w - e_ref) * HA_MEV
-            good += int(np.sum(err <= 1.0))
-            errs.append(err)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        prof = profile_op.profile_read() if profile_op is not None else None
-        if profile_op is not None:
-            profile_op.profile(False)
-        return good, dt, np.max(errs, axis=0), tims, prof
-
-    act, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W_grid, inst.V,
-                               precision=args.precision)
-    log(f'headline solver: {args.precision} sigma + FP64 subspace/recheck')
-    good, dt, err, tims, prof = timed(act, act64 if args.precision != 'fp64' else None,
-                                      args.steps, args.warmup, act.operator)
-    roots_per_s = good / dt
-    log(f'{args.precision}: {roots_per_s:.1f} roots/s, max err {err.max():.2e} meV, '
-        f'{tims.get("davidson_iterations")} cycles')
-    good64, dt64, err64, tims64, _ = timed(act64, None, args.steps, args.warmup)
-    plain = good64 / dt64
-    log(f'plain fp64: {plain:.1f} roots/s, {tims64.get("davidson_iterations")} cycles')
-
-    ex_ms = prof['ms']['exchange'] / prof['launches']
-    ex_flops = prof['exchange_flops'] / prof['launches']
-    achieved = ex_flops / (ex_ms * 1e-3) / 1e12
-    roofline = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[args.precision],
-                'unit': 'TFLOP/s', 'frac': achieved / PEAK_TFLOPS[args.precision],
-                'traffic': None, 'kernel': f'k_exchange<{"float" if args.precision == "fp32" else "double"}>',
-                'avg_ms': ex_ms, 'launches': prof['launches'],
-                'flops_per_launch': ex_flops,
-                'flop_definition': 'k (4 nocc M^2 + M^2) per launch, k = vectors in the batch',
-                'stage_ms_per_apply': {k: v / prof['launches'] for k, v in prof['ms'].items()}}
-
+    inst, W, out, w_s = headline(args)
+    lm = out[f'{args.precision}+learned']
+    pl = out['plain_fp64']
     result = {
         'metric': 'BSE Davidson roots/s at <=1 meV vs FP64 (learned+mixed vs plain); chi0 MFMA %',
-        'value': roots_per_s, 'unit': 'roots/s', 'n_gpus': 1, 'steps': args.steps,
-        'warmup': args.warmup, 'ms_per_step': dt / args.steps * 1e3,
+        'value': lm['roots_per_s'], 'unit': 'roots/s', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': lm['ms_per_step'],
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': args.precision, 'data': 'synthetic',
-        'config': {'workload': 'benzene/def2-SVP-size ISDF BSE-TDA singlet Davidson',
-                   'nocc': inst.nocc, 'nvir': inst.nvir, 'M': inst.M,
-                   'nroots': inst.nroots, 'tol': inst.tol, 'ntau_W': inst.ntau,
-                   'preconditioner': 'screened', 'seed': 0},
-        'max_err_meV': float(err.max()),
-        'plain_fp64': {'roots_per_s': plain, 'ms_per_step': dt64 / args.steps * 1e3,
-                       'max_err_meV': float(err64.max()),
-                       'cycles': tims64.get('davidson_iterations')},
-        'speedup_vs_plain_fp64': roots_per_s / plain if plain else None,
-        'cycles': tims.get('davidson_iterations'),
-        'W_build_s': w_build_s,
-        'roofline': roofline,
+        'config': {'workload': 'learned-preconditioner instance: ISDF BSE-TDA singlet Davidson',
+                   'nocc': inst.nocc, 'nvir': inst.nvir, 'M': inst.M, 'nroots': inst.nroots,
+                   'tol': inst.tol, 'ntau_W': inst.ntau, 'seed': 0,
+                   'solver': f'{args.precision} sigma + learned correction '
+                             f'({args.learned_mode}) + FP64 subspace + FP64 recheck'},
+        'max_err_meV': lm['max_err_meV'],
+        'speedup_vs_plain_fp64': lm['roots_per_s'] / pl['roots_per_s'],
+        'solvers': {k: v for k, v in out.items() if k != 'roofline'},
+        'W_build_s': w_s,
+        'roofline': out['roofline'],
     }
-    if not args.no_chi0:
-        log('chi0 leg (M=4000, 20 time points)')
-        result['chi0_roofline'] = chi0_leg()
-        log(f'chi0: {result["chi0_roofline"]["achieved"]:.1f} TFLOP/s')
+    if not args.no_legs:
+        log('leg: benzene-size instance')
+        result['benzene_size'] = leg_benzene()
+        log('leg: large instance (errors, E_c, chi0)')
+        large = leg_large()
+        result['chi0_roofline'] = large.pop('chi0_roofline')
+        result['large_instance'] = large
+        log('leg: batched workload')
+        result['batched'] = leg_batched()
     if not args.no_cpu:
         log('CPU baseline (FP64 oracle)')
-        result['cpu_baseline'] = cpu_baseline(inst, W_grid.cpu().numpy(), e_ref)
-        log(f'cpu: {result["cpu_baseline"]["value"]:.2f} roots/s')
+        result['cpu_baseline'] = cpu_baseline(inst, W.cpu().numpy(), None,
+                                              pl['sigma_vectors'])
+        log(f'cpu: {result["cpu_baseline"]["value"]:.3f} roots/s')
     print(json.dumps(result), flush=True)
 
 

@@ -446,26 +446,32 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
     refine = {}
     if refine_apply is not None:
+        # the FP64 check covers the requested roots; the extra tracked ones
+        # only bound the unconfirmed-root test and keep their residuals
         op64 = _operator_of(refine_apply)
         mm64 = matmat_of(op64)
-        AX = mm64(X)
-        theta = (X * AX).sum(0)
-        R = AX - X * theta[None, :]
+        Xr = X[:, :nroots].contiguous()
+        AX = mm64(Xr)
+        theta = (Xr * AX).sum(0)
+        R = AX - Xr * theta[None, :]
         rn64 = R.norm(dim=0).cpu().numpy()
         refine['rnorm_before'] = rn64.copy()
-        if not (rn64[:nroots] <= conv_tol).all():
+        if not (rn64 <= conv_tol).all():
             res2 = davidson(mm64, pdiag, track, X, tol=conv_tol, max_cycle=max_cycle,
                             max_space=max_space)
-            e, X, conv, rn64 = res2.e, res2.X, res2.conv, res2.rnorm
+            e, X, conv, rn = res2.e, res2.X, res2.conv, res2.rnorm
             refine['cycles'] = res2.cycles
             refine['vectors'] = res2.vectors
             cycles += res2.cycles
             vectors += res2.vectors
+            rn64 = rn[:nroots]
         else:
-            e = theta.cpu().numpy()
-            conv = rn64 <= conv_tol
-        rn = rn64
-        refine['rnorm_after'] = np.asarray(rn).copy()
+            e = np.array(e, dtype=float)
+            e[:nroots] = theta.cpu().numpy()
+            rn = np.array(rn, dtype=float)
+            rn[:nroots] = rn64
+            conv = rn <= conv_tol
+        refine['rnorm_after'] = np.asarray(rn64).copy()
     unconfirmed = np.zeros(nroots, bool)
     extra = np.flatnonzero(~conv[nroots:]) + nroots
     if extra.size:
