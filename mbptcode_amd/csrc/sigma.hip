@@ -15,6 +15,7 @@
 // a workgroup reads is shared by the k workgroups of one XCD through its L2.
 #include "common.h"
 #include "exchange_bf16.h"
+#include "gemm_km.h"
 
 #include <algorithm>
 #include <vector>
@@ -205,6 +206,96 @@ __global__ __launch_bounds__(256) void k_exchange(
   }
 }
 
+// u_q[b][nu] = sum_{mu in chunk q} p[b][mu] V[mu][nu]: V streamed once per pass
+// (each thread one column nu, coalesced rows of V), p staged in LDS 64 rows at
+// a time and broadcast; up to KV vectors per pass in registers.
+template <typename T, int KV>
+__global__ __launch_bounds__(256) void k_hartree_u(int M, int k, int chunk,
+                                                   const T *__restrict__ p,
+                                                   const T *__restrict__ V,
+                                                   T *__restrict__ u) {
+  __shared__ T ps[KV][64];
+  const int nu = blockIdx.x * 256 + threadIdx.x, q = blockIdx.y;
+  const int mu_beg = q * chunk, mu_end = min(M, mu_beg + chunk);
+  for (int v0 = 0; v0 < k; v0 += KV) {
+    const int nvec = min(KV, k - v0);
+    T acc[KV];
+#pragma unroll
+    for (int v = 0; v < KV; ++v) acc[v] = 0;
+    for (int mg = mu_beg; mg < mu_end; mg += 64) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < KV * 64; e += 256) {
+        const int v = e >> 6, r = e & 63, mu = mg + r;
+        ps[v][r] = (v < nvec && mu < mu_end) ? p[(long)(v0 + v) * M + mu] : T(0);
+      }
+      __syncthreads();
+      if (nu < M) {
+        // 16 rows of V in flight per thread before the FMAs that use them
+        for (int r0 = 0; r0 < 64 && mg + r0 < mu_end; r0 += 16) {
+          T x[16];
+#pragma unroll
+          for (int q = 0; q < 16; ++q) {
+            const int mu = mg + r0 + q;
+            x[q] = mu < mu_end ? V[(long)mu * M + nu] : T(0);
+          }
+#pragma unroll
+          for (int q = 0; q < 16; ++q)
+#pragma unroll
+            for (int v = 0; v < KV; ++v) acc[v] += ps[v][r0 + q] * x[q];
+        }
+      }
+    }
+    if (nu < M)
+      for (int v = 0; v < nvec; ++v) u[((long)q * k + v0 + v) * M + nu] = acc[v];
+  }
+}
+
+// T'^T (K-major, for the tail GEMM): Tct[mu][m] = sum_s Tpart_s[m][mu]
+// - kappa X_o^T[i][mu] sum_q u_q[b][mu], m = b n_o + i, with the padding up to
+// (Mp, ldt) written as zero; 64 x 64 tiles transposed through LDS.
+__global__ __launch_bounds__(256) void k_tail_prep_t(int k, int no, int nopad, int M, int Mp,
+                                                     int ldt, int nsplit,
+                                                     const float *__restrict__ Tp,
+                                                     const float *__restrict__ XoT,
+                                                     const float *__restrict__ u, int nqu,
+                                                     float kappa, float *__restrict__ Tct) {
+  __shared__ float t[64][65];
+  const int m0 = blockIdx.x * 64, mu0 = blockIdx.y * 64, rows = k * no;
+  const long sstride = (long)k * nopad * M;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63, m = m0 + r, mu = mu0 + c;
+    float val = 0.f;
+    if (m < rows && mu < M) {
+      const int b = m / no, i = m - b * no;
+      const long off = ((long)b * nopad + i) * M + mu;
+      val = Tp[off];
+      for (int q = 1; q < nsplit; ++q) val += Tp[off + q * sstride];
+      if (u) {
+        float ub = u[(long)b * M + mu];
+        for (int q = 1; q < nqu; ++q) ub += u[((long)q * k + b) * M + mu];
+        val -= kappa * XoT[(long)i * M + mu] * ub;
+      }
+    }
+    t[r][c] = val;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63, mu = mu0 + r, m = m0 + c;
+    if (mu < Mp && m < ldt) Tct[(long)mu * ldt + m] = t[c][r];
+  }
+}
+
+// row-padded copy out[r][c] = in[r][c] (zero outside rows x cols) of pr x pc
+__global__ void k_pad_copy(int rows, int cols, const float *__restrict__ in, int pr, int pc,
+                           float *__restrict__ out) {
+  const long n = (long)pr * pc;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / pc), c = (int)(e % pc);
+    out[e] = (r < rows && c < cols) ? in[(long)r * cols + c] : 0.f;
+  }
+}
+
 // ---- tail: Az = d o z - T' X_v, T' = sum_s T_s - kappa X_o^T o u -------------------
 // prep: the split partials summed and the Hartree term folded in, one coalesced
 // pass writing T' as (k nocc, M);  tail: T' X_v with the grid index split in
@@ -343,10 +434,14 @@ struct mbpt_sigma {
   size_t esz;              // bytes per working-precision element
   void *Xo, *Xv, *XoT, *W, *V;
   bf16x8 *frag;            // X_o split and packed in MFMA fragment order (MBPT_BF16X3)
+  // FP32 working precision: K-major, zero-padded copies for the head and tail
+  // GEMMs, X_v^T (nvp x Mq) and X_v (Mp x nvq); p pads to 16, q pads to 128
+  float *XvT, *XvP;
+  int nvp, Mq, Mp, nvq;
   double *d;
   // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
-  void *ws[6];
-  size_t ws_cap[6];        // bytes
+  void *ws[7];
+  size_t ws_cap[7];        // bytes
   size_t held;
   int prof;                // stage timing on
   hipEvent_t ev[5];
@@ -361,7 +456,7 @@ constexpr int kTargetBlocks = 2048;
 
 int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
 
-enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_N };
+enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_ZT, WS_N };
 
 void free_ws(mbpt_sigma *h) {
   for (int q = 0; q < WS_N; ++q) {
@@ -396,11 +491,14 @@ int ensure_buf(mbpt_sigma *h, int q, size_t bytes) {
   return MBPT_OK;
 }
 
-int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu) {
+int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu, int ldr) {
   const size_t e = h->esz, M = h->M, no = h->no, nv = h->nv;
+  const bool km = h->esz == 4;   // K-major head/tail operands (FP32 working precision)
   const size_t sizes[WS_N] = {e * k * no * M, e * k * M, e * (size_t)nqu * k * M,
-                              e * (size_t)nsplit * k * h->nopad * M, e * k * no * M,
-                              e * (size_t)nq * k * no * nv};
+                              e * (size_t)nsplit * k * h->nopad * M,
+                              km ? 0 : e * k * no * M,
+                              e * (size_t)nq * k * no * nv,
+                              km ? 4 * (size_t)std::max(h->nvp, h->Mp) * ldr : 0};
   for (int q = 0; q < WS_N; ++q) {
     int rc = ensure_buf(h, q, sizes[q]);
     if (rc) return rc;
@@ -443,16 +541,37 @@ void launch_exchange_bf16(const mbpt_sigma *h, int k, int nsplit, int rps,
       (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
 
+template <typename T, int KV>
+void launch_hartree_u(int M, int k, int chunk, int nqu, const T *p, const T *V, T *u,
+                      hipStream_t st) {
+  k_hartree_u<T, KV><<<dim3((M + 255) / 256, nqu), 256, 0, st>>>(M, k, chunk, p, V, u);
+}
+
 template <typename T>
 int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st) {
   const int M = h->M, no = h->no, nv = h->nv;
   const int nsplit = nsplit_for(h, k);
   const int rows = k * no;
-  const int nq = nq_for(rows, nv, M);
-  // Hartree u = p V: K = M split in nqu chunks of kchu (multiples of 16)
-  const int kchu = ((M + nq_for(k, M, M) - 1) / nq_for(k, M, M) + 15) / 16 * 16;
-  const int nqu = (M + kchu - 1) / kchu;
-  int rc = ensure_ws(h, k, nsplit, nq, nqu);
+  const bool km = sizeof(T) == 4;   // FP32 working precision: K-major head and tail
+  const int ldr = (rows + 127) / 128 * 128;
+  // tail: K = M split in nq chunks of kchunk (multiples of 16), enough
+  // workgroups for the skinny output
+  int nq, kchunk;
+  if (km) {
+    const int tiles = (ldr / 128) * ((nv + 127) / 128);
+    nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp / 64));
+    kchunk = ((h->Mp + nq - 1) / nq + 15) / 16 * 16;
+    nq = (h->Mp + kchunk - 1) / kchunk;
+  } else {
+    nq = nq_for(rows, nv, M);
+    kchunk = ((M + nq - 1) / nq + 15) / 16 * 16;
+    nq = (M + kchunk - 1) / kchunk;
+  }
+  // Hartree u = p V streamed: 256-column blocks x nqu row chunks
+  int nqu = std::max(1, std::min((512 + (M + 255) / 256 - 1) / ((M + 255) / 256), (M + 63) / 64));
+  const int hchunk = (M + nqu - 1) / nqu;
+  nqu = (M + hchunk - 1) / hchunk;
+  int rc = ensure_ws(h, k, nsplit, nq, nqu, ldr);
   if (rc) return rc;
   T *zXv = (T *)h->ws[WS_ZXV], *p = (T *)h->ws[WS_P], *u = (T *)h->ws[WS_U];
   T *Tpart = (T *)h->ws[WS_TPART], *Tc = (T *)h->ws[WS_TC], *Ep = (T *)h->ws[WS_EP];
@@ -461,7 +580,18 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   const int nsplit_eff = (M + rps - 1) / rps;
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[0], st));
 
-  {  // head
+  if (km) {  // head, K-major: Z^T padded, then (Z^T)^T X_v^T
+    float *Zt = (float *)h->ws[WS_ZT];
+    k_transpose_pad<double><<<dim3(ldr / 64, (h->nvp + 63) / 64), 256, 0, st>>>(
+        rows, nv, z, nv, Zt, ldr, ldr, h->nvp);
+    if ((rc = launch_status("k_transpose_pad(z)"))) return rc;
+    const int ntm = ldr / 128, ntn = (M + 127) / 128;
+    const int nblk = pad_grid((long)ntm * ntn);
+    k_gemm_km<EpiStore><<<nblk, 512, 0, st>>>(h->nvp, h->nvp, Zt, ldr, h->XvT, h->Mq, ntm,
+                                               ntn, 1, EpiStore{(float *)zXv, rows, M, M},
+                                               nblk);
+    if ((rc = launch_status("k_gemm_km(head)"))) return rc;
+  } else {  // head
     const int ntm = (rows + 63) / 64, ntn = (M + 63) / 64;
     const int nblk = pad_grid((long)ntm * ntn);
     k_head<T><<<nblk, 256, 0, st>>>(rows, M, nv, z, (const T *)h->Xv, zXv, ntm, ntn,
@@ -473,11 +603,10 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     k_hartree_p<T><<<grid1((long)k * M), 256, 0, st>>>(M, no, k, (const T *)h->XoT,
                                                         zXv, p);
     if ((rc = launch_status("k_hartree_p"))) return rc;
-    const int ntm = (k + 63) / 64, ntn = (M + 63) / 64;
-    const int nblk = pad_grid((long)ntm * ntn * nqu);
-    k_gemm_nn_splitk<T><<<nblk, 256, 0, st>>>(k, M, M, kchu, p, M, (const T *)h->V, M,
-                                              u, ntm, ntn, nqu, nblk);
-    if ((rc = launch_status("k_gemm_nn_splitk(hartree)"))) return rc;
+    if (k <= 16) launch_hartree_u<T, 16>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    else if (k <= 32) launch_hartree_u<T, 32>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    else launch_hartree_u<T, 64>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    if ((rc = launch_status("k_hartree_u"))) return rc;
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
   if (h->prec == MBPT_BF16X3) {  // exchange, split BF16
@@ -501,23 +630,33 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   }
   if ((rc = launch_status("k_exchange"))) return rc;
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
-  {  // tail
+  if (km) {  // tail, K-major: T'^T padded, then (T'^T)^T X_v split over K
+    float *Tct = (float *)h->ws[WS_ZT];
+    k_tail_prep_t<<<dim3(ldr / 64, (h->Mp + 63) / 64), 256, 0, st>>>(
+        k, no, h->nopad, M, h->Mp, ldr, nsplit_eff, (const float *)Tpart,
+        (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr, nqu,
+        (float)h->kappa, Tct);
+    if ((rc = launch_status("k_tail_prep_t"))) return rc;
+    const int ntm = ldr / 128, ntn = (nv + 127) / 128;
+    const int nblk = pad_grid((long)ntm * ntn * nq);
+    k_gemm_km<EpiStore><<<nblk, 512, 0, st>>>(h->Mp, kchunk, Tct, ldr, h->XvP, h->nvq, ntm,
+                                               ntn, nq, EpiStore{(float *)Ep, rows, nv, nv},
+                                               nblk);
+    if ((rc = launch_status("k_gemm_km(tail)"))) return rc;
+  } else {  // tail
     k_tail_prep<T><<<grid1((long)rows * M), 256, 0, st>>>(
         k, no, h->nopad, M, nsplit_eff, Tpart, (const T *)h->XoT,
         h->kappa != 0.0 ? u : nullptr, nqu, (T)h->kappa, Tc);
     if ((rc = launch_status("k_tail_prep"))) return rc;
     const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
-    const int kchunk = ((M + nq - 1) / nq + 15) / 16 * 16;
-    const int nq_eff = (M + kchunk - 1) / kchunk;
-    const int nblk = pad_grid((long)ntm * ntn * nq_eff);
+    const int nblk = pad_grid((long)ntm * ntn * nq);
     k_gemm_nn_splitk<T><<<nblk, 256, 0, st>>>(rows, nv, M, kchunk, Tc, M,
-                                              (const T *)h->Xv, nv, Ep, ntm, ntn,
-                                              nq_eff, nblk);
+                                              (const T *)h->Xv, nv, Ep, ntm, ntn, nq, nblk);
     if ((rc = launch_status("k_gemm_nn_splitk(tail)"))) return rc;
-    k_tail_finish<T><<<grid1((long)rows * nv), 256, 0, st>>>(rows, no, nv, nq_eff, Ep,
-                                                              h->d, z, Az);
-    if ((rc = launch_status("k_tail_finish"))) return rc;
   }
+  k_tail_finish<T><<<grid1((long)rows * nv), 256, 0, st>>>(rows, no, nv, nq, Ep, h->d, z,
+                                                            Az);
+  if ((rc = launch_status("k_tail_finish"))) return rc;
   if (h->prof) {
     MBPT_HIP_TRY(hipEventRecord(h->ev[4], st));
     MBPT_HIP_TRY(hipEventSynchronize(h->ev[4]));
@@ -617,6 +756,24 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
   int rc = precision == MBPT_FP64 ? create_impl<double>(h, X_dev, W_dev, V_dev, st)
                                   : create_impl<float>(h, X_dev, W_dev, V_dev, st);
   if (rc) return fail(rc);
+  if (precision != MBPT_FP64) {  // K-major, zero-padded X_v copies for head and tail
+    h->nvp = (nvir + 15) / 16 * 16;
+    h->Mq = (M + 127) / 128 * 128;
+    h->Mp = (M + 15) / 16 * 16;
+    h->nvq = (nvir + 127) / 128 * 128;
+    he = hipMalloc((void **)&h->XvT, sizeof(float) * (size_t)h->nvp * h->Mq);
+    if (he == hipSuccess) he = hipMalloc((void **)&h->XvP, sizeof(float) * (size_t)h->Mp * h->nvq);
+    if (he != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
+      return fail(MBPT_ERR_HIP);
+    }
+    h->held += sizeof(float) * ((size_t)h->nvp * h->Mq + (size_t)h->Mp * h->nvq);
+    k_transpose_pad<float><<<dim3(h->Mq / 64, (h->nvp + 63) / 64), 256, 0, st>>>(
+        M, nvir, (const float *)h->Xv, nvir, h->XvT, h->Mq, h->Mq, h->nvp);
+    k_pad_copy<<<grid1((long)h->Mp * h->nvq), 256, 0, st>>>(M, nvir, (const float *)h->Xv,
+                                                            h->Mp, h->nvq, h->XvP);
+    if ((rc = launch_status("X_v padded copies"))) return fail(rc);
+  }
   if (precision == MBPT_BF16X3) {
     const long slots = (long)((M + 31) / 32) * bf16_chunk_slots(h->nopad);
     he = hipMalloc((void **)&h->frag, sizeof(bf16x8) * slots);
@@ -690,7 +847,7 @@ int mbpt_sigma_destroy(mbpt_sigma *h) {
     for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
   free_ws(h);
   (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
-  (void)hipFree(h->frag);
+  (void)hipFree(h->frag); (void)hipFree(h->XvT); (void)hipFree(h->XvP);
   (void)hipFree(h->d);
   delete h;
   return MBPT_OK;

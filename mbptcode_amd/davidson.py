@@ -160,7 +160,7 @@ class DavidsonResult:
 
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
-             lindep=None, stall_cycles=None, on_stall=None, observe=None):
+             lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -177,7 +177,17 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     Ritz pairs (eigh of H) and the residual norms,
     |A x - theta x|^2 = y^T G y - theta^2 for x = V y, |x| = 1.
     The returned residual norms are recomputed explicitly from A X - X theta.
+    `phases`, a dict, receives the seconds of each part of the cycle (the
+    device is synchronised at every boundary, so only for profiling).
     """
+    last = [time.perf_counter()]
+
+    def mark(name):
+        if phases is not None:
+            torch.cuda.synchronize()
+            now = time.perf_counter()
+            phases[name] = phases.get(name, 0.0) + now - last[0]
+            last[0] = now
     dev = diag.device
     n = diag.numel()
     nroots = max(1, min(int(nroots), n))
@@ -222,9 +232,11 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     stalled = False
     cycles = 0
     y = None
+    mark('setup')
     for cycle in range(max_cycle):
         cycles = cycle + 1
         HG = torch.cat([Hb[:m, :m], Gb[:m, :m]], dim=1).cpu().numpy()
+        mark('rr_transfer')
         Hns = HG[:, :m]
         H = 0.5 * (Hns + Hns.T)
         G = 0.5 * (HG[:, m:] + HG[:, m:].T)
@@ -247,6 +259,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             wf = torch.as_tensor(w[fuzzy], device=dev)
             rn[fuzzy] = (AVb[:, :m] @ yf - (Vb[:, :m] @ yf) * wf[None, :]).norm(dim=0).cpu().numpy()
         conv = rn <= tol
+        mark('rr_eigh')
         trace.append({'cycle': cycle, 'space': int(m), 'max_residual': float(rn.max()),
                       'converged': int(conv.sum())})
         if conv.all():
@@ -266,9 +279,13 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         AXu = AVb[:, :m] @ yu
         wu = torch.as_tensor(w[idx], device=dev)
         Ru = AXu - Xu * wu[None, :]
+        mark('residuals')
         if observe is not None:
             observe(cycle, Vb[:, :m], Ru, Xu, wu)
-        T = _orthonormal_new(precond(Ru, Xu, wu, cycle), Vb[:, :m], drop)
+        Tc = precond(Ru, Xu, wu, cycle)
+        mark('precond')
+        T = _orthonormal_new(Tc, Vb[:, :m], drop)
+        mark('orthonormal')
         if T.shape[1] == 0 and on_stall is not None and not stalled:
             # the correction added nothing new: the same fallback as a stall
             stalled = True
@@ -290,9 +307,12 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             Hb[:k, :k] = Hc
             Gb[:k, :k] = Gc
             m = k
+        mark('collapse')
         AT = matmat(T)
+        mark('sigma')
         vectors += T.shape[1]
         append(T, AT)
+        mark('append')
     if y_rows != m:
         # the space grew after the last Rayleigh-Ritz step (cycle budget ran out)
         H = Hb[:m, :m].cpu().numpy()
@@ -367,7 +387,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
                  learned=None, stall_cycles=3, learned_until=100.0,
-                 learned_mode='augment'):
+                 learned_mode='augment', phases=None):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
 
@@ -441,7 +461,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     res = davidson(matmat, pdiag, track, x0, tol=conv_tol, max_cycle=max_cycle,
                    max_space=max_space, precond=precond,
                    stall_cycles=stall_cycles if preconditioner == 'learned' else None,
-                   on_stall=on_stall)
+                   on_stall=on_stall, phases=phases)
     cycles, vectors = res.cycles, res.vectors
     e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
     refine = {}
@@ -458,7 +478,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
         refine['rnorm_before'] = rn64.copy()
         if not (rn64 <= conv_tol).all():
             res2 = davidson(mm64, pdiag, track, X, tol=conv_tol, max_cycle=max_cycle,
-                            max_space=max_space)
+                            max_space=max_space, phases=phases)
             e, X, conv, rn = res2.e, res2.X, res2.conv, res2.rnorm
             refine['cycles'] = res2.cycles
             refine['vectors'] = res2.vectors
