@@ -54,6 +54,8 @@ def parse():
     p.add_argument('--precision', default='bf16x3', choices=['bf16x3', 'fp32'],
                    help='sigma-vector precision of the mixed solver')
     p.add_argument('--learned-mode', default='augment', choices=['augment', 'replace'])
+    p.add_argument('--learned-until', type=float, default=1000.0,
+                   help='learned correction while the largest residual exceeds this x tol')
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     p.add_argument('--no-legs', action='store_true', help='headline only')
     return p.parse_args()
@@ -86,10 +88,10 @@ class Solver:
     FP64 recheck."""
 
     def __init__(self, name, act, d, inst, refine=None, preconditioner='screened',
-                 learned=None, learned_mode='augment'):
+                 learned=None, learned_mode='augment', learned_until=1000.0):
         self.name, self.act, self.d, self.inst = name, act, d, inst
         self.refine, self.pre = refine, preconditioner
-        self.learned, self.mode = learned, learned_mode
+        self.learned, self.mode, self.until = learned, learned_mode, learned_until
 
     def __call__(self, tol=None, timings=None):
         from mbptcode_amd.davidson import run_davidson
@@ -97,7 +99,7 @@ class Solver:
                             self.inst.tol if tol is None else tol, 200, tda=True,
                             preconditioner=self.pre, refine_apply=self.refine,
                             learned=self.learned, learned_mode=self.mode,
-                            timings=timings)
+                            learned_until=self.until, timings=timings)
 
 
 def time_solver(solver, steps, warmup, e_ref, profile_op=None):
@@ -124,17 +126,56 @@ def time_solver(solver, steps, warmup, e_ref, profile_op=None):
     return good / dt, dt / steps * 1e3, np.max(errs, axis=0), tims, prof
 
 
-def exchange_roofline(prof, precision, kernel_shape):
+def exchange_roofline(prof, precision, nocc, M):
     ex_ms = prof['ms']['exchange'] / prof['launches']
     ex_flops = prof['exchange_flops'] / prof['launches']
     achieved = ex_flops / (ex_ms * 1e-3) / 1e12
-    return {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[precision],
-            'unit': 'TFLOP/s', 'frac': achieved / PEAK_TFLOPS[precision], 'traffic': None,
-            'kernel': KERNEL_NAMES[precision], 'shape': kernel_shape, 'avg_ms': ex_ms,
-            'launches': prof['launches'], 'flops_per_launch': ex_flops,
-            'flop_definition': 'k (4 nocc M^2 + M^2) per launch, k = trial vectors in '
-                               'the launch (algorithmic FP32-equivalent flops)',
-            'stage_ms_per_apply': {k: v / prof['launches'] for k, v in prof['ms'].items()}}
+    out = {'bound': 'mfma', 'achieved': achieved, 'peak': PEAK_TFLOPS[precision],
+           'unit': 'TFLOP/s', 'frac': achieved / PEAK_TFLOPS[precision], 'traffic': None,
+           'kernel': KERNEL_NAMES[precision], 'shape': f'nocc={nocc}, M={M}', 'avg_ms': ex_ms,
+           'launches': prof['launches'], 'flops_per_launch': ex_flops,
+           'flop_definition': 'k (4 nocc M^2 + M^2) per launch, k = trial vectors in '
+                              'the launch (algorithmic FP32-equivalent flops)',
+           'stage_ms_per_apply': {k: v / prof['launches'] for k, v in prof['ms'].items()}}
+    if precision == 'bf16x3':
+        # MFMAs the kernel issues: per (vector, 128-column block, 32-row chunk)
+        # 8 column tiles x (2 NP/32 + NP/16) fragments x 3 (hi/lo split), each
+        # a 16x16x32 BF16 MFMA of 16384 flops
+        np_ = (nocc + 31) // 32 * 32
+        k_avg = ex_flops / (4.0 * nocc * M * M + M * M)
+        hw = k_avg * -(-M // 128) * -(-M // 32) * 8 * (2 * np_ // 32 + np_ // 16) * 3 * 16384
+        out['mfma_hw_tflops'] = hw / (ex_ms * 1e-3) / 1e12
+        out['mfma_util'] = out['mfma_hw_tflops'] / PEAK_TFLOPS['bf16x3']
+        out['mfma_util_definition'] = ('issued BF16 MFMA flops/s (3 per FP32 product, '
+                                       'nocc padded to 32) / 2.5 PFLOP/s')
+    return out
+
+
+def chi0_roofline(inst, reps=5):
+    """The chi0(0) build (prep, fused main kernel, reduce) of an instance's
+    static W, FP32 MFMA, timed with HIP events on its stream."""
+    from mbptcode_amd import grids
+    from mbptcode_amd.space_time import Chi0Builder
+    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    b = Chi0Builder(inst.X, inst.eps, inst.nocc, precision='fp32')
+    chi = b.build(g.tau_points, g.cosft_wt[0])
+    sync()
+    b.profile(True)
+    for _ in range(reps):
+        b.build(g.tau_points, g.cosft_wt[0], out=chi)
+    sync()
+    pr = b.profile_read()
+    b.profile(False)
+    avg_ms = pr['ms'] / pr['launches']
+    achieved = pr['flops'] / pr['launches'] / (avg_ms * 1e-3) / 1e12
+    return {'kernel': 'k_chi0_prep + k_chi0_f32 + k_chi0_reduce (one build)', 'bound': 'mfma',
+            'achieved': achieved, 'peak': PEAK_TFLOPS['fp32'], 'unit': 'TFLOP/s',
+            'frac': achieved / PEAK_TFLOPS['fp32'], 'traffic': None, 'avg_ms': avg_ms,
+            'launches': pr['launches'], 'flops_per_launch': pr['flops'] / pr['launches'],
+            'workload': f'static chi0(0), M={inst.M}, nocc={inst.nocc}, nvir={inst.nvir}, '
+                        f'ntau={inst.ntau}',
+            'flop_definition': 'ntau M(M+1)(nocc+nvir+1): the upper triangle with diagonal',
+            'mfma_instruction': 'v_mfma_f32_16x16x4f32 (peak 157.3 TFLOP/s)'}
 
 
 def headline(args):
@@ -151,7 +192,8 @@ def headline(args):
     plain = Solver('plain_fp64', act64, d, inst)
     mixed = Solver(f'{args.precision}+screened', actm, d, inst, refine=act64)
     learned = Solver(f'{args.precision}+learned', actm, d, inst, refine=act64,
-                     preconditioner='learned', learned=net, learned_mode=args.learned_mode)
+                     preconditioner='learned', learned=net, learned_mode=args.learned_mode,
+                     learned_until=args.learned_until)
     e_plain, _, _ = plain()
     e_tight, _, _ = Solver('tight', act64, d, inst)(tol=1e-8)
     log(f'roots (eV): {np.round(e_plain * 27.211386, 4).tolist()}')
@@ -169,10 +211,12 @@ def headline(args):
                        'fp64_recheck': tims.get('davidson_refine'),
                        'learned_fallback': 'learned_fallback_cycle' in tims.get('davidson_flags', {})}
         if prof is not None:
-            out['roofline'] = exchange_roofline(prof, args.precision,
-                                                f'nocc={inst.nocc}, M={inst.M}')
+            out['roofline'] = exchange_roofline(prof, args.precision, inst.nocc, inst.M)
         log(f'{s.name}: {rps:.1f} roots/s, {ms:.1f} ms/solve, max err {err.max():.2e} meV, '
             f'{tims.get("davidson_iterations")} cycles, {tims.get("davidson_vectors")} vectors')
+    out['chi0_roofline'] = chi0_roofline(inst)
+    log(f'chi0 (headline instance): {out["chi0_roofline"]["achieved"]:.1f} TF '
+        f'({100 * out["chi0_roofline"]["frac"]:.0f}% of FP32 MFMA peak)')
     return inst, W, out, w_s
 
 
@@ -202,7 +246,7 @@ def leg_large():
     import torch
     from mbptcode_amd import grids
     from mbptcode_amd.block_actions import grid_block_action
-    from mbptcode_amd.space_time import Chi0Builder, rpa_correlation_energy_grid
+    from mbptcode_amd.space_time import rpa_correlation_energy_grid
     inst, W, _ = build_problem(2)
     act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
     e64, _, _ = Solver('fp64', act64, d, inst)()
@@ -230,29 +274,10 @@ def leg_large():
                      'seconds': {'fp64': ec['fp64_s'], 'fp32': ec['fp32_s']},
                      'nfreq': grid.nfreq}
     # chi0 kernel MFMA fraction: the static chi0(0) build, 20 time points
-    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
-    b = Chi0Builder(inst.X, inst.eps, inst.nocc, precision='fp32')
-    chi = b.build(g.tau_points, g.cosft_wt[0])
-    sync()
-    b.profile(True)
-    for _ in range(5):
-        b.build(g.tau_points, g.cosft_wt[0], out=chi)
-    sync()
-    pr = b.profile_read()
-    b.profile(False)
-    avg_ms = pr['ms'] / pr['launches']
-    achieved = pr['flops'] / pr['launches'] / (avg_ms * 1e-3) / 1e12
-    res['chi0_roofline'] = {
-        'kernel': 'k_chi0_prep + k_chi0_f32 + k_chi0_reduce (one build)', 'bound': 'mfma',
-        'achieved': achieved, 'peak': PEAK_TFLOPS['fp32'], 'unit': 'TFLOP/s',
-        'frac': achieved / PEAK_TFLOPS['fp32'], 'traffic': None, 'avg_ms': avg_ms,
-        'launches': pr['launches'], 'flops_per_launch': pr['flops'] / pr['launches'],
-        'workload': f'static chi0(0), M={inst.M}, nocc={inst.nocc}, nvir={inst.nvir}, '
-                    f'ntau={inst.ntau}',
-        'flop_definition': 'ntau M(M+1)(nocc+nvir+1): the upper triangle with diagonal'}
+    res['chi0_roofline'] = chi0_roofline(inst)
     log(f'large: fp32 {res["fp32"]["max_err_meV"]:.1e} meV, bf16x3 '
         f'{res["bf16x3"]["max_err_meV"]:.1e} meV, E_c err {res["rpa_ec"]["err_uHa"]:.2e} uHa, '
-        f'chi0 {achieved:.1f} TF')
+        f'chi0 {res["chi0_roofline"]["achieved"]:.1f} TF')
     return res
 
 
@@ -353,20 +378,20 @@ def main():
                    'nocc': inst.nocc, 'nvir': inst.nvir, 'M': inst.M, 'nroots': inst.nroots,
                    'tol': inst.tol, 'ntau_W': inst.ntau, 'seed': 0,
                    'solver': f'{args.precision} sigma + learned correction '
-                             f'({args.learned_mode}) + FP64 subspace + FP64 recheck'},
+                             f'({args.learned_mode}, above {args.learned_until:g} tol) + '
+                             f'FP64 subspace + FP64 recheck'},
         'max_err_meV': lm['max_err_meV'],
         'speedup_vs_plain_fp64': lm['roots_per_s'] / pl['roots_per_s'],
-        'solvers': {k: v for k, v in out.items() if k != 'roofline'},
+        'solvers': {k: v for k, v in out.items() if k not in ('roofline', 'chi0_roofline')},
         'W_build_s': w_s,
         'roofline': out['roofline'],
+        'chi0_roofline': out['chi0_roofline'],
     }
     if not args.no_legs:
         log('leg: benzene-size instance')
         result['benzene_size'] = leg_benzene()
         log('leg: large instance (errors, E_c, chi0)')
-        large = leg_large()
-        result['chi0_roofline'] = large.pop('chi0_roofline')
-        result['large_instance'] = large
+        result['large_instance'] = leg_large()
         log('leg: batched workload')
         result['batched'] = leg_batched()
     if not args.no_cpu:

@@ -538,7 +538,7 @@ def batched_guess(diag_host, npair, nroots, P, device):
 
 
 def solve_batched(op, nroots, tol, max_cycle=100, preconditioner='screened', refine_op=None,
-                  max_space=MAX_SPACE, graph=True, learned=None, learned_until=100.0,
+                  max_space=MAX_SPACE, graph=True, learned=None, learned_until=1000.0,
                   learned_mode='augment', pdiag=None, timings=None):
     """The `nroots` lowest TDA roots of every instance of `op` (a
     BatchedSigma).  refine_op: an FP64 BatchedSigma of the same batch; the

@@ -386,7 +386,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  preconditioner=DAVIDSON_PRECONDITIONER, read_roots=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
-                 learned=None, stall_cycles=3, learned_until=100.0,
+                 learned=None, stall_cycles=3, learned_until=1000.0,
                  learned_mode='augment', phases=None):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).

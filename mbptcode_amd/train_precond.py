@@ -40,7 +40,7 @@ def training_instances(count, seed=2):
     return [synth.batched_instance(i, sizes, seed=seed) for i in range(count)]
 
 
-def evaluate(net_params, instances, precision='fp64', residual_of_jd=True):
+def evaluate(net_params, instances, precision='fp64', residual_of_jd=True, learned_until=1000.0):
     """Per preconditioner: total cycles, sigma-vectors and seconds over the
     instances, and the worst root error (meV) against the screened solve."""
     modes = ('bare', 'screened', 'learned', 'learned_replace')
@@ -62,7 +62,7 @@ def evaluate(net_params, instances, precision='fp64', residual_of_jd=True):
             t0 = time.perf_counter()
             w, _, _ = run_davidson(act, d, inst.nroots, inst.tol, 200, tda=True,
                                    preconditioner=pre.split('_')[0], learned=net,
-                                   timings=tims,
+                                   timings=tims, learned_until=learned_until,
                                    learned_mode='replace' if pre.endswith('replace')
                                    else 'augment')
             torch.cuda.synchronize()
