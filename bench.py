@@ -58,7 +58,46 @@ def parse():
                    help='learned correction while the largest residual exceeds this x tol')
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     p.add_argument('--no-legs', action='store_true', help='headline only')
+    p.add_argument('--kernel-probe', action='store_true',
+                   help='only a few sigma applies of the headline instance (k = 20, the '
+                        'mixed precision) and one 256 MiB device copy, for counter runs')
     return p.parse_args()
+
+
+#: HBM traffic of the headline exchange kernel, from the counter runs recorded
+#: in profiles/exchange_bf16_headline_pmc.md (FETCH_SIZE + WRITE_SIZE per
+#: launch at k = 20, scaled by the same counters on a copy of known size)
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'exchange_bf16_headline_pmc.md')
+
+
+def recorded_traffic():
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith('traffic_bytes_per_launch_k20:'):
+                    return float(line.split(':', 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def kernel_probe(args):
+    """Counter-run workload: the headline instance's sigma-vector at k = 20 in
+    the mixed precision, three applies, then a 256 MiB device-to-device copy
+    (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+    import torch
+    from mbptcode_amd.block_actions import SigmaOperator
+    inst, W, _ = build_problem(args.config)
+    op = SigmaOperator(inst.eps, inst.nocc, inst.X, W, inst.V, precision=args.precision)
+    z = torch.randn((20, inst.nocc, inst.nvir), dtype=torch.float64, device='cuda')
+    for _ in range(3):
+        op.apply(z)
+    sync()
+    a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device='cuda')
+    b = torch.empty_like(a)
+    b.copy_(a)
+    sync()
+    log(f'kernel probe done: 3 applies (k=20, {args.precision}), one 256 MiB copy')
 
 
 def log(msg):
@@ -212,6 +251,12 @@ def headline(args):
                        'learned_fallback': 'learned_fallback_cycle' in tims.get('davidson_flags', {})}
         if prof is not None:
             out['roofline'] = exchange_roofline(prof, args.precision, inst.nocc, inst.M)
+            tr = recorded_traffic()
+            if tr is not None and args.precision == 'bf16x3' and args.config == 4:
+                out['roofline']['traffic'] = tr
+                out['roofline']['traffic_note'] = ('HBM bytes per launch at k = 20 from the '
+                                                   'counter runs in ' +
+                                                   os.path.relpath(TRAFFIC_FILE, ROOT))
         log(f'{s.name}: {rps:.1f} roots/s, {ms:.1f} ms/solve, max err {err.max():.2e} meV, '
             f'{tims.get("davidson_iterations")} cycles, {tims.get("davidson_vectors")} vectors')
     out['chi0_roofline'] = chi0_roofline(inst)
@@ -365,6 +410,9 @@ def main():
         sys.exit(2)
     import torch
     torch.cuda.set_device(0)
+    if args.kernel_probe:
+        kernel_probe(args)
+        return
     inst, W, out, w_s = headline(args)
     lm = out[f'{args.precision}+learned']
     pl = out['plain_fp64']
