@@ -100,11 +100,13 @@ __global__ void k_pack_bf16_xo(int M, int no, int np, const float *__restrict__ 
 }
 
 // ---- the exchange ----------------------------------------------------------------
+typedef __attribute__((address_space(3))) void lds_void;
 // Workgroup = 4 waves x 32 grid columns (two 16-column tiles per wave, so each
 // X_o fragment read from LDS feeds two MFMA chains), one vector b, one split
 // of the grid rows.  The packed X_o chunks stream through a double-buffered
-// LDS stage (plain 16-byte copies), and the next chunk's W values are loaded
-// into registers while the current chunk computes.
+// LDS stage with direct global-to-LDS 16-byte loads (global_load_lds_dwordx4:
+// no staging registers, so two waves fit per SIMD), and the next chunk's W
+// values are loaded into registers while the current chunk computes.
 template <int NP>
 __global__ __launch_bounds__(256) void k_exchange_bf16(
     int M, int no, int k, const bf16x8 *__restrict__ frag, const float *__restrict__ W,
@@ -157,23 +159,26 @@ __global__ __launch_bounds__(256) void k_exchange_bf16(
         }
       }
   };
-  if (c_beg < c_end) {
-    const bf16x8 *src = frag + (long)c_beg * SPC;
+  // chunk c into buf[n]: slot 256 q + tid from thread tid, each wave's
+  // instruction filling 1 KB of LDS at base + 16 lane
+  auto fetch = [&](int c, int n) {
+    const bf16x8 *src = frag + (long)c * SPC;
 #pragma unroll
-    for (int q = 0; q < LPT; ++q) buf[0][tid + 256 * q] = src[tid + 256 * q];
-  }
+    for (int q = 0; q < LPT; ++q)
+      __builtin_amdgcn_global_load_lds((const void *)(src + 256 * q + tid),
+                                       (lds_void *)&buf[n][256 * q + 64 * wave], 16, 0, 0);
+  };
+  if (c_beg < c_end) fetch(c_beg, 0);
   float w[2][C][4];
   load_w(c_beg, w);
+  __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   int cur = 0;
   for (int c = c_beg; c < c_end; ++c) {
     const bool more = c + 1 < c_end;
-    bf16x8 stage[LPT];
     float wn[2][C][4];
     if (more) {
-      const bf16x8 *src = frag + (long)(c + 1) * SPC;
-#pragma unroll
-      for (int q = 0; q < LPT; ++q) stage[q] = src[tid + 256 * q];
+      fetch(c + 1, cur ^ 1);   // buf[cur ^ 1] was last read before the previous barrier
       load_w(c + 1, wn);
     }
     const bf16x8 *fb = buf[cur];
@@ -211,15 +216,14 @@ __global__ __launch_bounds__(256) void k_exchange_bf16(
     }
     if (more) {
 #pragma unroll
-      for (int q = 0; q < LPT; ++q) buf[cur ^ 1][tid + 256 * q] = stage[q];
-#pragma unroll
       for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
         for (int cc = 0; cc < C; ++cc)
 #pragma unroll
           for (int r = 0; r < 4; ++r) w[tt][cc][r] = wn[tt][cc][r];
     }
-    __syncthreads();
+    __builtin_amdgcn_s_waitcnt(0);   // this wave's LDS loads (and W loads) landed
+    __syncthreads();                 // ... and every other wave's
     cur ^= 1;
   }
 #pragma unroll
