@@ -244,8 +244,8 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         k = min(nroots, m)
         w, y = evals[:k], evecs[:, :k]
         y_rows = m
-        sq = np.einsum('ik,ij,jk->k', y, G, y)
-        xax = np.einsum('ik,ij,jk->k', y, Hns, y)
+        sq = ((G @ y) * y).sum(0)
+        xax = ((Hns @ y) * y).sum(0)
         # |A x - w x|^2 = |A x|^2 - 2 w x^T A x + w^2 (|x| = 1), exact for a
         # non-symmetric A as well
         rn = np.sqrt(np.clip(sq - 2.0 * w * xax + w ** 2, 0.0, None))
