@@ -114,7 +114,7 @@ def test_davidson_with_learned_correction_and_fallback():
     tims = {}
     w, X, _ = run_davidson(apply_AB, d, inst.nroots, 1e-7, 200, tda=True,
                            preconditioner='learned', learned=net, stall_cycles=2,
-                           timings=tims)
+                           learned_mode='replace', learned_until=None, timings=tims)
     assert np.abs(w - gold['roots']).max() * 27211.386 < 1e-3
     assert net.calls >= 1
     assert 'learned_fallback_cycle' in tims['davidson_flags']
