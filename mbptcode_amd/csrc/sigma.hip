@@ -250,6 +250,17 @@ __global__ __launch_bounds__(256) void k_hartree_u(int M, int k, int chunk,
   }
 }
 
+// p[e] = sum_q p[q n + e], in place (each element owned by one thread)
+template <typename T>
+__global__ void k_sum_partials(long n, int nq, T *__restrict__ p) {
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    T s = p[e];
+    for (int q = 1; q < nq; ++q) s += p[(long)q * n + e];
+    p[e] = s;
+  }
+}
+
 // T'^T (K-major, for the tail GEMM): Tct[mu][m] = sum_s Tpart_s[m][mu]
 // - kappa X_o^T[i][mu] sum_q u_q[b][mu], m = b n_o + i, with the padding up to
 // (Mp, ldt) written as zero; 64 x 64 tiles transposed through LDS.
@@ -607,6 +618,10 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     else if (k <= 32) launch_hartree_u<T, 32>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
     else launch_hartree_u<T, 64>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
     if ((rc = launch_status("k_hartree_u"))) return rc;
+    if (nqu > 1) {   // one sum of the chunk partials, read once per (vector, point) below
+      k_sum_partials<T><<<grid1((long)k * M), 256, 0, st>>>((long)k * M, nqu, u);
+      if ((rc = launch_status("k_sum_partials"))) return rc;
+    }
   }
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
   if (h->prec == MBPT_BF16X3) {  // exchange, split BF16
@@ -634,7 +649,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     float *Tct = (float *)h->ws[WS_ZT];
     k_tail_prep_t<<<dim3(ldr / 64, (h->Mp + 63) / 64), 256, 0, st>>>(
         k, no, h->nopad, M, h->Mp, ldr, nsplit_eff, (const float *)Tpart,
-        (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr, nqu,
+        (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr, 1,
         (float)h->kappa, Tct);
     if ((rc = launch_status("k_tail_prep_t"))) return rc;
     const int ntm = ldr / 128, ntn = (nv + 127) / 128;
@@ -646,7 +661,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   } else {  // tail
     k_tail_prep<T><<<grid1((long)rows * M), 256, 0, st>>>(
         k, no, h->nopad, M, nsplit_eff, Tpart, (const T *)h->XoT,
-        h->kappa != 0.0 ? u : nullptr, nqu, (T)h->kappa, Tc);
+        h->kappa != 0.0 ? u : nullptr, 1, (T)h->kappa, Tc);
     if ((rc = launch_status("k_tail_prep"))) return rc;
     const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
     const int nblk = pad_grid((long)ntm * ntn * nq);

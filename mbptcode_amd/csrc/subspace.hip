@@ -53,14 +53,29 @@ __global__ __launch_bounds__(256) void k_tn_partial(int n, int a, int b, int chu
     }
 }
 
-__global__ void k_tn_sum(int a, int b, int nc, const double *__restrict__ P,
-                         double *__restrict__ C, int ldc) {
+// 16 output elements per block, each summed by 16 threads over interleaved
+// chunks (coalesced 128-byte rows of the partial buffer) and then across the
+// 16 threads in LDS: enough loads in flight for hundreds of chunks, and a
+// fixed summation order.
+__global__ __launch_bounds__(256) void k_tn_sum(int a, int b, int nc,
+                                                const double *__restrict__ P,
+                                                double *__restrict__ C, int ldc) {
+  __shared__ double red[16][17];
   const long count = (long)a * b;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < count;
-       e += (long)gridDim.x * blockDim.x) {
-    double s = P[e];
-    for (int c = 1; c < nc; ++c) s += P[(long)c * count + e];
-    C[(e / b) * ldc + (e % b)] = s;
+  const int el = threadIdx.x & 15, g = threadIdx.x >> 4;
+  for (long base = blockIdx.x * 16L; base < count; base += (long)gridDim.x * 16) {
+    const long e = base + el;
+    double s = 0.0;
+    if (e < count)
+      for (int c = g; c < nc; c += 16) s += P[(long)c * count + e];
+    red[g][el] = s;
+    __syncthreads();
+    if (g == 0 && e < count) {
+      double t = 0.0;
+      for (int q = 0; q < 16; ++q) t += red[q][el];
+      C[(e / b) * ldc + (e % b)] = t;
+    }
+    __syncthreads();
   }
 }
 
@@ -69,13 +84,11 @@ __global__ void k_tn_sum(int a, int b, int nc, const double *__restrict__ P,
 using namespace mbpt;
 
 namespace {
-// Rows per chunk: short chunks (each workgroup runs a few K steps, the
-// partial sums are latency-bound otherwise), about a thousand workgroups in
-// all, at least 64 rows.
+// Rows per chunk: about a thousand workgroups in all, at least 256 rows each.
 int tn_chunk(int n, int a, int b) {
   const long tiles = (long)((a + kTile - 1) / kTile) * ((b + kTile - 1) / kTile);
   long nc = std::max<long>(1, 1024 / tiles);
-  nc = std::min<long>(nc, (n + 63) / 64);
+  nc = std::min<long>(nc, (n + 255) / 256);
   const int c = (int)((n + nc - 1) / nc);
   return (c + 15) / 16 * 16;
 }
@@ -107,8 +120,8 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda, const double
   int rc = launch_status("k_tn_partial");
   if (rc) return rc;
   const long count = (long)a * b;
-  k_tn_sum<<<(int)std::min<long>((count + 255) / 256, 4096L), 256, 0, st>>>(a, b, nc, work,
-                                                                            C, ldc);
+  k_tn_sum<<<(int)std::min<long>((count + 15) / 16, 8192L), 256, 0, st>>>(a, b, nc, work,
+                                                                          C, ldc);
   return launch_status("k_tn_sum");
 }
 
