@@ -159,6 +159,29 @@ int mbpt_syev_batched_f64(int nbatch, int m, const double *A_dev, int lda,
                           void *stream);
 
 /* ------------------------------------------------------------------------
+ * Batched subspace products (FP64) of many Davidson runs whose vectors are
+ * padded to P pairs with rows_dev[b] (int32, device) real rows each: the
+ * V^T W, V y and T - V C steps of the Rayleigh-Ritz and orthogonalisation
+ * inside pyscf's lib.davidson1 (called at src/Solvers/davidson.py:277), for
+ * a batch.  Operand b starts at ptr + b * stride; rows are row-major with
+ * the given leading dimensions.  Rows at and past rows[b] are not read.
+ *
+ * NN: Y_b = A_b S_b (subtract = 0; rows past rows[b] of Y_b set to zero) or
+ *     Y_b -= A_b S_b (subtract = 1; rows past rows[b] untouched), A_b (P x m),
+ *     S_b (m x c), m c <= 8192.
+ * TN: C_b = A_b^T B_b, A_b (P x a), B_b (P x c), C_b (a x c); work of
+ *     mbpt_btn_f64_work(B, P, a, c) doubles.
+ * ------------------------------------------------------------------------ */
+int mbpt_bnn_f64(int B, int P, int m, int c, const int *rows_dev,
+                 const double *A, long sA, int lda, const double *S, long sS,
+                 int lds, double *Y, long sY, int ldy, int subtract, void *stream);
+long mbpt_btn_f64_work(int B, int P, int a, int c);
+int mbpt_btn_f64(int B, int P, int a, int c, const int *rows_dev,
+                 const double *A, long sA, int lda, const double *Bm, long sB,
+                 int ldb, double *C, long sC, int ldc, double *work,
+                 long work_elems, void *stream);
+
+/* ------------------------------------------------------------------------
  * Batched sigma-vectors of n independent ISDF instances of uneven size, the
  * TDA `apply_AB` (block_actions.py:563-661) of every instance in one pass.
  * The reference has no batched path: it would run n separate closures.

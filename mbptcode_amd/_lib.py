@@ -28,7 +28,7 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_mlp_nparams', 'mbpt_mlp_create', 'mbpt_mlp_get_params',
            'mbpt_mlp_forward', 'mbpt_mlp_gradient', 'mbpt_mlp_train_step',
            'mbpt_mlp_correction', 'mbpt_mlp_correction_batched', 'mbpt_mlp_destroy',
-           'mbpt_syev_batched_f64',
+           'mbpt_syev_batched_f64', 'mbpt_bnn_f64', 'mbpt_btn_f64_work', 'mbpt_btn_f64',
            'mbpt_batch_create', 'mbpt_batch_plan', 'mbpt_batch_screened_diag',
            'mbpt_batch_npair_max',
            'mbpt_batch_held_bytes', 'mbpt_batch_apply', 'mbpt_batch_destroy')
@@ -70,6 +70,11 @@ _SIGNATURES = {
     'mbpt_mlp_destroy': (_i, [_vp]),
     'mbpt_syev_batched_f64': (_i, [_i, _i, _vp, _i, ctypes.c_long, _vp, ctypes.c_long,
                                    _vp, _i, ctypes.c_long, _i, _vp]),
+    'mbpt_bnn_f64': (_i, [_i, _i, _i, _i, _vp, _vp, ctypes.c_long, _i, _vp, ctypes.c_long, _i,
+                          _vp, ctypes.c_long, _i, _i, _vp]),
+    'mbpt_btn_f64_work': (ctypes.c_long, [_i, _i, _i, _i]),
+    'mbpt_btn_f64': (_i, [_i, _i, _i, _i, _vp, _vp, ctypes.c_long, _i, _vp, ctypes.c_long, _i,
+                          _vp, ctypes.c_long, _i, _vp, ctypes.c_long, _vp]),
     'mbpt_batch_create': (_i, [ctypes.POINTER(_vp), _i, _ip, _ip, _ip, _i, _d,
                                ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                ctypes.POINTER(_vp), _dp, _i, _vp]),

@@ -87,6 +87,77 @@ def syev_batched(A, max_sweeps=30):
 
 
 # ---------------------------------------------------------------------------
+# batched subspace products over the real rows of each instance
+# ---------------------------------------------------------------------------
+
+_work_cache = {}
+
+
+def _rows_ok(t):
+    return t if t.stride(-1) == 1 else t.contiguous()
+
+
+def bnn(A, S, rows, out=None, subtract=False):
+    """A_b S_b for every instance (A (B, P, m), S (B, m, c)) over the first
+    rows[b] rows; the result's padding rows are zero.  With `subtract`,
+    out -= A S in place.  Device tensors go through the batched FP64 kernel,
+    host tensors through torch.bmm."""
+    B, P, m = A.shape
+    c = S.shape[2]
+    if not A.is_cuda or rows is None or m * c > 8192 or m == 0:
+        R = torch.bmm(A, S) if m else torch.zeros((B, P, c), dtype=A.dtype, device=A.device)
+        if subtract:
+            out -= R
+            return out
+        if out is None:
+            return R
+        out.copy_(R)
+        return out
+    A, S = _rows_ok(A), _rows_ok(S)
+    if out is None:
+        out = torch.empty((B, P, c), dtype=torch.float64, device=A.device)
+    if out.stride(-1) != 1:
+        raise ValueError('bnn: the output rows must be unit-stride')
+    _lib.check(_lib.lib().mbpt_bnn_f64(
+        B, P, m, c, ctypes.c_void_p(rows.data_ptr()), ctypes.c_void_p(A.data_ptr()),
+        A.stride(0), A.stride(1), ctypes.c_void_p(S.data_ptr()), S.stride(0), S.stride(1),
+        ctypes.c_void_p(out.data_ptr()), out.stride(0), out.stride(1), int(subtract),
+        _lib.stream_ptr()), 'mbpt_bnn_f64')
+    return out
+
+
+def btn(A, Bm, rows, out=None):
+    """A_b^T B_b for every instance (A (B, P, a), Bm (B, P, c)) over the
+    first rows[b] rows, into `out` (B, a, c; any row stride) or a new tensor."""
+    B, P, a = A.shape
+    c = Bm.shape[2]
+    if not A.is_cuda or rows is None:
+        R = torch.bmm(A.transpose(1, 2), Bm)
+        if out is None:
+            return R
+        out.copy_(R)
+        return out
+    A, Bm = _rows_ok(A), _rows_ok(Bm)
+    if out is None:
+        out = torch.empty((B, a, c), dtype=torch.float64, device=A.device)
+    if out.stride(-1) != 1:
+        raise ValueError('btn: the output rows must be unit-stride')
+    L = _lib.lib()
+    need = int(L.mbpt_btn_f64_work(B, P, a, c))
+    key = A.device
+    w = _work_cache.get(key)
+    if w is None or w.numel() < need:
+        w = torch.empty(max(need, 1), dtype=torch.float64, device=A.device)
+        _work_cache[key] = w
+    _lib.check(L.mbpt_btn_f64(
+        B, P, a, c, ctypes.c_void_p(rows.data_ptr()), ctypes.c_void_p(A.data_ptr()),
+        A.stride(0), A.stride(1), ctypes.c_void_p(Bm.data_ptr()), Bm.stride(0), Bm.stride(1),
+        ctypes.c_void_p(out.data_ptr()), out.stride(0), out.stride(1),
+        ctypes.c_void_p(w.data_ptr()), w.numel(), _lib.stream_ptr()), 'mbpt_btn_f64')
+    return out
+
+
+# ---------------------------------------------------------------------------
 # batched sigma-vectors
 # ---------------------------------------------------------------------------
 
@@ -366,7 +437,7 @@ def jd_correction_batched(R, U, diag, theta, floor=1e-8):
     return Kr - alpha[:, None, :] * Ku
 
 
-def orthonormalize_batched(T, V, drop):
+def orthonormalize_batched(T, V, drop, rows=None):
     """Columns of T (B, P, c) projected off the orthonormal columns V (B, P, m)
     twice and orthonormalised among themselves (eigendecomposition of the
     Gram matrix), per instance.  Columns whose norm after projection is at or
@@ -374,18 +445,17 @@ def orthonormalize_batched(T, V, drop):
     kept directions first, zero columns after, and their count (B,) int32."""
     T = T / T.norm(dim=1, keepdim=True).clamp_min(1e-300)
     if V is not None and V.shape[2]:
-        Vt = V.transpose(1, 2)
-        T = T - torch.bmm(V, torch.bmm(Vt, T))
-        T = T - torch.bmm(V, torch.bmm(Vt, T))
+        for _ in range(2):
+            bnn(V, btn(V, T, rows), rows, out=T, subtract=True)
     keep = T.norm(dim=1) > drop
     T = T * keep[:, None, :]
-    G = torch.bmm(T.transpose(1, 2), T)
+    G = btn(T, T, rows)
     s, U = syev_batched(G)
     smax = s.max(dim=1, keepdim=True).values
     good = (s > 1e-10 * smax) & (s > 1e-200)
     scale = torch.where(good, s.clamp_min(1e-300).rsqrt(), torch.zeros_like(s))
-    C = (U * scale[:, None, :]).flip(2)      # largest Gram eigenvalues first
-    return torch.bmm(T, C), good.sum(1).to(torch.int32)
+    C = (U * scale[:, None, :]).flip(2).contiguous()   # largest Gram eigenvalues first
+    return bnn(T, C, rows), good.sum(1).to(torch.int32)
 
 
 class BatchedResult:
@@ -399,7 +469,7 @@ class BatchedResult:
 
 def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
                      max_space=MAX_SPACE, precond=None, lindep=None, ax0=None,
-                     observe=None):
+                     observe=None, rows=None):
     """Block Davidson for the `nroots` lowest eigenpairs of B symmetric
     operators at once, FP64 on the device.
 
@@ -408,7 +478,9 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
     x0 (B, P, c0) orthonormal guess columns, live0 (B,) how many are real;
     ax0 optionally their image.  precond(R, U, theta, cycle) -> corrections
     (B, P, nroots); default Jacobi-Davidson on `diag`.  The space collapses
-    onto the Ritz vectors when it would exceed max_space (<= 64).
+    onto the Ritz vectors when it would exceed max_space (<= 64).  rows
+    (B,) int32 on the device: real pairs per instance (the subspace products
+    skip the padding); None for all P.
     Returns a BatchedResult with e (B, nroots) and rnorm host arrays,
     X (B, P, nroots) device.
     """
@@ -437,9 +509,9 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
         c = T.shape[2]
         Vb[:, :, m:m + c] = T
         AVb[:, :, m:m + c] = AT
-        H[:, :m + c, m:m + c] = torch.bmm(Vb[:, :, :m + c].transpose(1, 2), AT)
+        btn(Vb[:, :, :m + c], AT, rows, out=H[:, :m + c, m:m + c])
         if m:
-            H[:, m:m + c, :m] = torch.bmm(T.transpose(1, 2), AVb[:, :, :m])
+            btn(T, AVb[:, :, :m], rows, out=H[:, m:m + c, :m])
         valid[:, m:m + c] = torch.arange(c, device=dev)[None, :] < live[:, None]
         m += c
 
@@ -455,8 +527,8 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
         w_all, Y = syev_batched(Hm)
         y = Y[:, :, :r]
         w = w_all[:, :r].contiguous()
-        X = torch.bmm(Vb[:, :, :m], y)
-        AX = torch.bmm(AVb[:, :, :m], y)
+        X = bnn(Vb[:, :, :m], y, rows)
+        AX = bnn(AVb[:, :, :m], y, rows)
         R = AX - X * w[:, None, :]
         rn = R.norm(dim=1)
         ts = time.time()
@@ -473,7 +545,7 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
         open_ = torch.as_tensor(~conv, device=dev).to(torch.float64)
         Tc = precond(R, X, w, cycle)
         Tc = Tc * open_.repeat(1, Tc.shape[2] // r)[:, None, :]
-        T, live = orthonormalize_batched(Tc, Vb[:, :, :m], drop)
+        T, live = orthonormalize_batched(Tc, Vb[:, :, :m], drop, rows)
         ts = time.time()
         cmax = int(live.max())
         sync_s += time.time() - ts
@@ -498,8 +570,8 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
         w_all, Y = syev_batched(Hm)
         y = Y[:, :, :r]
         w = w_all[:, :r].contiguous()
-        X = torch.bmm(Vb[:, :, :m], y)
-        AX = torch.bmm(AVb[:, :, :m], y)
+        X = bnn(Vb[:, :, :m], y, rows)
+        AX = bnn(AVb[:, :, :m], y, rows)
         rn = (AX - X * w[:, None, :]).norm(dim=1)
         w_h, rn_h = w.cpu().numpy(), rn.cpu().numpy()
         conv = rn_h <= tol
@@ -570,8 +642,9 @@ def solve_batched(op, nroots, tol, max_cycle=100, preconditioner='screened', ref
                 return jd
             t = learned.correction_batched(R, U, theta, pdiag, op.npair, jd)
             return torch.cat([jd, t], dim=2) if learned_mode == 'augment' else t
+    rows = torch.as_tensor(np.asarray(op.npair, np.int32), device=dev)
     res = davidson_batched(apply, pdiag, track, x0, live0, tol, max_cycle=max_cycle,
-                           max_space=max_space, precond=precond)
+                           max_space=max_space, precond=precond, rows=rows)
     info = {'cycles': res.cycles, 'vectors': res.vectors, 'trace': res.trace,
             'sync_s': res.sync_s, 'track': track}
     e, X, rn = res.e, res.X, res.rnorm
@@ -585,7 +658,7 @@ def solve_batched(op, nroots, tol, max_cycle=100, preconditioner='screened', ref
         info['rnorm_fp64_before'] = float(rn64.max())
         if not (rn64 <= tol).all():
             res2 = davidson_batched(apply64, pdiag, track, X, live, tol, max_cycle=max_cycle,
-                                    max_space=max_space, ax0=AX)
+                                    max_space=max_space, ax0=AX, rows=rows)
             e, X, rn = res2.e, res2.X, res2.rnorm
             info['refine_cycles'] = res2.cycles
             info['refine_vectors'] = res2.vectors

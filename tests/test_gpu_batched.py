@@ -266,3 +266,30 @@ def test_solve_batched_matches_per_instance(mode):
         xb = Xb.reshape(nroots, -1).cpu().numpy()
         rn = np.linalg.norm(AX - xb * e[b][:, None], axis=1)
         assert rn.max() <= 1.01 * tol
+
+
+def test_batched_subspace_products_match_bmm():
+    from mbptcode_amd.batched import bnn, btn
+    print('batched subspace products', flush=True)
+    rng = np.random.default_rng(8)
+    B, P, m, c = 5, 1300, 37, 9
+    rows_h = np.array([1300, 700, 0, 1, 513], np.int32)
+    rows = torch.as_tensor(rows_h, device='cuda')
+    A = torch.zeros((B, P, 64), dtype=torch.float64, device='cuda')
+    Bm = torch.zeros((B, P, c), dtype=torch.float64, device='cuda')
+    for b in range(B):
+        A[b, :rows_h[b], :m] = torch.as_tensor(rng.normal(size=(rows_h[b], m)), device='cuda')
+        Bm[b, :rows_h[b]] = torch.as_tensor(rng.normal(size=(rows_h[b], c)), device='cuda')
+    Av = A[:, :, :m]                                   # strided view, row stride 64
+    ref_tn = torch.bmm(Av.transpose(1, 2).contiguous(), Bm)
+    H = torch.full((B, 50, 50), 7.0, dtype=torch.float64, device='cuda')
+    btn(Av, Bm, rows, out=H[:, 3:3 + m, 5:5 + c])
+    assert torch.allclose(H[:, 3:3 + m, 5:5 + c], ref_tn, rtol=1e-12, atol=1e-10)
+    assert (H[:, :3] == 7.0).all() and (H[:, :, :5] == 7.0).all()
+    S = torch.as_tensor(rng.normal(size=(B, m, c)), device='cuda')
+    ref_nn = torch.bmm(Av.contiguous(), S)
+    Y = bnn(Av, S, rows)
+    assert torch.allclose(Y, ref_nn, rtol=1e-12, atol=1e-10)
+    Z = Bm.clone()
+    bnn(Av, S, rows, out=Z, subtract=True)
+    assert torch.allclose(Z, Bm - ref_nn, rtol=1e-12, atol=1e-10)
