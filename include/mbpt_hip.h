@@ -168,9 +168,9 @@ int mbpt_syev_batched_f64(int nbatch, int m, const double *A_dev, int lda,
  *
  * NN: Y_b = A_b S_b (subtract = 0; rows past rows[b] of Y_b set to zero) or
  *     Y_b -= A_b S_b (subtract = 1; rows past rows[b] untouched), A_b (P x m),
- *     S_b (m x c), m c <= 8192.
- * TN: C_b = A_b^T B_b, A_b (P x a), B_b (P x c), C_b (a x c); work of
- *     mbpt_btn_f64_work(B, P, a, c) doubles.
+ *     S_b (m x c), m <= 64.
+ * TN: C_b = A_b^T B_b, A_b (P x a), B_b (P x c), C_b (a x c), a, c <= 64;
+ *     work of mbpt_btn_f64_work(B, P, a, c) doubles.
  * ------------------------------------------------------------------------ */
 int mbpt_bnn_f64(int B, int P, int m, int c, const int *rows_dev,
                  const double *A, long sA, int lda, const double *S, long sS,

@@ -104,7 +104,7 @@ def bnn(A, S, rows, out=None, subtract=False):
     host tensors through torch.bmm."""
     B, P, m = A.shape
     c = S.shape[2]
-    if not A.is_cuda or rows is None or m * c > 8192 or m == 0:
+    if not A.is_cuda or rows is None or m > 64 or m == 0:
         R = torch.bmm(A, S) if m else torch.zeros((B, P, c), dtype=A.dtype, device=A.device)
         if subtract:
             out -= R
@@ -131,7 +131,7 @@ def btn(A, Bm, rows, out=None):
     first rows[b] rows, into `out` (B, a, c; any row stride) or a new tensor."""
     B, P, a = A.shape
     c = Bm.shape[2]
-    if not A.is_cuda or rows is None:
+    if not A.is_cuda or rows is None or a > 64 or c > 64:
         R = torch.bmm(A.transpose(1, 2), Bm)
         if out is None:
             return R

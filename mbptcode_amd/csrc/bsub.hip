@@ -15,97 +15,109 @@
 namespace mbpt {
 namespace {
 
-constexpr int kNnCols = 16;   // output columns per pass (registers)
-
-// one thread per row p, S_b staged in LDS, kNnCols accumulators per pass
+// NN: 64 rows per block.  The rows' A values (m <= 64) are staged in LDS with
+// coalesced loads, S 16 columns at a time; thread (row tid/4, column
+// quarter tid%4) computes 4 outputs per column block.
 __global__ __launch_bounds__(256) void k_bnn(int P, int m, int c, const int *__restrict__ rows,
                                              const double *__restrict__ A, long sA, int lda,
                                              const double *__restrict__ S, long sS, int lds,
                                              double *__restrict__ Y, long sY, int ldy,
                                              int subtract) {
-  extern __shared__ double ss[];   // m x c
-  const int b = blockIdx.y, p = blockIdx.x * 256 + threadIdx.x;
+  __shared__ double sa[64][65];
+  __shared__ double st[64][17];
+  const int b = blockIdx.y, r0 = blockIdx.x * 64, tid = threadIdx.x;
   const int nr = rows[b];
-  const double *Ab = A + b * sA, *Sb = S + b * sS;
   double *Yb = Y + b * sY;
-  if ((int)(blockIdx.x * 256) >= nr) {   // a block of padding only
-    if (!subtract && p < P)
-      for (int j = 0; j < c; ++j) Yb[(long)p * ldy + j] = 0.0;
-    return;
-  }
-  for (int e = threadIdx.x; e < m * c; e += 256) ss[e] = Sb[(long)(e / c) * lds + e % c];
-  __syncthreads();
-  if (p >= P) return;
-  if (p >= nr) {
+  if (r0 >= nr) {   // a block of padding only
     if (!subtract)
-      for (int j = 0; j < c; ++j) Yb[(long)p * ldy + j] = 0.0;
+      for (int e = tid; e < 64 * c; e += 256) {
+        const int p = r0 + e / c;
+        if (p < P) Yb[(long)p * ldy + e % c] = 0.0;
+      }
     return;
   }
-  const double *ar = Ab + (long)p * lda;
-  for (int j0 = 0; j0 < c; j0 += kNnCols) {
-    double acc[kNnCols];
-#pragma unroll
-    for (int j = 0; j < kNnCols; ++j) acc[j] = 0.0;
-    const int nj = min(kNnCols, c - j0);
-    for (int i = 0; i < m; ++i) {
-      const double a = ar[i];
-      const double *si = ss + i * c + j0;
-#pragma unroll
-      for (int j = 0; j < kNnCols; ++j)
-        if (j < nj) acc[j] += a * si[j];
+  const double *Ab = A + b * sA, *Sb = S + b * sS;
+  for (int e = tid; e < 64 * 64; e += 256) {
+    const int r = e >> 6, i = e & 63;
+    sa[r][i] = (r0 + r < nr && i < m) ? Ab[(long)(r0 + r) * lda + i] : 0.0;
+  }
+  const int r = tid >> 2, jq = tid & 3, p = r0 + r;
+  for (int cb = 0; cb < c; cb += 16) {
+    __syncthreads();
+    for (int e = tid; e < 64 * 16; e += 256) {
+      const int i = e >> 4, j = e & 15;
+      st[i][j] = (i < m && cb + j < c) ? Sb[(long)i * lds + cb + j] : 0.0;
     }
-    double *yr = Yb + (long)p * ldy + j0;
+    __syncthreads();
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < m; ++i) {
+      const double a = sa[r][i];
 #pragma unroll
-    for (int j = 0; j < kNnCols; ++j)
-      if (j < nj) yr[j] = subtract ? yr[j] - acc[j] : acc[j];
+      for (int u = 0; u < 4; ++u) acc[u] += a * st[i][jq + 4 * u];
+    }
+    if (p < P) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = cb + jq + 4 * u;
+        if (j >= c) continue;
+        double *y = Yb + (long)p * ldy + j;
+        if (p < nr) *y = subtract ? *y - acc[u] : acc[u];
+        else if (!subtract) *y = 0.0;
+      }
+    }
   }
 }
 
-struct LdAtB {  // A(m = i, k = r) = A[r][i] for r < K
-  const double *p;
-  int ld, rows, K;
-  __device__ double operator()(int m, int k) const {
-    return (m < rows && k < K) ? p[(long)k * ld + m] : 0.0;
-  }
-};
-struct LdBrB {  // B(k = r, n = j) = B[r][j]
-  const double *p;
-  int ld, K, cols;
-  __device__ double operator()(int k, int n) const {
-    return (k < K && n < cols) ? p[(long)k * ld + n] : 0.0;
-  }
-};
+constexpr int kTnChunk = 256;   // pair rows per partial product
 
-constexpr int kTnChunk = 512;   // pair rows per partial product
-
-// grid (tiles x chunks, B): the chunk's partial product as a 64x64 MFMA tile
+// TN partial over one 256-row chunk of one instance: 64-row sub-chunks of A
+// (a <= 64 columns) and B (c <= 64 columns) staged in LDS; thread (output row
+// tid/4, column quarter tid%4) keeps 16 sums (4 per 16-column block).
 __global__ __launch_bounds__(256) void k_btn_partial(int a, int c, int nch,
                                                      const int *__restrict__ rows,
                                                      const double *__restrict__ A, long sA,
                                                      int lda, const double *__restrict__ Bm,
-                                                     long sB, int ldb, double *__restrict__ W,
-                                                     int nta, int ntb) {
-  const int b = blockIdx.y;
-  const int t = blockIdx.x % (nta * ntb), q = blockIdx.x / (nta * ntb);
-  const int tb = t % ntb, ta = t / ntb;
-  const int r0 = q * kTnChunk, rc = min(kTnChunk, rows[b] - r0);
-  if (rc <= 0) return;
-  __shared__ TileSmem<double> sm;
-  MF<double>::acc acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) zero(acc[i][j]);
-  gemm64<double, false, false>(LdAtB{A + b * sA + (long)r0 * lda, lda, a, rc},
-                               LdBrB{Bm + b * sB + (long)r0 * ldb, ldb, rc, c}, ta * kTile,
-                               tb * kTile, rc, acc, sm);
-  double *out = W + ((long)b * nch + q) * a * c;
-  for (int fm = 0; fm < 2; ++fm)
-    for (int fn = 0; fn < 2; ++fn) {
-      const int j = tile_col(tb * kTile, fn);
-      for (int r = 0; r < 4; ++r) {
-        const int i = tile_row<double>(ta * kTile, fm, r);
-        if (i < a && j < c) out[(long)i * c + j] = acc[fm][fn][r];
-      }
+                                                     long sB, int ldb, double *__restrict__ W) {
+  __shared__ double sa[64][65];
+  __shared__ double sb[64][65];
+  const int b = blockIdx.y, q = blockIdx.x, tid = threadIdx.x;
+  const int nr = rows[b], p0 = q * kTnChunk;
+  if (p0 >= nr) return;
+  const double *Ab = A + b * sA, *Bb = Bm + b * sB;
+  const int i = tid >> 2, jq = tid & 3;
+  double acc[4][4];
+#pragma unroll
+  for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc[cb][u] = 0.0;
+  const int pend = min(nr, p0 + kTnChunk);
+  for (int s0 = p0; s0 < pend; s0 += 64) {
+    __syncthreads();
+    for (int e = tid; e < 64 * 64; e += 256) {
+      const int r = e >> 6, k = e & 63, p = s0 + r;
+      const bool ok = p < pend;
+      sa[r][k] = (ok && k < a) ? Ab[(long)p * lda + k] : 0.0;
+      sb[r][k] = (ok && k < c) ? Bb[(long)p * ldb + k] : 0.0;
     }
+    __syncthreads();
+    for (int r = 0; r < 64; ++r) {
+      const double x = sa[r][i];
+#pragma unroll
+      for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[cb][u] += x * sb[r][16 * cb + jq + 4 * u];
+    }
+  }
+  if (i < a) {
+    double *out = W + ((long)b * nch + q) * a * c + (long)i * c;
+#pragma unroll
+    for (int cb = 0; cb < 4; ++cb)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int j = 16 * cb + jq + 4 * u;
+        if (j < c) out[j] = acc[cb][u];
+      }
+  }
 }
 
 __global__ void k_btn_sum(int B, int a, int c, int nch, const int *__restrict__ rows,
@@ -134,18 +146,12 @@ int mbpt_bnn_f64(int B, int P, int m, int c, const int *rows_dev, const double *
                  int subtract, void *stream) {
   MBPT_REQUIRE(B >= 0 && P >= 0 && m >= 0 && c >= 0, "mbpt_bnn_f64: negative size");
   if (B == 0 || P == 0 || c == 0) return MBPT_OK;
+  MBPT_REQUIRE(m <= 64, "mbpt_bnn_f64: m > 64 is not supported");
   MBPT_REQUIRE(rows_dev && A && S && Y && lda >= m && lds >= c && ldy >= c,
                "mbpt_bnn_f64: bad pointer or leading dimension");
-  MBPT_REQUIRE((long)m * c * 8 <= 64 * 1024, "mbpt_bnn_f64: m c exceeds the 64 KiB LDS stage");
-  static bool attr = false;
-  if (!attr) {
-    MBPT_HIP_TRY(hipFuncSetAttribute((const void *)k_bnn,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, 64 * 1024));
-    attr = true;
-  }
-  dim3 grid((P + 255) / 256, B);
-  k_bnn<<<grid, 256, sizeof(double) * (size_t)std::max(m * c, 1), (hipStream_t)stream>>>(
-      P, m, c, rows_dev, A, sA, lda, S, sS, lds, Y, sY, ldy, subtract);
+  dim3 grid((P + 63) / 64, B);
+  k_bnn<<<grid, 256, 0, (hipStream_t)stream>>>(P, m, c, rows_dev, A, sA, lda, S, sS, lds, Y,
+                                               sY, ldy, subtract);
   return launch_status("k_bnn");
 }
 
@@ -159,15 +165,14 @@ int mbpt_btn_f64(int B, int P, int a, int c, const int *rows_dev, const double *
                  double *work, long work_elems, void *stream) {
   MBPT_REQUIRE(B >= 0 && P >= 0 && a >= 0 && c >= 0, "mbpt_btn_f64: negative size");
   if (B == 0 || a == 0 || c == 0) return MBPT_OK;
+  MBPT_REQUIRE(a <= 64 && c <= 64, "mbpt_btn_f64: a and c up to 64");
   MBPT_REQUIRE(rows_dev && A && Bm && C && work && lda >= a && ldb >= c && ldc >= c,
                "mbpt_btn_f64: bad pointer or leading dimension");
   const int nch = std::max(1, (P + kTnChunk - 1) / kTnChunk);
   MBPT_REQUIRE(work_elems >= (long)B * nch * a * c, "mbpt_btn_f64: work buffer too small");
   hipStream_t st = (hipStream_t)stream;
-  const int nta = (a + kTile - 1) / kTile, ntb = (c + kTile - 1) / kTile;
-  dim3 grid(nta * ntb * nch, B);
-  k_btn_partial<<<grid, 256, 0, st>>>(a, c, nch, rows_dev, A, sA, lda, Bm, sB, ldb, work, nta,
-                                      ntb);
+  k_btn_partial<<<dim3(nch, B), 256, 0, st>>>(a, c, nch, rows_dev, A, sA, lda, Bm, sB, ldb,
+                                              work);
   int rc = launch_status("k_btn_partial");
   if (rc) return rc;
   const long n = (long)B * a * c;
