@@ -70,53 +70,51 @@ __global__ __launch_bounds__(256) void k_bnn(int P, int m, int c, const int *__r
 
 constexpr int kTnChunk = 256;   // pair rows per partial product
 
-// TN partial over one 256-row chunk of one instance: 64-row sub-chunks of A
-// (a <= 64 columns) and B (c <= 64 columns) staged in LDS; thread (output row
-// tid/4, column quarter tid%4) keeps 16 sums (4 per 16-column block).
+// TN partial over one 256-row chunk of one instance on v_mfma_f64_16x16x4f64:
+// the output's 16x16 tiles (a, c <= 64: at most 4 x 4) are dealt round-robin
+// to the 4 waves; per 4 pair rows a lane loads its A element (row k = l/16,
+// output row l%16 of the tile) and its B element straight from global memory
+// (the 4 x 16 patches are 128-byte row pieces), no LDS.
 __global__ __launch_bounds__(256) void k_btn_partial(int a, int c, int nch,
                                                      const int *__restrict__ rows,
                                                      const double *__restrict__ A, long sA,
                                                      int lda, const double *__restrict__ Bm,
                                                      long sB, int ldb, double *__restrict__ W) {
-  __shared__ double sa[64][65];
-  __shared__ double sb[64][65];
-  const int b = blockIdx.y, q = blockIdx.x, tid = threadIdx.x;
+  const int b = blockIdx.y, q = blockIdx.x;
   const int nr = rows[b], p0 = q * kTnChunk;
   if (p0 >= nr) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int nti = (a + 15) / 16, ntj = (c + 15) / 16, nt = nti * ntj;
   const double *Ab = A + b * sA, *Bb = Bm + b * sB;
-  const int i = tid >> 2, jq = tid & 3;
-  double acc[4][4];
-#pragma unroll
-  for (int cb = 0; cb < 4; ++cb)
-#pragma unroll
-    for (int u = 0; u < 4; ++u) acc[cb][u] = 0.0;
   const int pend = min(nr, p0 + kTnChunk);
-  for (int s0 = p0; s0 < pend; s0 += 64) {
-    __syncthreads();
-    for (int e = tid; e < 64 * 64; e += 256) {
-      const int r = e >> 6, k = e & 63, p = s0 + r;
-      const bool ok = p < pend;
-      sa[r][k] = (ok && k < a) ? Ab[(long)p * lda + k] : 0.0;
-      sb[r][k] = (ok && k < c) ? Bb[(long)p * ldb + k] : 0.0;
-    }
-    __syncthreads();
-    for (int r = 0; r < 64; ++r) {
-      const double x = sa[r][i];
+  f64x4 acc[4];
 #pragma unroll
-      for (int cb = 0; cb < 4; ++cb)
+  for (int u = 0; u < 4; ++u) zero(acc[u]);
+  for (int p = p0; p < pend; p += 4) {
+    const int pr = p + lq;
+    const bool rok = pr < pend;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc[cb][u] += x * sb[r][16 * cb + jq + 4 * u];
+    for (int u = 0; u < 4; ++u) {
+      const int t = wave + 4 * u;
+      if (t >= nt) break;
+      const int i = 16 * (t / ntj) + lr, j = 16 * (t % ntj) + lr;
+      const double x = (rok && i < a) ? Ab[(long)pr * lda + i] : 0.0;
+      const double y = (rok && j < c) ? Bb[(long)pr * ldb + j] : 0.0;
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, acc[u], 0, 0, 0);
     }
   }
-  if (i < a) {
-    double *out = W + ((long)b * nch + q) * a * c + (long)i * c;
+  double *out = W + ((long)b * nch + q) * a * c;
 #pragma unroll
-    for (int cb = 0; cb < 4; ++cb)
+  for (int u = 0; u < 4; ++u) {
+    const int t = wave + 4 * u;
+    if (t >= nt) break;
+    const int j = 16 * (t % ntj) + lr;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = 16 * cb + jq + 4 * u;
-        if (j < c) out[j] = acc[cb][u];
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * (t / ntj) + MF<double>::orow(lq, r);
+      if (i < a && j < c) out[(long)i * c + j] = acc[u][r];
+    }
   }
 }
 
