@@ -137,6 +137,57 @@ __global__ __launch_bounds__(256) void g_hartree_p(Tables t, int k, const int *_
   }
 }
 
+// ---- Hartree u_q[b] = sum_{mu in chunk q} p[b][mu] V[mu][:], V streamed once -------------
+// Items: (256-column block, row chunk of kchu rows).  Each thread owns one
+// column; the live vectors' p values of 64 rows at a time sit in LDS; 16 rows
+// of V are in flight per thread before the FMAs that use them.
+template <typename T, int KV>
+__global__ __launch_bounds__(256) void g_hartree_u(Tables t, int k, const int *__restrict__ kact,
+                                                   const T *__restrict__ pool,
+                                                   T *__restrict__ ws) {
+  __shared__ T ps[KV][64];
+  const int G = gridDim.x;
+  for (int item = xcd_logical(blockIdx.x, G); item < t.total; item += G) {
+    const int b = find_inst(t.pre, t.n, item);
+    const Inst I = t.inst[b];
+    const int M = I.M, kl = live(kact, I.id, k), ncb = (M + 255) / 256;
+    const int local = item - t.pre[b], cb = local % ncb, q = local / ncb;
+    const int nu = cb * 256 + threadIdx.x;
+    const int mu_beg = q * I.kchu, mu_end = min(M, mu_beg + I.kchu);
+    const T *p = ws + I.p, *V = pool + I.v;
+    T *u = ws + I.u;
+    for (int v0 = 0; v0 < kl; v0 += KV) {
+      const int nvec = min(KV, kl - v0);
+      T acc[KV];
+#pragma unroll
+      for (int v = 0; v < KV; ++v) acc[v] = 0;
+      for (int mg = mu_beg; mg < mu_end; mg += 64) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < KV * 64; e += 256) {
+          const int v = e >> 6, r = e & 63, mu = mg + r;
+          ps[v][r] = (v < nvec && mu < mu_end) ? p[(long)(v0 + v) * M + mu] : T(0);
+        }
+        __syncthreads();
+        if (nu < M)
+          for (int r0 = 0; r0 < 64 && mg + r0 < mu_end; r0 += 16) {
+            T x[16];
+#pragma unroll
+            for (int qq = 0; qq < 16; ++qq) {
+              const int mu = mg + r0 + qq;
+              x[qq] = mu < mu_end ? V[(long)mu * M + nu] : T(0);
+            }
+#pragma unroll
+            for (int qq = 0; qq < 16; ++qq)
+#pragma unroll
+              for (int v = 0; v < KV; ++v) acc[v] += ps[v][r0 + qq] * x[qq];
+          }
+      }
+      if (nu < M)
+        for (int v = 0; v < nvec; ++v) u[((long)q * kl + v0 + v) * M + nu] = acc[v];
+    }
+  }
+}
+
 // ---- split-K NN products: Hartree u_q = p V (WHICH 0), tail E_q = T' X_v (WHICH 1) ----
 // Items: 64x64 output tiles times K chunks.  The chunk partials are laid out
 // [q][rows][cols] with rows = live vectors (x n_o for the tail), which is how
@@ -463,8 +514,11 @@ int apply_impl(mbpt_batch *h, const double *z, double *Az, const int *kact, hipS
   if (h->kappa != 0.0) {
     g_hartree_p<T><<<grid_for(h, h->total[S_HP]), 256, 0, st>>>(tab(S_HP), k, kact, pool, ws);
     if ((rc = launch_status("g_hartree_p"))) return rc;
-    g_nn<T, 0><<<grid_for(h, h->total[S_HU]), 256, 0, st>>>(tab(S_HU), k, kact, pool, ws);
-    if ((rc = launch_status("g_nn(hartree)"))) return rc;
+    if (k <= 16)
+      g_hartree_u<T, 16><<<grid_for(h, h->total[S_HU]), 256, 0, st>>>(tab(S_HU), k, kact, pool, ws);
+    else
+      g_hartree_u<T, 64><<<grid_for(h, h->total[S_HU]), 256, 0, st>>>(tab(S_HU), k, kact, pool, ws);
+    if ((rc = launch_status("g_hartree_u"))) return rc;
   }
   for (int q = 0; q < kBuckets; ++q) {
     const int tot = h->total[S_X16 + q];
@@ -536,7 +590,7 @@ int mbpt_batch_plan(mbpt_batch *h, int rows_per_split, int tail_kchunk, int hart
     const long work[kStages] = {
         (long)((rows + 63) / 64) * nmt,
         ((long)k * I.M + 255) / 256,
-        h->kappa != 0.0 ? (long)((k + 63) / 64) * nmt * I.nqu : 0L,
+        h->kappa != 0.0 ? (long)((I.M + 255) / 256) * I.nqu : 0L,
         ((long)rows * I.M + 255) / 256,
         (long)((rows + 63) / 64) * ((I.nv + 63) / 64) * I.nq,
         ((long)rows * I.nv + 255) / 256};
