@@ -151,12 +151,14 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda,
  * For b < nbatch: the symmetric part of A_b = A_dev + b strideA (m x m, row
  * stride lda) is diagonalised; w_dev + b strideW receives the eigenvalues
  * ascending, V_dev + b strideV (row stride ldv) the eigenvectors as columns.
- * 1 <= m <= 64; cyclic Jacobi, at most max_sweeps sweeps (10-20 suffice).
+ * 1 <= m <= 64; cyclic Jacobi, at most max_sweeps sweeps (10-20 suffice),
+ * stopping when every |a_pq| <= 1e-15 max(sqrt|a_pp a_qq|, scale): scale is
+ * the magnitude below which entries count as zero (the spectrum's size).
  * ------------------------------------------------------------------------ */
 int mbpt_syev_batched_f64(int nbatch, int m, const double *A_dev, int lda,
                           long strideA, double *w_dev, long strideW,
                           double *V_dev, int ldv, long strideV, int max_sweeps,
-                          void *stream);
+                          double scale, void *stream);
 
 /* ------------------------------------------------------------------------
  * Batched subspace products (FP64) of many Davidson runs whose vectors are

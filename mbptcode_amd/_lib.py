@@ -69,7 +69,7 @@ _SIGNATURES = {
                                          _vp, _vp, _i, _vp]),
     'mbpt_mlp_destroy': (_i, [_vp]),
     'mbpt_syev_batched_f64': (_i, [_i, _i, _vp, _i, ctypes.c_long, _vp, ctypes.c_long,
-                                   _vp, _i, ctypes.c_long, _i, _vp]),
+                                   _vp, _i, ctypes.c_long, _i, _d, _vp]),
     'mbpt_bnn_f64': (_i, [_i, _i, _i, _i, _vp, _vp, ctypes.c_long, _i, _vp, ctypes.c_long, _i,
                           _vp, ctypes.c_long, _i, _i, _vp]),
     'mbpt_btn_f64_work': (ctypes.c_long, [_i, _i, _i, _i]),

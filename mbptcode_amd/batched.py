@@ -66,10 +66,12 @@ def _device():
 # batched small eigensolver
 # ---------------------------------------------------------------------------
 
-def syev_batched(A, max_sweeps=30):
+def syev_batched(A, max_sweeps=30, scale=1.0):
     """(w, V) of the symmetric parts of A (B, m, m) FP64: ascending eigenvalues
     (B, m) and eigenvectors as columns (B, m, m).  On the device through the
-    batched Jacobi kernel (m <= 64); elsewhere torch.linalg.eigh."""
+    batched Jacobi kernel (m <= 64; `scale` the size of the spectrum, below
+    1e-15 scale an off-diagonal entry counts as zero); elsewhere
+    torch.linalg.eigh."""
     if not A.is_cuda:
         return torch.linalg.eigh(0.5 * (A + A.transpose(1, 2)))
     B, m, m2 = A.shape
@@ -82,7 +84,7 @@ def syev_batched(A, max_sweeps=30):
     _lib.check(_lib.lib().mbpt_syev_batched_f64(
         B, m, ctypes.c_void_p(A.data_ptr()), A.stride(1), A.stride(0),
         ctypes.c_void_p(w.data_ptr()), m, ctypes.c_void_p(V.data_ptr()), m, m * m,
-        max_sweeps, _lib.stream_ptr()), 'mbpt_syev_batched_f64')
+        max_sweeps, float(scale), _lib.stream_ptr()), 'mbpt_syev_batched_f64')
     return w, V
 
 
@@ -496,6 +498,9 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
         def precond(R, U, theta, cycle):
             return jd_correction_batched(R, U, diag, theta)
     t0 = time.time()
+    # the Rayleigh matrices' eigenvalues are bounded by the diagonal's size
+    dscale = float(torch.where(diag.abs() < _PAD_DIAG, diag.abs(),
+                               torch.zeros_like(diag)).max().clamp_min(1.0))
     cap = max_space
     Vb = torch.zeros((B, P, cap), dtype=torch.float64, device=dev)
     AVb = torch.zeros_like(Vb)
@@ -524,7 +529,7 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
     for cycle in range(max_cycle):
         cycles = cycle + 1
         Hm = H[:, :m, :m] + torch.diag_embed(_SLOT_SHIFT * (~valid[:, :m]).double())
-        w_all, Y = syev_batched(Hm)
+        w_all, Y = syev_batched(Hm, scale=dscale)
         y = Y[:, :, :r]
         w = w_all[:, :r].contiguous()
         X = bnn(Vb[:, :, :m], y, rows)
@@ -567,7 +572,7 @@ def davidson_batched(apply, diag, nroots, x0, live0, tol, max_cycle=100,
     else:
         # cycle budget ran out after an append: one more Rayleigh-Ritz step
         Hm = H[:, :m, :m] + torch.diag_embed(_SLOT_SHIFT * (~valid[:, :m]).double())
-        w_all, Y = syev_batched(Hm)
+        w_all, Y = syev_batched(Hm, scale=dscale)
         y = Y[:, :, :r]
         w = w_all[:, :r].contiguous()
         X = bnn(Vb[:, :, :m], y, rows)

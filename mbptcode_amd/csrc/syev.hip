@@ -6,17 +6,19 @@
 // and column are zero and never rotate).  Each round computes its rotations
 // from the 2x2 blocks, then applies all of them to the rows, then to the
 // columns of A and of the eigenvector matrix V, both held in LDS.  Sweeps
-// stop when every off-diagonal entry is below 1e-15 sqrt(|a_pp a_qq|), the
-// relative criterion under which Jacobi resolves small eigenvalues to high
-// relative accuracy and decoupled blocks of very different scale (such as
-// slots shifted out of the way by the caller) do not loosen each other's
-// tolerance (Golub & Van Loan, "Matrix Computations", sections 8.5.2-8.5.5:
+// stop when every off-diagonal entry is below 1e-15 max(sqrt(|a_pp a_qq|),
+// scale): relative to its own diagonal pair, so that decoupled blocks of very
+// different size (such as slots shifted out of the way by the caller) do not
+// loosen each other's tolerance, and absolute below the caller's scale, so
+// that near-zero eigenvalues do not demand more than working precision (Golub & Van Loan, "Matrix Computations", sections 8.5.2-8.5.5:
 // the 2x2 symmetric Schur decomposition and the parallel ordering; Demmel &
 // Veselic, SIAM J. Matrix Anal. Appl. 13 (1992) 1204, for the criterion).  The input is
 // symmetrised on load, (A + A^T) / 2, so the Rayleigh matrix of a
 // lower-precision operator, which is not exactly symmetric, is accepted as is.
 // Output: eigenvalues ascending and the matching eigenvectors as columns.
 #include "common.h"
+
+#include <algorithm>
 
 namespace mbpt {
 namespace {
@@ -39,7 +41,7 @@ __device__ __forceinline__ double block_max(double x, double *red) {
 __global__ __launch_bounds__(256) void k_syev_jacobi(int m, const double *__restrict__ A,
                                                      int lda, long sA, double *__restrict__ w,
                                                      long sw, double *__restrict__ V, int ldv,
-                                                     long sV, int max_sweeps) {
+                                                     long sV, int max_sweeps, double scale2) {
   extern __shared__ double smem[];
   double *a = smem, *v = smem + kMaxM * kLD;
   __shared__ double cc[kMaxM / 2], ss[kMaxM / 2], red[4], ev[kMaxM];
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(256) void k_syev_jacobi(int m, const double *__rest
       const int i = e / mp, j = e % mp;
       const double x = a[i * kLD + j];
       if (i != j && x != 0.0)
-        off = fmax(off, x * x / fmax(fabs(a[i * kLD + i] * a[j * kLD + j]), 1e-300));
+        off = fmax(off, x * x / fmax(fabs(a[i * kLD + i] * a[j * kLD + j]), scale2));
     }
     off = block_max(off, red);
     if (!(off > 1e-30)) break;
@@ -128,12 +130,13 @@ extern "C" {
 
 int mbpt_syev_batched_f64(int nbatch, int m, const double *A_dev, int lda, long strideA,
                           double *w_dev, long strideW, double *V_dev, int ldv, long strideV,
-                          int max_sweeps, void *stream) {
+                          int max_sweeps, double scale, void *stream) {
   MBPT_REQUIRE(nbatch >= 0 && m >= 1 && m <= kMaxM,
                "mbpt_syev_batched_f64: need 1 <= m <= 64 and nbatch >= 0");
   MBPT_REQUIRE(lda >= m && ldv >= m && strideW >= m && max_sweeps >= 1,
                "mbpt_syev_batched_f64: bad leading dimension or stride");
   MBPT_REQUIRE(A_dev && w_dev && V_dev, "mbpt_syev_batched_f64: NULL pointer");
+  MBPT_REQUIRE(scale >= 0.0, "mbpt_syev_batched_f64: scale must be non-negative");
   if (nbatch == 0) return MBPT_OK;
   static bool attr_set = false;
   if (!attr_set) {
@@ -142,7 +145,8 @@ int mbpt_syev_batched_f64(int nbatch, int m, const double *A_dev, int lda, long 
     attr_set = true;
   }
   k_syev_jacobi<<<nbatch, 256, kSyevSmem, (hipStream_t)stream>>>(
-      m, A_dev, lda, strideA, w_dev, strideW, V_dev, ldv, strideV, max_sweeps);
+      m, A_dev, lda, strideA, w_dev, strideW, V_dev, ldv, strideV, max_sweeps,
+      std::max(scale * scale, 1e-300));
   return launch_status("k_syev_jacobi");
 }
 
