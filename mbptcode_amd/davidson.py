@@ -160,7 +160,8 @@ class DavidsonResult:
 
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
-             lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None):
+             lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None,
+             keep=None):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -179,6 +180,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     The returned residual norms are recomputed explicitly from A X - X theta.
     `phases`, a dict, receives the seconds of each part of the cycle (the
     device is synchronised at every boundary, so only for profiling).
+    `keep`: Ritz vectors kept at a collapse (thick restart; default nroots).
     """
     last = [time.perf_counter()]
 
@@ -296,17 +298,20 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         if T.shape[1] == 0:
             break
         if m + T.shape[1] > max_space:
-            # collapse onto the tracked Ritz vectors (T is already orthogonal to them)
-            yt = torch.as_tensor(y, device=dev)
+            # collapse onto the lowest `keep` Ritz vectors (T is already
+            # orthogonal to the whole space)
+            kk = min(m, max(k, keep or k))
+            yk = evecs[:, :kk]
+            yt = torch.as_tensor(yk, device=dev)
             X = Vb[:, :m] @ yt
             AX = AVb[:, :m] @ yt
-            Hc = torch.as_tensor(y.T @ Hns @ y, device=dev)
-            Gc = torch.as_tensor(y.T @ G @ y, device=dev)
-            Vb[:, :k] = X
-            AVb[:, :k] = AX
-            Hb[:k, :k] = Hc
-            Gb[:k, :k] = Gc
-            m = k
+            Hc = torch.as_tensor(yk.T @ Hns @ yk, device=dev)
+            Gc = torch.as_tensor(yk.T @ G @ yk, device=dev)
+            Vb[:, :kk] = X
+            AVb[:, :kk] = AX
+            Hb[:kk, :kk] = Hc
+            Gb[:kk, :kk] = Gc
+            m = kk
         mark('collapse')
         AT = matmat(T)
         mark('sigma')
@@ -387,7 +392,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
                  learned=None, stall_cycles=3, learned_until=1000.0,
-                 learned_mode='augment', phases=None):
+                 learned_mode='augment', phases=None, keep=None):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
 
@@ -461,7 +466,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     res = davidson(matmat, pdiag, track, x0, tol=conv_tol, max_cycle=max_cycle,
                    max_space=max_space, precond=precond,
                    stall_cycles=stall_cycles if preconditioner == 'learned' else None,
-                   on_stall=on_stall, phases=phases)
+                   on_stall=on_stall, phases=phases, keep=keep)
     cycles, vectors = res.cycles, res.vectors
     e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
     refine = {}
