@@ -12,13 +12,20 @@
 // operands; the 16x16 FP32 result leaves D[4(l/16)+r][l%16] in element r.
 //
 // Per 16 grid columns nu and 32 grid rows mu (two 16-row P tiles):
-//   P_t (16 mu x 16 nu) = X_o[mu, :] zXv[b][:, nu]          K = NP, 3 MFMAs/32
+//   P_t (16 mu x 16 nu) = X_o[mu, :] zXv[b][:, nu]          K = n_o
 //   S = P o W                                               FP32, in registers
 //   T[:, nu] += X_o[mu, :]^T S                              K = 32 mu, 3 MFMAs
 // The two P tiles' result elements (rows 4q+r and 16+4q+r of lane q = l/16)
 // are the eight K slots of lane l in the second product: slot s holds row
 // pi(q, s) = 4q + s (s < 4) or 16 + 4q + (s - 4) (s >= 4), and the X_o^T
 // operand is read in the same order, so the product needs no shuffle.
+//
+// Padding of n_o.  The P products run over K blocks of 32 occupied indices.
+// The last block holds r = n_o - 32 (NKB - 1) real ones; when 3r <= 64 its
+// three split terms are laid side by side along K (hi.hi over the first r
+// slots, lo.hi over the next r, hi.lo over the next r), so the block costs
+// ceil(3r / 32) MFMAs instead of three.  The T product computes only the
+// ceil(n_o / 16) row tiles of T that hold real rows.
 #pragma once
 
 #include "common.h"
@@ -49,51 +56,69 @@ __device__ __forceinline__ void split8(const float (&x)[8], bf16x8 &hi, bf16x8 &
   }
 }
 
-// ---- X_o in MFMA fragment order -------------------------------------------------
-// The two products read X_o as A operands in two shapes: X_o[mu][j] for P
-// (K = j) and X_o^T[i][mu] for T (K = mu, in the slot order pi above).  Both
-// are packed once per handle, split into hi and lo, so that every operand a
-// lane needs is one 16-byte slot and a wave's fragment is 1 KB contiguous.
-// Per 32-row chunk of the grid, in 16-byte slots:
-//   P part  [tt < 2][kb < NP/32][hl < 2][lane]  rows 16 tt + l%16, j = 32 kb + 8 (l/16) + s
-//   T part  [it < NP/16][hl < 2][lane]          rows pi(l/16, s),  i = 16 it + l%16
-// rows >= M and columns >= nocc are zero.
-__host__ __device__ constexpr int bf16_chunk_slots(int np) { return 64 * (4 * (np / 32) + 2 * (np / 16)); }
+// ---- shape of the packed X_o --------------------------------------------------
+// np: n_o padded to 32.  tail: MFMAs of the last K block of the P products
+// (1 or 2 with the split terms side by side along K, 3 = the plain hi/lo
+// pair).  nie: row tiles of T computed.
+struct Bf16Shape {
+  int np, nkb, nie, tail, fp, spp, spc;
+};
 
-__global__ void k_pack_bf16_xo(int M, int no, int np, const float *__restrict__ Xo,
+__host__ __device__ inline Bf16Shape bf16_shape(int no) {
+  Bf16Shape s;
+  s.np = (no + 31) / 32 * 32;
+  s.nkb = s.np / 32;
+  const int r = no - 32 * (s.nkb - 1);
+  s.tail = 3 * r <= 32 ? 1 : (3 * r <= 64 ? 2 : 3);
+  s.nie = (no + 15) / 16;
+  s.fp = 2 * (s.nkb - 1) + (s.tail == 3 ? 2 : s.tail);   // P fragments per 16-row tile
+  s.spp = 128 * s.fp;                                     // P slots per chunk (two tiles)
+  s.spc = s.spp + 128 * s.nie;                            // + T slots (hi, lo per tile)
+  return s;
+}
+
+// ---- X_o in MFMA fragment order -------------------------------------------------
+// Per 32-row chunk of the grid, in 16-byte slots (one bf16x8 per lane):
+//   P part [tt < 2][f < fp][lane]   rows 16 tt + l%16; fragments 2kb + hl for
+//                                   the full K blocks (j = 32 kb + 8 (l/16) + s),
+//                                   then the last block (hi, lo, or the
+//                                   side-by-side split terms)
+//   T part [it < nie][hl < 2][lane]  rows pi(l/16, s), i = 16 it + l%16
+// rows >= M and indices >= n_o are zero.
+__global__ void k_pack_bf16_xo(int M, int no, const float *__restrict__ Xo,
                                bf16x8 *__restrict__ frag) {
-  const int nkb = np / 32, spc = bf16_chunk_slots(np), toff = 64 * 4 * nkb;
-  const long n = (long)((M + 31) / 32) * spc;
+  const Bf16Shape sh = bf16_shape(no);
+  const int j0 = 32 * (sh.nkb - 1), r = no - j0;
+  const long n = (long)((M + 31) / 32) * sh.spc;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
        e += (long)gridDim.x * blockDim.x) {
-    const int chunk = (int)(e / spc), rem = (int)(e % spc);
-    int lane, hl, rowbase, jbase, mode;
-    if (rem < toff) {
-      lane = rem & 63;
-      const int g = rem >> 6;
-      hl = g & 1;
-      const int kb = (g >> 1) % nkb, tt = (g >> 1) / nkb;
-      rowbase = 32 * chunk + 16 * tt + (lane & 15);
-      jbase = 32 * kb + 8 * (lane >> 4);
-      mode = 0;
-    } else {
-      const int r2 = rem - toff;
-      lane = r2 & 63;
-      const int g = r2 >> 6;
-      hl = g & 1;
-      rowbase = 32 * chunk;
-      jbase = 16 * (g >> 1) + (lane & 15);
-      mode = 1;
-    }
+    const int chunk = (int)(e / sh.spc), rem = (int)(e % sh.spc);
+    const int lane = rem & 63, q = lane >> 4, g = rem >> 6;
     bf16x8 out;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-      const int q = lane >> 4;
-      const int row = mode == 0 ? rowbase : rowbase + (s < 4 ? 4 * q + s : 16 + 4 * q + (s - 4));
-      const int j = mode == 0 ? jbase + s : jbase;
+      int row, j, lo;
+      if (rem < sh.spp) {
+        const int tt = g / sh.fp, f = g % sh.fp;
+        row = 32 * chunk + 16 * tt + (lane & 15);
+        if (f < 2 * (sh.nkb - 1) || sh.tail == 3) {
+          j = 32 * (f >> 1) + 8 * q + s;
+          lo = f & 1;
+        } else {
+          const int kap = 32 * (f - 2 * (sh.nkb - 1)) + 8 * q + s;
+          const int term = kap < r ? 0 : (kap < 2 * r ? 1 : (kap < 3 * r ? 2 : 3));
+          j = term == 3 ? no : j0 + kap - term * r;
+          lo = term == 1;
+        }
+      } else {
+        const int g2 = (rem - sh.spp) >> 6;
+        row = 32 * chunk + (s < 4 ? 4 * q + s : 16 + 4 * q + (s - 4));
+        j = 16 * (g2 >> 1) + (lane & 15);
+        lo = g2 & 1;
+      }
       const float x = (row < M && j < no) ? Xo[(long)row * no + j] : 0.f;
       const __bf16 h = (__bf16)x;
-      out[s] = hl == 0 ? h : (__bf16)(x - (float)h);
+      out[s] = lo ? (__bf16)(x - (float)h) : h;
     }
     frag[e] = out;
   }
@@ -101,140 +126,195 @@ __global__ void k_pack_bf16_xo(int M, int no, int np, const float *__restrict__ 
 
 // ---- the exchange ----------------------------------------------------------------
 typedef __attribute__((address_space(3))) void lds_void;
-// Workgroup = 4 waves x 32 grid columns (two 16-column tiles per wave, so each
-// X_o fragment read from LDS feeds two MFMA chains), one vector b, one split
-// of the grid rows.  The packed X_o chunks stream through a double-buffered
-// LDS stage with direct global-to-LDS 16-byte loads (global_load_lds_dwordx4:
-// no staging registers, so two waves fit per SIMD), and the next chunk's W
-// values are loaded into registers while the current chunk computes.
-template <int NP>
-__global__ __launch_bounds__(256) void k_exchange_bf16(
+// Workgroup = 4 waves; a wave computes NV vectors x NC 16-column tiles (NV = 2,
+// NC = 1 shares every W value and every X_o fragment between two vectors; NV =
+// 1, NC = 2 serves a single vector).  The packed X_o chunks stream through a
+// double-buffered LDS stage with direct global-to-LDS 16-byte loads
+// (global_load_lds_dwordx4: no staging registers, two waves fit per SIMD), and
+// the next chunk's W values are loaded into registers while the current chunk
+// computes.  W is read at clamped indices: a row >= M meets a zero row of the
+// packed X_o (P = 0 there) and a column >= M a zero column of zXv.
+template <int NP, int NIE, int TAIL, int NV, int NC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_exchange_bf16(
     int M, int no, int k, const bf16x8 *__restrict__ frag, const float *__restrict__ W,
     const float *__restrict__ zXv, float *__restrict__ Tpart, int nsplit,
     int rows_per_split, int nvb, int nblk) {
-  constexpr int NKB = NP / 32, NI = NP / 16, C = 2;
-  constexpr int SPC = bf16_chunk_slots(NP), TOFF = 64 * 4 * NKB, LPT = SPC / 256;
+  constexpr int NKB = NP / 32, NFB = TAIL == 3 ? NKB : NKB - 1, NT = NV * NC;
+  constexpr int TP = TAIL == 3 ? 2 : TAIL, FP = 2 * (NKB - 1) + TP;
+  constexpr int SPP = 128 * FP, SPC = SPP + 128 * NIE, COLS = 64 * NC;
   __shared__ bf16x8 buf[2][SPC];
+  const int kp = (k + NV - 1) / NV;
   const int lid = xcd_logical(blockIdx.x, nblk);
-  if (lid >= nvb * k * nsplit) return;
-  const int b = lid % k, t = lid / k, vb = t % nvb, s = t / nvb;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (lid >= nvb * kp * nsplit) return;
+  const int bp = lid % kp, t = lid / kp, vb = t % nvb, s = t / nvb;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lq = lane >> 4;
-  const int cbase = vb * 128 + wave * 32;
-
-  // zXv[b][:, col] as the B operand of the P products, split once
-  bf16x8 bzh[C][NKB], bzl[C][NKB];
+  int colv[NC];
+  unsigned woff[NC];
 #pragma unroll
-  for (int cc = 0; cc < C; ++cc) {
-    const int col = cbase + 16 * cc + lr;
-#pragma unroll
-    for (int kb = 0; kb < NKB; ++kb) {
-      float v[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int j = 32 * kb + 8 * lq + q;
-        v[q] = (j < no && col < M) ? zXv[((long)b * no + j) * M + col] : 0.f;
-      }
-      split8(v, bzh[cc][kb], bzl[cc][kb]);
-    }
+  for (int ct = 0; ct < NC; ++ct) {
+    colv[ct] = vb * COLS + wave * 16 * NC + 16 * ct + lr;
+    woff[ct] = (unsigned)(4 * lq * M + min(colv[ct], M - 1));
   }
-  f32x4 acc[NI][C];
+
+  // zXv[b][:, col] as the B operands of the P products, split once
+  bf16x8 bzh[NT][NFB > 0 ? NFB : 1], bzl[NT][NFB > 0 ? NFB : 1], bzt[NT][TAIL == 3 ? 1 : TAIL];
+  const int j0 = 32 * (NKB - 1), r = no - j0;
 #pragma unroll
-  for (int it = 0; it < NI; ++it)
+  for (int v = 0; v < NV; ++v)
 #pragma unroll
-    for (int cc = 0; cc < C; ++cc) zero(acc[it][cc]);
+    for (int ct = 0; ct < NC; ++ct) {
+      const int b = NV * bp + v, col = colv[ct], u = v * NC + ct;
+      const bool ok = b < k && col < M;
+      const float *zc = zXv + (long)b * no * M + col;
+#pragma unroll
+      for (int kb = 0; kb < NFB; ++kb) {
+        float x[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int j = 32 * kb + 8 * lq + q;
+          x[q] = (ok && j < no) ? zc[(long)j * M] : 0.f;
+        }
+        split8(x, bzh[u][kb], bzl[u][kb]);
+      }
+      if (TAIL < 3) {
+#pragma unroll
+        for (int tf = 0; tf < (TAIL == 3 ? 1 : TAIL); ++tf)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const int kap = 32 * tf + 8 * lq + q;
+            const int term = kap < r ? 0 : (kap < 2 * r ? 1 : (kap < 3 * r ? 2 : 3));
+            const int j = j0 + kap - term * r;
+            const float x = (ok && term < 3) ? zc[(long)j * M] : 0.f;
+            const __bf16 h = (__bf16)x;
+            bzt[u][tf][q] = term == 2 ? (__bf16)(x - (float)h) : (term == 3 ? (__bf16)0.f : h);
+          }
+      }
+    }
+  f32x4 acc[NIE][NT];
+#pragma unroll
+  for (int it = 0; it < NIE; ++it)
+#pragma unroll
+    for (int u = 0; u < NT; ++u) zero(acc[it][u]);
 
   const int c_beg = (s * rows_per_split) / 32;
   const int c_end = min((M + 31) / 32, ((s + 1) * rows_per_split) / 32);
-  auto load_w = [&](int c, float (&w)[2][C][4]) {
+  auto load_w = [&](int c, float (&w)[NC][2][4]) {
+    if (32 * c + 32 <= M) {   // whole chunk inside the grid: uniform row bases
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt)
+      for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-      for (int cc = 0; cc < C; ++cc) {
-        const int col = cbase + 16 * cc + lr;
+        for (int rr = 0; rr < 4; ++rr) {
+          const float *base = W + (long)(32 * c + 16 * tt + rr) * M;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = 32 * c + 16 * tt + 4 * lq + r;
-          w[tt][cc][r] = (mu < M && col < M) ? W[(long)mu * M + col] : 0.f;
+          for (int ct = 0; ct < NC; ++ct) w[ct][tt][rr] = base[woff[ct]];
         }
-      }
+    } else {
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct)
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr)
+            w[ct][tt][rr] = W[(long)min(32 * c + 16 * tt + 4 * lq + rr, M - 1) * M +
+                              min(colv[ct], M - 1)];
+    }
   };
   // chunk c into buf[n]: slot 256 q + tid from thread tid, each wave's
   // instruction filling 1 KB of LDS at base + 16 lane
   auto fetch = [&](int c, int n) {
     const bf16x8 *src = frag + (long)c * SPC;
 #pragma unroll
-    for (int q = 0; q < LPT; ++q)
-      __builtin_amdgcn_global_load_lds((const void *)(src + 256 * q + tid),
-                                       (lds_void *)&buf[n][256 * q + 64 * wave], 16, 0, 0);
+    for (int q = 0; q < (SPC + 255) / 256; ++q)
+      if (SPC % 256 == 0 || 256 * q + 64 * wave < SPC)
+        __builtin_amdgcn_global_load_lds((const void *)(src + 256 * q + tid),
+                                         (lds_void *)&buf[n][256 * q + 64 * wave], 16, 0, 0);
   };
   if (c_beg < c_end) fetch(c_beg, 0);
-  float w[2][C][4];
+  float w[NC][2][4];
   load_w(c_beg, w);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   int cur = 0;
   for (int c = c_beg; c < c_end; ++c) {
     const bool more = c + 1 < c_end;
-    float wn[2][C][4];
+    float wn[NC][2][4];
     if (more) {
       fetch(c + 1, cur ^ 1);   // buf[cur ^ 1] was last read before the previous barrier
       load_w(c + 1, wn);
     }
     const bf16x8 *fb = buf[cur];
-    f32x4 P[2][C];
+    f32x4 P[2][NT];
 #pragma unroll
     for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-      for (int cc = 0; cc < C; ++cc) zero(P[tt][cc]);
+      for (int u = 0; u < NT; ++u) zero(P[tt][u]);
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt)
+    for (int tt = 0; tt < 2; ++tt) {
 #pragma unroll
-      for (int kb = 0; kb < NKB; ++kb) {
-        const bf16x8 ah = fb[((tt * NKB + kb) * 2 + 0) * 64 + lane];
-        const bf16x8 al = fb[((tt * NKB + kb) * 2 + 1) * 64 + lane];
+      for (int kb = 0; kb < NFB; ++kb) {
+        const bf16x8 ah = fb[(tt * FP + 2 * kb + 0) * 64 + lane];
+        const bf16x8 al = fb[(tt * FP + 2 * kb + 1) * 64 + lane];
 #pragma unroll
-        for (int cc = 0; cc < C; ++cc)
-          P[tt][cc] = mma3(ah, al, bzh[cc][kb], bzl[cc][kb], P[tt][cc]);
+        for (int u = 0; u < NT; ++u) P[tt][u] = mma3(ah, al, bzh[u][kb], bzl[u][kb], P[tt][u]);
       }
-    bf16x8 sh[C], sl[C];
+      if (TAIL < 3) {
 #pragma unroll
-    for (int cc = 0; cc < C; ++cc) {
-      float sv[8];
+        for (int tf = 0; tf < (TAIL == 3 ? 1 : TAIL); ++tf) {
+          const bf16x8 a = fb[(tt * FP + 2 * (NKB - 1) + tf) * 64 + lane];
 #pragma unroll
-      for (int tt = 0; tt < 2; ++tt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) sv[4 * tt + r] = P[tt][cc][r] * w[tt][cc][r];
-      split8(sv, sh[cc], sl[cc]);
+          for (int u = 0; u < NT; ++u) P[tt][u] = mma_bf16(a, bzt[u][tf], P[tt][u]);
+        }
+      }
     }
+    bf16x8 sh[NT], sl[NT];
 #pragma unroll
-    for (int it = 0; it < NI; ++it) {
-      const bf16x8 ah = fb[TOFF + (it * 2 + 0) * 64 + lane];
-      const bf16x8 al = fb[TOFF + (it * 2 + 1) * 64 + lane];
+    for (int v = 0; v < NV; ++v)
 #pragma unroll
-      for (int cc = 0; cc < C; ++cc) acc[it][cc] = mma3(ah, al, sh[cc], sl[cc], acc[it][cc]);
+      for (int ct = 0; ct < NC; ++ct) {
+        float sv[8];
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) sv[4 * tt + rr] = P[tt][v * NC + ct][rr] * w[ct][tt][rr];
+        split8(sv, sh[v * NC + ct], sl[v * NC + ct]);
+      }
+#pragma unroll
+    for (int it = 0; it < NIE; ++it) {
+      const bf16x8 ah = fb[SPP + (it * 2 + 0) * 64 + lane];
+      const bf16x8 al = fb[SPP + (it * 2 + 1) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < NT; ++u) acc[it][u] = mma3(ah, al, sh[u], sl[u], acc[it][u]);
     }
     if (more) {
 #pragma unroll
-      for (int tt = 0; tt < 2; ++tt)
+      for (int ct = 0; ct < NC; ++ct)
 #pragma unroll
-        for (int cc = 0; cc < C; ++cc)
+        for (int tt = 0; tt < 2; ++tt)
 #pragma unroll
-          for (int r = 0; r < 4; ++r) w[tt][cc][r] = wn[tt][cc][r];
+          for (int rr = 0; rr < 4; ++rr) w[ct][tt][rr] = wn[ct][tt][rr];
     }
     __builtin_amdgcn_s_waitcnt(0);   // this wave's LDS loads (and W loads) landed
     __syncthreads();                 // ... and every other wave's
     cur ^= 1;
   }
+  // rows i >= 16 NIE of T (all beyond n_o) are not written
 #pragma unroll
-  for (int cc = 0; cc < C; ++cc) {
-    const int col = cbase + 16 * cc + lr;
-    if (col >= M) continue;
+  for (int v = 0; v < NV; ++v) {
+    const int b = NV * bp + v;
+    if (b >= k) continue;
 #pragma unroll
-    for (int it = 0; it < NI; ++it)
+    for (int ct = 0; ct < NC; ++ct) {
+      const int col = colv[ct];
+      if (col >= M) continue;
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        Tpart[(((long)s * k + b) * NP + 16 * it + 4 * lq + r) * M + col] = acc[it][cc][r];
+      for (int it = 0; it < NIE; ++it)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr)
+          Tpart[(((long)s * k + b) * NP + 16 * it + 4 * lq + rr) * M + col] =
+              acc[it][v * NC + ct][rr];
+    }
   }
 }
 

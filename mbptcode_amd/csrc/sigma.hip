@@ -542,14 +542,35 @@ void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_
                                              (T *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
 
-template <int NP>
-void launch_exchange_bf16(const mbpt_sigma *h, int k, int nsplit, int rps,
-                          hipStream_t st) {
-  const int nvb = (h->M + 127) / 128;
-  const int nblk = pad_grid((long)nvb * k * nsplit);
-  k_exchange_bf16<NP><<<nblk, 256, 0, st>>>(
+template <int NP, int NIE, int TAIL, int NV>
+void launch_exchange_bf16_nv(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
+  constexpr int NC = NV == 2 ? 1 : 2;
+  const int nvb = (h->M + 64 * NC - 1) / (64 * NC), kp = (k + NV - 1) / NV;
+  const int nblk = pad_grid((long)nvb * kp * nsplit);
+  k_exchange_bf16<NP, NIE, TAIL, NV, NC><<<nblk, 256, 0, st>>>(
       h->M, h->no, k, h->frag, (const float *)h->W, (const float *)h->ws[WS_ZXV],
       (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
+}
+
+// two vectors per wave whenever there are two
+template <int NP, int NIE, int TAIL>
+void launch_exchange_bf16_k(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
+  if (k >= 2) launch_exchange_bf16_nv<NP, NIE, TAIL, 2>(h, k, nsplit, rps, st);
+  else launch_exchange_bf16_nv<NP, NIE, TAIL, 1>(h, k, nsplit, rps, st);
+}
+
+// the (row tiles, tail MFMAs) pairs an n_o in (NP - 32, NP] can have
+template <int NP>
+int launch_exchange_bf16(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
+  constexpr int NI = NP / 16;
+  const Bf16Shape sh = bf16_shape(h->no);
+  if (sh.np != NP) { set_error("internal: bf16 padding"); return MBPT_ERR_VALUE; }
+  if (sh.nie == NI - 1 && sh.tail == 1) launch_exchange_bf16_k<NP, NI - 1, 1>(h, k, nsplit, rps, st);
+  else if (sh.nie == NI - 1 && sh.tail == 2) launch_exchange_bf16_k<NP, NI - 1, 2>(h, k, nsplit, rps, st);
+  else if (sh.nie == NI && sh.tail == 2) launch_exchange_bf16_k<NP, NI, 2>(h, k, nsplit, rps, st);
+  else if (sh.nie == NI && sh.tail == 3) launch_exchange_bf16_k<NP, NI, 3>(h, k, nsplit, rps, st);
+  else { set_error("internal: bf16 shape"); return MBPT_ERR_VALUE; }
+  return MBPT_OK;
 }
 
 template <typename T, int KV>
@@ -626,12 +647,13 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
   if (h->prec == MBPT_BF16X3) {  // exchange, split BF16
     switch (h->nopad) {
-      case 32: launch_exchange_bf16<32>(h, k, nsplit_eff, rps, st); break;
-      case 64: launch_exchange_bf16<64>(h, k, nsplit_eff, rps, st); break;
-      case 96: launch_exchange_bf16<96>(h, k, nsplit_eff, rps, st); break;
-      case 128: launch_exchange_bf16<128>(h, k, nsplit_eff, rps, st); break;
+      case 32: rc = launch_exchange_bf16<32>(h, k, nsplit_eff, rps, st); break;
+      case 64: rc = launch_exchange_bf16<64>(h, k, nsplit_eff, rps, st); break;
+      case 96: rc = launch_exchange_bf16<96>(h, k, nsplit_eff, rps, st); break;
+      case 128: rc = launch_exchange_bf16<128>(h, k, nsplit_eff, rps, st); break;
       default: set_error("internal: nopad"); return MBPT_ERR_VALUE;
     }
+    if (rc) return rc;
   } else switch (h->nopad) {  // exchange, FP32 / FP64
     case 16: launch_exchange<T, 16>(h, k, nsplit_eff, rps, st); break;
     case 32: launch_exchange<T, 32>(h, k, nsplit_eff, rps, st); break;
@@ -790,15 +812,14 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
     if ((rc = launch_status("X_v padded copies"))) return fail(rc);
   }
   if (precision == MBPT_BF16X3) {
-    const long slots = (long)((M + 31) / 32) * bf16_chunk_slots(h->nopad);
+    const long slots = (long)((M + 31) / 32) * bf16_shape(nocc).spc;
     he = hipMalloc((void **)&h->frag, sizeof(bf16x8) * slots);
     if (he != hipSuccess) {
       set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
       return fail(MBPT_ERR_HIP);
     }
     h->held += sizeof(bf16x8) * slots;
-    k_pack_bf16_xo<<<grid1(slots), 256, 0, st>>>(M, nocc, h->nopad, (const float *)h->Xo,
-                                                  h->frag);
+    k_pack_bf16_xo<<<grid1(slots), 256, 0, st>>>(M, nocc, (const float *)h->Xo, h->frag);
     if ((rc = launch_status("k_pack_bf16_xo"))) return fail(rc);
   }
   *out = h;
