@@ -206,47 +206,72 @@ __global__ __launch_bounds__(256) void k_exchange(
   }
 }
 
-// u_q[b][nu] = sum_{mu in chunk q} p[b][mu] V[mu][nu]: V streamed once per pass
-// (each thread one column nu, coalesced rows of V), p staged in LDS 64 rows at
-// a time and broadcast; up to KV vectors per pass in registers.
-template <typename T, int KV>
+// u_q[b][nu] = sum_{mu in chunk q} p[b][mu] V[mu][nu] on 16x16x4 MFMAs (FP32 or
+// FP64): a wave owns 64 columns (four 16-column tiles), a workgroup 256; the
+// trial vectors go in KT tiles of 16 (16 KT per pass over V, so V is streamed
+// once for k <= 64).  p is staged in LDS 64 rows at a time (row-major [mu][b],
+// the A operand read straight from it); each lane loads its 16 values of V
+// (4 rows x 4 tiles) before the MFMAs that use them.
+template <typename T, int KT>
 __global__ __launch_bounds__(256) void k_hartree_u(int M, int k, int chunk,
                                                    const T *__restrict__ p,
                                                    const T *__restrict__ V,
                                                    T *__restrict__ u) {
-  __shared__ T ps[KV][64];
-  const int nu = blockIdx.x * 256 + threadIdx.x, q = blockIdx.y;
+  constexpr int KV = 16 * KT, LD = KV + 1;
+  __shared__ T ps[64][LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  const int nu0 = blockIdx.x * 256 + wave * 64, q = blockIdx.y;
   const int mu_beg = q * chunk, mu_end = min(M, mu_beg + chunk);
   for (int v0 = 0; v0 < k; v0 += KV) {
-    const int nvec = min(KV, k - v0);
-    T acc[KV];
+    typename MF<T>::acc acc[KT][4];
 #pragma unroll
-    for (int v = 0; v < KV; ++v) acc[v] = 0;
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) zero(acc[kt][ct]);
     for (int mg = mu_beg; mg < mu_end; mg += 64) {
       __syncthreads();
-      for (int e = threadIdx.x; e < KV * 64; e += 256) {
+      for (int e = tid; e < KV * 64; e += 256) {
         const int v = e >> 6, r = e & 63, mu = mg + r;
-        ps[v][r] = (v < nvec && mu < mu_end) ? p[(long)(v0 + v) * M + mu] : T(0);
+        ps[r][v] = (v0 + v < k && mu < mu_end) ? p[(long)(v0 + v) * M + mu] : T(0);
       }
       __syncthreads();
-      if (nu < M) {
-        // 16 rows of V in flight per thread before the FMAs that use them
-        for (int r0 = 0; r0 < 64 && mg + r0 < mu_end; r0 += 16) {
-          T x[16];
+      const int rows = min(64, mu_end - mg);
+      for (int s0 = 0; s0 < rows; s0 += 16) {
+        T x[4][4];
 #pragma unroll
-          for (int q = 0; q < 16; ++q) {
-            const int mu = mg + r0 + q;
-            x[q] = mu < mu_end ? V[(long)mu * M + nu] : T(0);
+        for (int st = 0; st < 4; ++st) {
+          const int mu = mg + s0 + 4 * st + lq;
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct) {
+            const int nu = nu0 + 16 * ct + lr;
+            x[st][ct] = (mu < mu_end && nu < M) ? V[(long)mu * M + nu] : T(0);
           }
+        }
 #pragma unroll
-          for (int q = 0; q < 16; ++q)
+        for (int st = 0; st < 4; ++st) {
+          T a[KT];
 #pragma unroll
-            for (int v = 0; v < KV; ++v) acc[v] += ps[v][r0 + q] * x[q];
+          for (int kt = 0; kt < KT; ++kt) a[kt] = ps[s0 + 4 * st + lq][16 * kt + lr];
+#pragma unroll
+          for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+              acc[kt][ct] = MF<T>::mma(a[kt], x[st][ct], acc[kt][ct]);
         }
       }
     }
-    if (nu < M)
-      for (int v = 0; v < nvec; ++v) u[((long)q * k + v0 + v) * M + nu] = acc[v];
+#pragma unroll
+    for (int kt = 0; kt < KT; ++kt)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct) {
+        const int nu = nu0 + 16 * ct + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int b = v0 + 16 * kt + MF<T>::orow(lq, r);
+          if (b < k && nu < M) u[((long)q * k + b) * M + nu] = acc[kt][ct][r];
+        }
+      }
   }
 }
 
@@ -573,10 +598,10 @@ int launch_exchange_bf16(const mbpt_sigma *h, int k, int nsplit, int rps, hipStr
   return MBPT_OK;
 }
 
-template <typename T, int KV>
+template <typename T, int KT>
 void launch_hartree_u(int M, int k, int chunk, int nqu, const T *p, const T *V, T *u,
                       hipStream_t st) {
-  k_hartree_u<T, KV><<<dim3((M + 255) / 256, nqu), 256, 0, st>>>(M, k, chunk, p, V, u);
+  k_hartree_u<T, KT><<<dim3((M + 255) / 256, nqu), 256, 0, st>>>(M, k, chunk, p, V, u);
 }
 
 template <typename T>
@@ -635,9 +660,11 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     k_hartree_p<T><<<grid1((long)k * M), 256, 0, st>>>(M, no, k, (const T *)h->XoT,
                                                         zXv, p);
     if ((rc = launch_status("k_hartree_p"))) return rc;
-    if (k <= 16) launch_hartree_u<T, 16>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
-    else if (k <= 32) launch_hartree_u<T, 32>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
-    else launch_hartree_u<T, 64>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    if (k <= 16) launch_hartree_u<T, 1>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    else if (k <= 32) launch_hartree_u<T, 2>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    else if (k <= 48 || sizeof(T) == 8)   // FP64: 48 vectors a pass keeps two waves per SIMD
+      launch_hartree_u<T, 3>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
+    else launch_hartree_u<T, 4>(M, k, hchunk, nqu, p, (const T *)h->V, u, st);
     if ((rc = launch_status("k_hartree_u"))) return rc;
     if (nqu > 1) {   // one sum of the chunk partials, read once per (vector, point) below
       k_sum_partials<T><<<grid1((long)k * M), 256, 0, st>>>((long)k * M, nqu, u);
