@@ -137,21 +137,31 @@ __global__ void k_hartree_p(int M, int no, int k, const T *__restrict__ XoT,
 // The P result layout (row orow(l/16, r) of lane l) is used directly as the B
 // operand of step r of the second product, whose k index is taken to be that
 // same row, so no shuffle is needed between the two MFMAs.
+//
+// X_o comes packed (Xp: rows padded to 32, columns to NOPAD, zeros outside
+// M x n_o) and streams through double-buffered 32-row LDS stages by direct
+// 16-byte global-to-LDS loads, one instruction per row (NOPAD * size / 16
+// lanes), issued a stage ahead; the next 16 rows' W are prefetched into
+// registers.  Rows of a stage past M are zero rows of Xp, so P = 0 there and W
+// is read at a clamped row.  LD keeps every LDS row 16-byte aligned.
+typedef __attribute__((address_space(3))) void lds_ptr_t;
 template <typename T, int NOPAD>
 __global__ __launch_bounds__(256) void k_exchange(
-    int M, int no, int k, const T *__restrict__ Xo, const T *__restrict__ W,
+    int M, int no, int k, const T *__restrict__ Xp, const T *__restrict__ W,
     const T *__restrict__ zXv, T *__restrict__ Tpart, int nsplit,
     int rows_per_split, int nvb, int nblk) {
+  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, ROWS = 32;
+  constexpr int LD = NOPAD + 16 / (int)sizeof(T), NL = NOPAD * (int)sizeof(T) / 16;
+  __shared__ __attribute__((aligned(16))) T xs[2][ROWS * LD];
   const int lid = xcd_logical(blockIdx.x, nblk);
   if (lid >= nvb * k * nsplit) return;
   const int b = lid % k, t = lid / k, vb = t % nvb, s = t / nvb;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lq = lane >> 4;
   const int col = vb * 64 + wave * 16 + lr;
   const bool colok = col < M;
-  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, CH = 4, ROWS = CH * 16,
-                LD = NOPAD + 1;
-  __shared__ T xs[ROWS * LD];
+  const int colc = min(col, M - 1);
 
   T bz[NS];
 #pragma unroll
@@ -163,37 +173,56 @@ __global__ __launch_bounds__(256) void k_exchange(
 #pragma unroll
   for (int it = 0; it < NI; ++it) zero(acc[it]);
 
-  const int mu_beg = s * rows_per_split;
-  const int mu_end = min(M, mu_beg + rows_per_split);
-  for (int mg = mu_beg; mg < mu_end; mg += ROWS) {
-    __syncthreads();
-    for (int e = tid; e < ROWS * NOPAD; e += 256) {
-      const int r = e / NOPAD, c = e % NOPAD, mu = mg + r;
-      xs[r * LD + c] = (mu < mu_end && c < no) ? Xo[(long)mu * no + c] : T(0);
+  // rows_per_split is a multiple of 32: a stage never straddles two splits
+  const int st_beg = (s * rows_per_split) / ROWS;
+  const int st_end = min((M + ROWS - 1) / ROWS, ((s + 1) * rows_per_split) / ROWS);
+  auto fetch = [&](int st, int n) {
+#pragma unroll
+    for (int q = 0; q < ROWS / 4; ++q) {
+      const int r = 4 * q + wave;   // wave-uniform row of the stage
+      if (lane < NL)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)((const char *)(Xp + (long)(ROWS * st + r) * NOPAD) + 16 * lane),
+            (lds_ptr_t *)&xs[n][r * LD], 16, 0, 0);
     }
+  };
+  auto load_w = [&](int mu0, T (&w)[4]) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) w[r] = W[(long)min(mu0 + MF<T>::orow(lq, r), M - 1) * M + colc];
+  };
+  if (st_beg < st_end) {
+    fetch(st_beg, 0);
+    T w[4];
+    load_w(ROWS * st_beg, w);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    int cur = 0;
+    for (int st = st_beg; st < st_end; ++st) {
+      if (st + 1 < st_end) fetch(st + 1, cur ^ 1);   // xs[cur ^ 1] last read before the barrier
+      const T *x = xs[cur];
 #pragma unroll
-    for (int ch = 0; ch < CH; ++ch) {
-      const int r0 = ch * 16;
-      if (mg + r0 >= mu_end) break;
-      typename MF<T>::acc P;
-      zero(P);
+      for (int ch = 0; ch < ROWS / 16; ++ch) {
+        const int r0 = ch * 16;
+        T wn[4];
+        load_w(ROWS * st + r0 + 16, wn);
+        typename MF<T>::acc P;
+        zero(P);
 #pragma unroll
-      for (int q = 0; q < NS; ++q)
-        P = MF<T>::mma(xs[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
-      T S[4];
+        for (int q = 0; q < NS; ++q) P = MF<T>::mma(x[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
+        T S[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int mu = mg + r0 + MF<T>::orow(lq, r);
-        const T w = (mu < mu_end && colok) ? W[(long)mu * M + col] : T(0);
-        S[r] = P[r] * w;
-      }
-#pragma unroll
-      for (int it = 0; it < NI; ++it)
+        for (int r = 0; r < 4; ++r) S[r] = P[r] * w[r];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          acc[it] = MF<T>::mma(xs[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr],
-                               S[r], acc[it]);
+#pragma unroll
+          for (int it = 0; it < NI; ++it)
+            acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r], acc[it]);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) w[r] = wn[r];
+      }
+      __builtin_amdgcn_s_waitcnt(0);   // this wave's stage loads (and W loads) landed
+      __syncthreads();                 // ... and every other wave's
+      cur ^= 1;
     }
   }
   if (colok) {
@@ -203,6 +232,18 @@ __global__ __launch_bounds__(256) void k_exchange(
       for (int r = 0; r < 4; ++r)
         Tpart[(((long)s * k + b) * NOPAD + 16 * it + MF<T>::orow(lq, r)) * M + col] =
             acc[it][r];
+  }
+}
+
+// Xp (Mp x nopad) = X_o zero-padded, the exchange's staged copy
+template <typename T>
+__global__ void k_pack_xo(int M, int no, int Mp, int nopad, const T *__restrict__ Xo,
+                          T *__restrict__ Xp) {
+  const long n = (long)Mp * nopad;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(e / nopad), c = (int)(e % nopad);
+    Xp[e] = (r < M && c < no) ? Xo[(long)r * no + c] : T(0);
   }
 }
 
@@ -469,6 +510,7 @@ struct mbpt_sigma {
   double kappa;
   size_t esz;              // bytes per working-precision element
   void *Xo, *Xv, *XoT, *W, *V;
+  void *Xp;                // X_o zero-padded to (M rounded to 32) x nopad (FP32 / FP64)
   bf16x8 *frag;            // X_o split and packed in MFMA fragment order (MBPT_BF16X3)
   // FP32 working precision: K-major, zero-padded copies for the head and tail
   // GEMMs, X_v^T (nvp x Mq) and X_v (Mp x nvq); p pads to 16, q pads to 128
@@ -562,7 +604,7 @@ template <typename T, int NOPAD>
 void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
   const int nvb = (h->M + 63) / 64;
   const int nblk = pad_grid((long)nvb * k * nsplit);
-  k_exchange<T, NOPAD><<<nblk, 256, 0, st>>>(h->M, h->no, k, (const T *)h->Xo,
+  k_exchange<T, NOPAD><<<nblk, 256, 0, st>>>(h->M, h->no, k, (const T *)h->Xp,
                                              (const T *)h->W, (const T *)h->ws[WS_ZXV],
                                              (T *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
@@ -838,6 +880,23 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
                                                             h->Mp, h->nvq, h->XvP);
     if ((rc = launch_status("X_v padded copies"))) return fail(rc);
   }
+  if (precision != MBPT_BF16X3) {  // the exchange's zero-padded X_o
+    const int Mp32 = (M + 31) / 32 * 32;
+    const size_t bytes = e * (size_t)Mp32 * h->nopad;
+    he = hipMalloc(&h->Xp, bytes);
+    if (he != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
+      return fail(MBPT_ERR_HIP);
+    }
+    h->held += bytes;
+    if (precision == MBPT_FP64)
+      k_pack_xo<double><<<grid1((long)Mp32 * h->nopad), 256, 0, st>>>(
+          M, nocc, Mp32, h->nopad, (const double *)h->Xo, (double *)h->Xp);
+    else
+      k_pack_xo<float><<<grid1((long)Mp32 * h->nopad), 256, 0, st>>>(
+          M, nocc, Mp32, h->nopad, (const float *)h->Xo, (float *)h->Xp);
+    if ((rc = launch_status("k_pack_xo"))) return fail(rc);
+  }
   if (precision == MBPT_BF16X3) {
     const long slots = (long)((M + 31) / 32) * bf16_shape(nocc).spc;
     he = hipMalloc((void **)&h->frag, sizeof(bf16x8) * slots);
@@ -909,7 +968,7 @@ int mbpt_sigma_destroy(mbpt_sigma *h) {
   if (h->prof)
     for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
   free_ws(h);
-  (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
+  (void)hipFree(h->Xo); (void)hipFree(h->Xp); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
   (void)hipFree(h->frag); (void)hipFree(h->XvT); (void)hipFree(h->XvP);
   (void)hipFree(h->d);
   delete h;
