@@ -34,6 +34,8 @@ struct Inst {
   int nsplit, rps, nq, kch, nqu, kchu;
 };
 
+typedef __attribute__((address_space(3))) void lds_ptr;
+
 struct Tables {
   const Inst *inst;
   const int *pre;   // n + 1 prefix sums of the work items
@@ -235,9 +237,16 @@ template <typename T, int NOPAD>
 __global__ __launch_bounds__(256) void g_exchange(Tables t, int k, const int *__restrict__ kact,
                                                   const T *__restrict__ pool,
                                                   T *__restrict__ ws) {
-  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, ROWS = 64, LD = NOPAD + 1;
-  __shared__ T xs[ROWS * LD];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // X_o of an instance sits in the pool zero-padded to (M rounded to 32) x
+  // NOPAD, and streams through double-buffered 32-row LDS stages by direct
+  // 16-byte global-to-LDS loads (one instruction per row, a stage ahead); the
+  // next 16 rows' W are prefetched.  Rows past M are zero rows (P = 0 there,
+  // W read at a clamped row).
+  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, ROWS = 32;
+  constexpr int LD = NOPAD + 16 / (int)sizeof(T), NL = NOPAD * (int)sizeof(T) / 16;
+  __shared__ __attribute__((aligned(16))) T xs[2][ROWS * LD];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lq = lane >> 4;
   const int G = gridDim.x;
   for (int item = xcd_logical(blockIdx.x, G); item < t.total; item += G) {
@@ -247,9 +256,10 @@ __global__ __launch_bounds__(256) void g_exchange(Tables t, int k, const int *__
     const int local = item - t.pre[b];
     const int v = local % k, tt = local / k, vb = tt % nvb, s = tt / nvb;
     if (v >= live(kact, I.id, k)) continue;
-    const T *Xo = pool + I.xo, *W = pool + I.w, *zXv = ws + I.zxv;
+    const T *Xp = pool + I.xo, *W = pool + I.w, *zXv = ws + I.zxv;
     const int col = vb * 64 + wave * 16 + lr;
     const bool colok = col < M;
+    const int colc = min(col, M - 1);
     T bz[NS];
 #pragma unroll
     for (int q = 0; q < NS; ++q) {
@@ -259,34 +269,59 @@ __global__ __launch_bounds__(256) void g_exchange(Tables t, int k, const int *__
     typename MF<T>::acc acc[NI];
 #pragma unroll
     for (int it = 0; it < NI; ++it) zero(acc[it]);
-    const int mu_beg = s * I.rps, mu_end = min(M, mu_beg + I.rps);
-    for (int mg = mu_beg; mg < mu_end; mg += ROWS) {
-      __syncthreads();
-      for (int e = tid; e < ROWS * NOPAD; e += 256) {
-        const int r = e / NOPAD, c = e % NOPAD, mu = mg + r;
-        xs[r * LD + c] = (mu < mu_end && c < no) ? Xo[(long)mu * no + c] : T(0);
+    // rps is a multiple of 64: a stage never straddles two splits
+    const int st_beg = (s * I.rps) / ROWS;
+    const int st_end = min((M + ROWS - 1) / ROWS, ((s + 1) * I.rps) / ROWS);
+    auto fetch = [&](int st, int n) {
+#pragma unroll
+      for (int q = 0; q < ROWS / 4; ++q) {
+        const int r = 4 * q + wave;
+        if (lane < NL)
+          __builtin_amdgcn_global_load_lds(
+              (const void *)((const char *)(Xp + (long)(ROWS * st + r) * NOPAD) + 16 * lane),
+              (lds_ptr *)&xs[n][r * LD], 16, 0, 0);
       }
+    };
+    auto load_w = [&](int mu0, T (&w)[4]) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        w[r] = W[(long)min(mu0 + MF<T>::orow(lq, r), M - 1) * M + colc];
+    };
+    if (st_beg < st_end) {
+      __syncthreads();   // the previous item's last stage is no longer read
+      fetch(st_beg, 0);
+      T w[4];
+      load_w(ROWS * st_beg, w);
+      __builtin_amdgcn_s_waitcnt(0);
       __syncthreads();
+      int cur = 0;
+      for (int st = st_beg; st < st_end; ++st) {
+        if (st + 1 < st_end) fetch(st + 1, cur ^ 1);
+        const T *x = xs[cur];
 #pragma unroll
-      for (int ch = 0; ch < ROWS / 16; ++ch) {
-        const int r0 = ch * 16;
-        if (mg + r0 >= mu_end) break;
-        typename MF<T>::acc P;
-        zero(P);
+        for (int ch = 0; ch < ROWS / 16; ++ch) {
+          const int r0 = ch * 16;
+          T wn[4];
+          load_w(ROWS * st + r0 + 16, wn);
+          typename MF<T>::acc P;
+          zero(P);
 #pragma unroll
-        for (int q = 0; q < NS; ++q) P = MF<T>::mma(xs[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
-        T S[4];
+          for (int q = 0; q < NS; ++q) P = MF<T>::mma(x[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
+          T S[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int mu = mg + r0 + MF<T>::orow(lq, r);
-          S[r] = P[r] * ((mu < mu_end && colok) ? W[(long)mu * M + col] : T(0));
-        }
-#pragma unroll
-        for (int it = 0; it < NI; ++it)
+          for (int r = 0; r < 4; ++r) S[r] = P[r] * w[r];
 #pragma unroll
           for (int r = 0; r < 4; ++r)
-            acc[it] = MF<T>::mma(xs[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r],
-                                 acc[it]);
+#pragma unroll
+            for (int it = 0; it < NI; ++it)
+              acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r],
+                                   acc[it]);
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w[r] = wn[r];
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        cur ^= 1;
       }
     }
     if (colok) {
@@ -416,7 +451,7 @@ __global__ __launch_bounds__(256) void g_sdiag(Tables t, const T *__restrict__ p
 
 // ---- operand upload: X split into X_o, X_v, X_o^T; W and V converted ----------------------
 template <typename T>
-__global__ void g_split_X(int M, int no, int nv, const double *__restrict__ X,
+__global__ void g_split_X(int M, int no, int nv, int nopad, const double *__restrict__ X,
                           T *__restrict__ Xo, T *__restrict__ Xv, T *__restrict__ XoT) {
   const long n = (long)M * (no + nv);
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
@@ -424,7 +459,7 @@ __global__ void g_split_X(int M, int no, int nv, const double *__restrict__ X,
     const int mu = (int)(e / (no + nv)), p = (int)(e % (no + nv));
     const T x = (T)X[e];
     if (p < no) {
-      Xo[(long)mu * no + p] = x;
+      Xo[(long)mu * nopad + p] = x;   // the padding stays zero (pool cleared at create)
       XoT[(long)p * M + mu] = x;
     } else {
       Xv[(long)mu * nv + (p - no)] = x;
@@ -490,7 +525,8 @@ template <typename T> int upload(mbpt_batch *h, const double *const *X, const do
   for (int b = 0; b < h->n; ++b) {
     const Inst &I = h->hinst[b];
     const long M = I.M;
-    g_split_X<T><<<grid1(M * (I.no + I.nv)), 256, 0, st>>>(I.M, I.no, I.nv, X[b], pool + I.xo,
+    g_split_X<T><<<grid1(M * (I.no + I.nv)), 256, 0, st>>>(I.M, I.no, I.nv, I.nopad, X[b],
+                                                           pool + I.xo,
                                                            pool + I.xv, pool + I.xot);
     g_convert<T><<<grid1(M * M), 256, 0, st>>>(M * M, W[b], pool + I.w);
     if (h->kappa != 0.0) g_convert<T><<<grid1(M * M), 256, 0, st>>>(M * M, V[b], pool + I.v);
@@ -671,7 +707,7 @@ int mbpt_batch_create(mbpt_batch **out, int n, const int *M, const int *nocc, co
     I.nopad = (I.no + 15) / 16 * 16;
     npmax = std::max(npmax, I.no * I.nv);
     const long Mb = I.M;
-    I.xo = fo; fo += Mb * I.no;
+    I.xo = fo; fo += (Mb + 31) / 32 * 32 * I.nopad;   // zero-padded for the exchange
     I.xv = fo; fo += Mb * I.nv;
     I.xot = fo; fo += Mb * I.no;
     I.w = fo; fo += Mb * Mb;
@@ -690,6 +726,8 @@ int mbpt_batch_create(mbpt_batch **out, int n, const int *M, const int *nocc, co
     eo += I.no + I.nv;
   }
   hipError_t e = hipMalloc(&h->pool, h->esz * fo);
+  // zero padding of the packed X_o (the upload writes only the real entries)
+  if (e == hipSuccess) e = hipMemsetAsync(h->pool, 0, h->esz * fo, st);
   if (e == hipSuccess) e = hipMalloc(&h->dpool, sizeof(double) * dof);
   if (e == hipSuccess)
     e = hipMemcpyAsync(h->dpool, dh.data(), sizeof(double) * dof, hipMemcpyHostToDevice, st);
