@@ -38,6 +38,11 @@ __device__ __forceinline__ float silu_grad(float z) {
   const float s = 1.f / (1.f + __expf(-z));
   return s * (1.f + z * (1.f - s));
 }
+// SiLU with the hardware exponential and reciprocal (inference: the result is
+// rounded to BF16 or scaled by a BF16-trained weight next)
+__device__ __forceinline__ float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.f + __expf(-z));
+}
 __device__ __forceinline__ f32x4 mma(bf16x8 a, bf16x8 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
@@ -88,7 +93,7 @@ struct Feat {
   const float *X;                  // mode 0: (N, 3)
   const double *R;                 // mode 1: residuals (n, k), row stride ldr
   const double *d, *theta, *rms;   // (n), (k), (k)
-  int n, ldr;
+  int kc, ldr;                     // mode 1: network row = i k + j (pair i, column j)
   Batch bt;                        // mode 2 when bt.off is set
   __device__ void get(long row, float (&x)[3]) const {
     if (X) {
@@ -101,7 +106,7 @@ struct Feat {
       x[1] = (float)(d[pos] - theta[c]);
       x[2] = (float)theta[c];
     } else {
-      const int j = (int)(row / n), i = (int)(row % n);
+      const int i = (int)(row / kc), j = (int)(row % kc);
       x[0] = (float)(R[(long)i * ldr + j] / rms[j]);
       x[1] = (float)(d[i] - theta[j]);
       x[2] = (float)theta[j];
@@ -113,7 +118,7 @@ struct Out {
   float *y;                        // mode 0: (N)
   double *T;                       // mode 1: corrections (n, k), row stride ldt
   const double *rms;
-  int n, ldt;
+  int kc, ldt;                     // mode 1: network row = i k + j
   Batch bt;                        // mode 2 when bt.off is set
   __device__ void put(long row, float v) const {
     if (y) {
@@ -124,7 +129,7 @@ struct Out {
       bt.locate(row, pos, c);
       T[pos * ldt + c % bt.k] = (double)v * rms[c];
     } else {
-      const int j = (int)(row / n), i = (int)(row % n);
+      const int i = (int)(row / kc), j = (int)(row % kc);
       T[(long)i * ldt + j] = (double)v * rms[j];
     }
   }
@@ -195,6 +200,96 @@ __global__ __launch_bounds__(256) void k_mlp_fwd(long N, const float *__restrict
       const long row = row0 + 4 * lq + r;
       if (lr == 0 && row < N) out.put(row, v + b3);
     }
+  }
+}
+
+// Inference forward (no saved activations): 8 waves share one LDS copy of W2;
+// a wave takes 64 rows (four 16-row tiles) per step, so each W2 fragment read
+// from LDS feeds four MFMAs, and each lane's layer-1 weights (8 consecutive
+// hidden units per K block, two 16-byte reads per array) serve all four tiles.
+constexpr int kInferRT = 4;   // 16-row tiles per wave step
+__global__ __launch_bounds__(512) void k_mlp_infer(long N, const float *__restrict__ P,
+                                                   const bf16x8 *__restrict__ wfwd,
+                                                   Feat feat, Out out) {
+  extern __shared__ unsigned char smem_raw[];
+  bf16x8 *sw = (bf16x8 *)smem_raw;
+  float *sp = (float *)(smem_raw + sizeof(bf16x8) * KB * NT * 64);  // W1, b1, b2, w3
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lr = lane & 15, lq = lane >> 4;
+  for (int e = tid; e < KB * NT * 64; e += 512) sw[e] = wfwd[e];
+  for (int e = tid; e < 4 * H; e += 512) sp[e] = P[e];
+  for (int e = tid; e < H; e += 512) {
+    sp[4 * H + e] = P[OFF_B2 + e];
+    sp[5 * H + e] = P[OFF_W3 + e];
+  }
+  __syncthreads();
+  const float b3 = P[OFF_B3];
+  constexpr int RT = kInferRT;
+  const long nsteps = (N + 16 * RT - 1) / (16 * RT);
+  for (long step = (long)blockIdx.x * 8 + wave; step < nsteps; step += (long)gridDim.x * 8) {
+    const long row0 = step * 16 * RT;
+    float x[RT][3];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const long row = row0 + 16 * rt + lr;
+      x[rt][0] = x[rt][1] = x[rt][2] = 0.f;
+      if (row < N) feat.get(row, x[rt]);
+    }
+    bf16x8 a[RT][KB];
+#pragma unroll
+    for (int kb = 0; kb < KB; ++kb) {
+      const int h0 = 32 * kb + 8 * lq;
+      float wa[8], wb[8], wc[8], bb[8];
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        wa[s] = sp[h0 + s];
+        wb[s] = sp[H + h0 + s];
+        wc[s] = sp[2 * H + h0 + s];
+        bb[s] = sp[3 * H + h0 + s];
+      }
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          a[rt][kb][s] = (__bf16)silu_fast(x[rt][0] * wa[s] + x[rt][1] * wb[s] +
+                                           x[rt][2] * wc[s] + bb[s]);
+      __builtin_amdgcn_sched_barrier(0);   // keep each block's weight loads local
+    }
+    float part[RT][4];
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) part[rt][r] = 0.f;
+#pragma unroll 2
+    for (int t = 0; t < NT; ++t) {
+      f32x4 acc[RT];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) zero(acc[rt]);
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const bf16x8 bw = sw[(kb * NT + t) * 64 + lane];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) acc[rt] = mma(a[rt][kb], bw, acc[rt]);
+      }
+      const int n = 16 * t + lr;
+      const float b2 = sp[4 * H + n], w3 = sp[5 * H + n];
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) part[rt][r] += silu_fast(acc[rt][r] + b2) * w3;
+    }
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        float v = part[rt][r];
+        v += __shfl_xor(v, 1);
+        v += __shfl_xor(v, 2);
+        v += __shfl_xor(v, 4);
+        v += __shfl_xor(v, 8);
+        const long row = row0 + 16 * rt + 4 * lq + r;
+        if (lr == 0 && row < N) out.put(row, v + b3);
+      }
   }
 }
 
@@ -481,6 +576,11 @@ int grid_rows(long N) {
   return (int)std::max<long>(1, std::min<long>(kRowBlocks, (N + 63) / 64));
 }
 
+// persistent inference workgroups: at most one per CU (W2 fills most of the LDS)
+int grid_infer(long N) {
+  return (int)std::max<long>(1, std::min<long>(kRowBlocks, (N + 511) / 512));
+}
+
 constexpr long kDw2Chunks = 64;
 
 int gradient(mbpt_mlp *h, long N, const float *X, const float *T, hipStream_t st) {
@@ -533,6 +633,8 @@ int mbpt_mlp_create(mbpt_mlp **out, const float *params_host, void *stream) {
   if (!e) e = hipStreamSynchronize(st);
   if (!e) e = hipFuncSetAttribute((const void *)k_mlp_fwd,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFwdSmem);
+  if (!e) e = hipFuncSetAttribute((const void *)k_mlp_infer,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)kFwdSmem);
   if (!e) e = hipFuncSetAttribute((const void *)k_mlp_bwd_rows,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)kBwdSmem);
   if (e != hipSuccess) {
@@ -562,8 +664,8 @@ int mbpt_mlp_forward(mbpt_mlp *h, long N, const float *X_dev, float *y_dev, void
   hipStream_t st = (hipStream_t)stream;
   Feat f{X_dev, nullptr, nullptr, nullptr, nullptr, 0, 0};
   Out o{y_dev, nullptr, nullptr, 0, 0};
-  k_mlp_fwd<<<grid_rows(N), 256, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o, nullptr, nullptr);
-  return launch_status("k_mlp_fwd");
+  k_mlp_infer<<<grid_infer(N), 512, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o);
+  return launch_status("k_mlp_infer");
 }
 
 int mbpt_mlp_gradient(mbpt_mlp *h, long N, const float *X_dev, const float *T_dev,
@@ -619,10 +721,10 @@ int mbpt_mlp_correction(mbpt_mlp *h, int n, int k, const double *R_dev, int ldr,
   int rc = launch_status("k_col_rms");
   if (rc) return rc;
   const long N = (long)n * k;
-  Feat f{nullptr, R_dev, d_dev, theta_dev, h->rms, n, ldr};
-  Out o{nullptr, T_dev, h->rms, n, ldt};
-  k_mlp_fwd<<<grid_rows(N), 256, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o, nullptr, nullptr);
-  return launch_status("k_mlp_fwd");
+  Feat f{nullptr, R_dev, d_dev, theta_dev, h->rms, k, ldr};
+  Out o{nullptr, T_dev, h->rms, k, ldt};
+  k_mlp_infer<<<grid_infer(N), 512, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o);
+  return launch_status("k_mlp_infer");
 }
 
 int mbpt_mlp_correction_batched(mbpt_mlp *h, int nbatch, int P, int k, const int *off_dev,
@@ -650,8 +752,8 @@ int mbpt_mlp_correction_batched(mbpt_mlp *h, int nbatch, int P, int k, const int
   const long N = ntot * k;
   Feat f{nullptr, R_dev, d_dev, theta_dev, h->rms, 0, ldr, bt};
   Out o{nullptr, T_dev, h->rms, 0, ldt, bt};
-  k_mlp_fwd<<<grid_rows(N), 256, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o, nullptr, nullptr);
-  return launch_status("k_mlp_fwd");
+  k_mlp_infer<<<grid_infer(N), 512, kFwdSmem, st>>>(N, h->P, h->wfwd, f, o);
+  return launch_status("k_mlp_infer");
 }
 
 int mbpt_mlp_destroy(mbpt_mlp *h) {
