@@ -15,6 +15,7 @@
 // a workgroup reads is shared by the k workgroups of one XCD through its L2.
 #include "common.h"
 #include "exchange_bf16.h"
+#include "gemm_nt3.h"
 #include "gemm_km.h"
 
 #include <algorithm>
@@ -362,6 +363,31 @@ __global__ __launch_bounds__(256) void k_tail_prep_t(int k, int no, int nopad, i
   }
 }
 
+// T' = sum_s Tpart_s - kappa X_o^T o u as hi / lo BF16 rows [m][mu] (m = b n_o
+// + i), zero-padded to ldr x Mp32: the A operand of the split-BF16 tail GEMM
+__global__ void k_tail_prep_b3(int k, int no, int nopad, int M, int Mp32, int ldr, int nsplit,
+                               const float *__restrict__ Tp, const float *__restrict__ XoT,
+                               const float *__restrict__ u, float kappa,
+                               __bf16 *__restrict__ Th, __bf16 *__restrict__ Tl) {
+  const int rows = k * no;
+  const long sstride = (long)k * nopad * M, n = (long)ldr * Mp32;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int m = (int)(e / Mp32), mu = (int)(e % Mp32);
+    float val = 0.f;
+    if (m < rows && mu < M) {
+      const int b = m / no, i = m - b * no;
+      const long off = ((long)b * nopad + i) * M + mu;
+      val = Tp[off];
+      for (int q = 1; q < nsplit; ++q) val += Tp[off + q * sstride];
+      if (u) val -= kappa * XoT[(long)i * M + mu] * u[(long)b * M + mu];
+    }
+    const __bf16 h = (__bf16)val;
+    Th[e] = h;
+    Tl[e] = (__bf16)(val - (float)h);
+  }
+}
+
 // row-padded copy out[r][c] = in[r][c] (zero outside rows x cols) of pr x pc
 __global__ void k_pad_copy(int rows, int cols, const float *__restrict__ in, int pr, int pc,
                            float *__restrict__ out) {
@@ -516,6 +542,10 @@ struct mbpt_sigma {
   // GEMMs, X_v^T (nvp x Mq) and X_v (Mp x nvq); p pads to 16, q pads to 128
   float *XvT, *XvP;
   int nvp, Mq, Mp, nvq;
+  // split-BF16 head and tail operands (hi, lo), K contiguous: X_v (Mq x nvp32)
+  // and X_v^T (nvq x Mp32); nvp32 / Mp32 pad n_vir / M to 32
+  __bf16 *XvB[2], *XvTB[2];
+  int nvp32, Mp32;
   double *d;
   // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
   void *ws[7];
@@ -576,7 +606,7 @@ int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu, int ldr) {
                               e * (size_t)nsplit * k * h->nopad * M,
                               km ? 0 : e * k * no * M,
                               e * (size_t)nq * k * no * nv,
-                              km ? 4 * (size_t)std::max(h->nvp, h->Mp) * ldr : 0};
+                              km ? 4 * (size_t)std::max(h->nvp32, h->Mp32) * ldr : 0};
   for (int q = 0; q < WS_N; ++q) {
     int rc = ensure_buf(h, q, sizes[q]);
     if (rc) return rc;
@@ -656,7 +686,13 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   // tail: K = M split in nq chunks of kchunk (multiples of 16), enough
   // workgroups for the skinny output
   int nq, kchunk;
-  if (km) {
+  const bool b3 = h->prec == MBPT_BF16X3;   // split-BF16 head and tail GEMMs
+  if (b3) {
+    const int tiles = (ldr / 128) * (h->nvq / 128);
+    nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp32 / 64));
+    kchunk = ((h->Mp32 + nq - 1) / nq + 31) / 32 * 32;
+    nq = (h->Mp32 + kchunk - 1) / kchunk;
+  } else if (km) {
     const int tiles = (ldr / 128) * ((nv + 127) / 128);
     nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp / 64));
     kchunk = ((h->Mp + nq - 1) / nq + 15) / 16 * 16;
@@ -679,7 +715,18 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   const int nsplit_eff = (M + rps - 1) / rps;
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[0], st));
 
-  if (km) {  // head, K-major: Z^T padded, then (Z^T)^T X_v^T
+  if (b3) {  // head, split BF16: z as hi / lo rows, times X_v^T
+    __bf16 *Zh = (__bf16 *)h->ws[WS_ZT], *Zl = Zh + (size_t)ldr * h->nvp32;
+    k_split_rows_bf16<double, false><<<grid1((long)ldr * h->nvp32), 256, 0, st>>>(
+        rows, nv, z, nv, ldr, h->nvp32, Zh, Zl);
+    if ((rc = launch_status("k_split_rows_bf16(z)"))) return rc;
+    const int ntm = ldr / 128, ntn = h->Mq / 128;
+    const int nblk = pad_grid((long)ntm * ntn);
+    k_gemm_nt3<EpiStore><<<nblk, 256, 0, st>>>(h->nvp32, h->nvp32, Zh, Zl, h->nvp32, h->XvB[0],
+                                                h->XvB[1], h->nvp32, ntm, ntn, 1,
+                                                EpiStore{(float *)zXv, rows, M, M}, nblk);
+    if ((rc = launch_status("k_gemm_nt3(head)"))) return rc;
+  } else if (km) {  // head, K-major: Z^T padded, then (Z^T)^T X_v^T
     float *Zt = (float *)h->ws[WS_ZT];
     k_transpose_pad<double><<<dim3(ldr / 64, (h->nvp + 63) / 64), 256, 0, st>>>(
         rows, nv, z, nv, Zt, ldr, ldr, h->nvp);
@@ -736,7 +783,20 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   }
   if ((rc = launch_status("k_exchange"))) return rc;
   if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
-  if (km) {  // tail, K-major: T'^T padded, then (T'^T)^T X_v split over K
+  if (b3) {  // tail, split BF16: T' as hi / lo rows, times X_v, split over K
+    __bf16 *Th = (__bf16 *)h->ws[WS_ZT], *Tl = Th + (size_t)ldr * h->Mp32;
+    k_tail_prep_b3<<<grid1((long)ldr * h->Mp32), 256, 0, st>>>(
+        k, no, h->nopad, M, h->Mp32, ldr, nsplit_eff, (const float *)Tpart,
+        (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr,
+        (float)h->kappa, Th, Tl);
+    if ((rc = launch_status("k_tail_prep_b3"))) return rc;
+    const int ntm = ldr / 128, ntn = h->nvq / 128;
+    const int nblk = pad_grid((long)ntm * ntn * nq);
+    k_gemm_nt3<EpiStore><<<nblk, 256, 0, st>>>(h->Mp32, kchunk, Th, Tl, h->Mp32, h->XvTB[0],
+                                                h->XvTB[1], h->Mp32, ntm, ntn, nq,
+                                                EpiStore{(float *)Ep, rows, nv, nv}, nblk);
+    if ((rc = launch_status("k_gemm_nt3(tail)"))) return rc;
+  } else if (km) {  // tail, K-major: T'^T padded, then (T'^T)^T X_v split over K
     float *Tct = (float *)h->ws[WS_ZT];
     k_tail_prep_t<<<dim3(ldr / 64, (h->Mp + 63) / 64), 256, 0, st>>>(
         k, no, h->nopad, M, h->Mp, ldr, nsplit_eff, (const float *)Tpart,
@@ -867,6 +927,26 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
     h->Mq = (M + 127) / 128 * 128;
     h->Mp = (M + 15) / 16 * 16;
     h->nvq = (nvir + 127) / 128 * 128;
+    h->nvp32 = (nvir + 31) / 32 * 32;
+    h->Mp32 = (M + 31) / 32 * 32;
+  }
+  if (precision == MBPT_BF16X3) {  // split-BF16 X_v (head) and X_v^T (tail), hi / lo rows
+    const size_t nb = (size_t)h->Mq * h->nvp32, nt = (size_t)h->nvq * h->Mp32;
+    for (int hl = 0; hl < 2 && he == hipSuccess; ++hl) {
+      he = hipMalloc((void **)&h->XvB[hl], sizeof(__bf16) * nb);
+      if (he == hipSuccess) he = hipMalloc((void **)&h->XvTB[hl], sizeof(__bf16) * nt);
+    }
+    if (he != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
+      return fail(MBPT_ERR_HIP);
+    }
+    h->held += sizeof(__bf16) * 2 * (nb + nt);
+    k_split_rows_bf16<float, false><<<grid1((long)nb), 256, 0, st>>>(
+        M, nvir, (const float *)h->Xv, nvir, h->Mq, h->nvp32, h->XvB[0], h->XvB[1]);
+    k_split_rows_bf16<float, true><<<grid1((long)nt), 256, 0, st>>>(
+        nvir, M, (const float *)h->Xv, nvir, h->nvq, h->Mp32, h->XvTB[0], h->XvTB[1]);
+    if ((rc = launch_status("split-BF16 X_v copies"))) return fail(rc);
+  } else if (precision != MBPT_FP64) {
     he = hipMalloc((void **)&h->XvT, sizeof(float) * (size_t)h->nvp * h->Mq);
     if (he == hipSuccess) he = hipMalloc((void **)&h->XvP, sizeof(float) * (size_t)h->Mp * h->nvq);
     if (he != hipSuccess) {
@@ -970,6 +1050,7 @@ int mbpt_sigma_destroy(mbpt_sigma *h) {
   free_ws(h);
   (void)hipFree(h->Xo); (void)hipFree(h->Xp); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
   (void)hipFree(h->frag); (void)hipFree(h->XvT); (void)hipFree(h->XvP);
+  for (int hl = 0; hl < 2; ++hl) { (void)hipFree(h->XvB[hl]); (void)hipFree(h->XvTB[hl]); }
   (void)hipFree(h->d);
   delete h;
   return MBPT_OK;
