@@ -42,3 +42,7 @@ DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
 #: Hartree -> meV and -> eV.
 HARTREE_TO_MEV = 27211.386245988
 HARTREE_TO_EV = 27.211386245988
+
+# seconds git may take to name the commit a record was made on
+# (Base/numerics/constants.py:1112)
+GIT_PROVENANCE_TIMEOUT = 10.0
