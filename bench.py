@@ -345,6 +345,41 @@ def leg_large():
     return res
 
 
+def leg_full_action(inst, W, k=20, reps=5):
+    """The full Casida action (A z, B z) against the Tamm-Dancoff A z alone on
+    the headline instance: per-apply times of k vectors and the B block's
+    extra cost (its exchange is 4 n_o M^2 flops per vector, computed in FP32
+    by the split-BF16 handle), with B z's error against FP64."""
+    import torch
+    from mbptcode_amd.block_actions import SigmaOperator
+    z = torch.randn((k, inst.nocc, inst.nvir), dtype=torch.float64, device='cuda')
+    res, Bs = {}, {}
+    for prec in ('fp64', 'bf16x3'):
+        op = SigmaOperator(inst.eps, inst.nocc, inst.X, W, inst.V, precision=prec)
+        times = {}
+        for name, fn in (('A', lambda: op.apply(z)), ('AB', lambda: op.apply_ab(z))):
+            fn()
+            sync()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                fn()
+            e1.record()
+            sync()
+            times[name] = e0.elapsed_time(e1) / reps
+        Bs[prec] = op.apply_ab(z)[1]
+        extra = times['AB'] - times['A']
+        res[prec] = {'A_ms': times['A'], 'AB_ms': times['AB'], 'B_extra_ms': extra,
+                     'B_exchange_tflops': k * 4.0 * inst.nocc * inst.M ** 2 / (extra * 1e-3) / 1e12}
+        del op
+    err = float((Bs['bf16x3'] - Bs['fp64']).norm() / Bs['fp64'].norm())
+    log(f'full action (k={k}): fp64 A {res["fp64"]["A_ms"]:.2f} / A+B {res["fp64"]["AB_ms"]:.2f} ms, '
+        f'bf16x3 A {res["bf16x3"]["A_ms"]:.2f} / A+B {res["bf16x3"]["AB_ms"]:.2f} ms; '
+        f'B rel err bf16x3 vs fp64 {err:.1e}')
+    return {'workload': f'headline instance, k = {k} trial vectors per apply',
+            'precisions': res, 'B_rel_err_bf16x3_vs_fp64': err}
+
+
 def leg_batched():
     """The batched workload: 256 instances, 5 roots each, tol 1e-5."""
     import torch
@@ -459,6 +494,8 @@ def main():
         result['benzene_size'] = leg_benzene()
         log('leg: large instance (errors, E_c, chi0)')
         result['large_instance'] = leg_large()
+        log('leg: full Casida action (B block)')
+        result['full_casida_action'] = leg_full_action(inst, W)
         log('leg: batched workload')
         result['batched'] = leg_batched()
     if not args.no_cpu:

@@ -70,6 +70,14 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir,
 int mbpt_sigma_apply(mbpt_sigma *h, int k, const double *z_dev,
                      double *Az_dev, void *stream);
 
+/* (Az, Bz) = (A z, B z) of the full Casida action for k trial vectors, all
+ * (k, nocc, nvir) FP64 and distinct: `apply_AB(z)` with tda=False,
+ * block_actions.py:563-661 (the B block, kappa (ia|jb) - (ib|W|ja), from
+ * U = (W o P) X_o and U^T X_v, :632-652).  The split-BF16 handle computes the
+ * B block's exchange in FP32. */
+int mbpt_sigma_apply_ab(mbpt_sigma *h, int k, const double *z_dev, double *Az_dev,
+                        double *Bz_dev, void *stream);
+
 /* out = d - (ii|W|aa), (nocc, nvir) FP64: `screened_diagonal()`,
  * block_actions.py:663-689 (grid gauge: (X_o o X_o)^T W (X_v o X_v)). */
 int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream);

@@ -20,7 +20,7 @@ CHI0_MAX_TAU = 64
 
 #: Every symbol include/mbpt_hip.h declares.
 EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
-           'mbpt_sigma_apply', 'mbpt_sigma_screened_diag',
+           'mbpt_sigma_apply', 'mbpt_sigma_apply_ab', 'mbpt_sigma_screened_diag',
            'mbpt_sigma_held_bytes', 'mbpt_sigma_profile',
            'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
            'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
@@ -44,6 +44,7 @@ _SIGNATURES = {
     'mbpt_sigma_create': (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _d, _vp, _vp,
                                _vp, _dp, _vp]),
     'mbpt_sigma_apply': (_i, [_vp, _i, _vp, _vp, _vp]),
+    'mbpt_sigma_apply_ab': (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     'mbpt_sigma_screened_diag': (_i, [_vp, _vp, _vp]),
     'mbpt_sigma_held_bytes': (ctypes.c_size_t, [_vp]),
     'mbpt_sigma_profile': (_i, [_vp, _i]),

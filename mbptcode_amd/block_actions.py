@@ -140,6 +140,26 @@ class SigmaOperator:
         self.applied_calls += 1
         return out
 
+    def apply_ab(self, z, out_a=None, out_b=None):
+        """(A z, B z) of the full Casida action for a device tensor z (k,
+        nocc, nvir) or (k, npair), FP64; fresh tensors of z's shape unless
+        given."""
+        if z.dtype != torch.float64 or not z.is_cuda:
+            raise ValueError('z must be a float64 device tensor')
+        z = z.contiguous()
+        k = z.numel() // self.npair
+        if k * self.npair != z.numel():
+            raise ValueError(f'z has {z.numel()} elements, not a multiple of '
+                             f'the {self.npair} pairs')
+        out_a = torch.empty_like(z) if out_a is None else out_a
+        out_b = torch.empty_like(z) if out_b is None else out_b
+        _lib.check(self._lib.mbpt_sigma_apply_ab(self._h, k, _lib.dptr(z), _lib.dptr(out_a),
+                                                  _lib.dptr(out_b), _lib.stream_ptr()),
+                   'mbpt_sigma_apply_ab')
+        self.applied_vectors += k
+        self.applied_calls += 1
+        return out_a, out_b
+
     def profile(self, enable=True):
         """Start (and zero) or stop per-stage device timing of `apply`."""
         _lib.check(self._lib.mbpt_sigma_profile(self._h, int(enable)),
@@ -171,28 +191,35 @@ class SigmaOperator:
 
 
 def grid_block_action(eps, nocc, X, W, V, kappa=2.0, precision='fp32',
-                      device=None):
-    """(apply_AB, diag_d) from grid-gauge W and V: the Tamm-Dancoff action,
-    apply_AB(z) -> (A z, None) with z (k, nocc, nvir), NumPy or device FP64."""
+                      device=None, tda=True):
+    """(apply_AB, diag_d) from grid-gauge W and V: apply_AB(z) -> (A z, None)
+    with z (k, nocc, nvir), NumPy or device FP64; with tda=False the full
+    Casida action, (A z, B z)."""
     op = SigmaOperator(eps, nocc, X, W, V, kappa=kappa, precision=precision,
                        device=device)
-    return _wrap(op)
+    return _wrap(op, tda=tda)
 
 
-def _wrap(op):
+def _wrap(op, tda=True):
     comm_clock, piece_clock = _CommClock(), _PieceClock()
 
     def apply_AB(z):
         t0 = time.time()
-        if isinstance(z, torch.Tensor):
-            zz = z.to(device=op.device, dtype=torch.float64)
-            out = op.apply(zz.reshape(-1, op.nocc, op.nvir)).reshape(z.shape)
-        else:
+        host = not isinstance(z, torch.Tensor)
+        if host:
             z = np.asarray(z, dtype=np.float64)
             zz = torch.as_tensor(np.ascontiguousarray(z), device=op.device)
-            out = op.apply(zz.reshape(-1, op.nocc, op.nvir)).cpu().numpy().reshape(z.shape)
+        else:
+            zz = z.to(device=op.device, dtype=torch.float64)
+        zk = zz.reshape(-1, op.nocc, op.nvir)
+        if tda:
+            outs = (op.apply(zk), None)
+        else:
+            outs = op.apply_ab(zk)
+        outs = tuple(None if o is None else
+                     (o.cpu().numpy() if host else o).reshape(z.shape) for o in outs)
         piece_clock.by_piece['gemm'] += time.time() - t0
-        return out, None
+        return outs
 
     def screened_diagonal():
         return op.screened_diagonal_device().cpu().numpy()
@@ -214,19 +241,17 @@ def isdf_block_action(lr_solver, nocc, lBSE, W_aux, isdf_factors,
 
     lr_solver: anything with `.eps` and `.spin_mode`.  isdf_factors: (X_mo, D)
     or (X_mo, D, ...).  W_aux: screened interaction in D's gauge, None for
-    TDHF (bare exchange), unused when lBSE is False (RPA).  tda must be True:
-    the B block of the full Casida problem is not built here.  comm: one
-    GPU only, so None or a one-rank communicator.  precision: 'fp64',
-    'fp32' or 'bf16x3' for the bulk GEMMs.
+    TDHF (bare exchange), unused when lBSE is False (RPA).  tda=False gives
+    the full Casida action, apply_AB(z) -> (A z, B z) (the B block,
+    block_actions.py:632-652; the split-BF16 handle computes its exchange in
+    FP32).  comm: one GPU only, so None or a one-rank communicator.
+    precision: 'fp64', 'fp32' or 'bf16x3' for the bulk GEMMs.
     """
     if getattr(lr_solver, 'spin_mode', 'restricted') == 'unrestricted':
         raise NotImplementedError('The ISDF Davidson route is restricted-spin only.')
     if comm is not None and comm.Get_size() > 1:
         raise NotImplementedError('this action runs on one GPU; a rank split '
                                   'of W is not built')
-    if not tda:
-        raise NotImplementedError('only the Tamm-Dancoff (A-half) action is '
-                                  'built on the GPU; pass tda=True')
     if spin not in KAPPA:
         raise ValueError(f"spin {spin!r}: one of {', '.join(sorted(KAPPA))}")
     if not tile_memory_gb > 0:
@@ -259,7 +284,7 @@ def isdf_block_action(lr_solver, nocc, lBSE, W_aux, isdf_factors,
     else:
         Zt = torch.zeros_like(V)
     return grid_block_action(eps, nocc, X_mo, Zt, V, kappa=KAPPA[spin],
-                             precision=precision, device=dev)
+                             precision=precision, device=dev, tda=tda)
 
 
 def block_action(lr_solver, nocc, polarizability, W_aux, isdf_factors,

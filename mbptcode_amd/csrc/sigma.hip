@@ -236,6 +236,104 @@ __global__ __launch_bounds__(256) void k_exchange(
   }
 }
 
+// ---- exchange of the B block (full Casida): U^T[b] = X_o^T (W o P[b]^T) ---------
+// With W symmetric, W o P^T = (W o P)^T, and the B block's exchange is U^T X_v
+// with U^T[i][mu] = sum_nu X_o[nu][i] W[nu][mu] P^T[nu][mu], P^T = zXv^T X_o^T
+// (block_actions.py:632-652 builds the same U as (W o P) X_o).  It is the
+// exchange above with the roles of zXv and X_o swapped in the first product:
+// a wave owns 16 columns mu, X_o[mu, :] sits in registers as the B operand of
+// P^T (16 nu x 16 mu) = zXv[b][:, nu]^T X_o[mu, :]^T, and the rows nu stream
+// -- zXv[b][:, nu] staged transposed in LDS, X_o[nu, :] by the same
+// double-buffered direct loads as above.  The P^T result element r of lane l
+// (row orow(l/16, r)) is again the K slot of the second product.  Partials
+// U^T_s are written in the layout of T_s, so the tail is shared.
+template <typename T, int NOPAD>
+__global__ __launch_bounds__(256) void k_exchange_b(
+    int M, int no, int k, const T *__restrict__ Xp, const T *__restrict__ W,
+    const T *__restrict__ zXv, T *__restrict__ Upart, int nsplit,
+    int rows_per_split, int nmb, int nblk) {
+  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, ROWS = 32;
+  constexpr int LD = NOPAD + 16 / (int)sizeof(T), NL = NOPAD * (int)sizeof(T) / 16;
+  constexpr int LDZ = NOPAD + 1;
+  __shared__ __attribute__((aligned(16))) T xs[2][ROWS * LD];
+  __shared__ T zs[ROWS * LDZ];
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= nmb * k * nsplit) return;
+  const int b = lid % k, t = lid / k, mb = t % nmb, s = t / nmb;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lq = lane >> 4;
+  const int col = mb * 64 + wave * 16 + lr;   // this lane's mu
+  const bool colok = col < M;
+  const int colc = min(col, M - 1);
+
+  T xb[NS];   // X_o[mu][4q + l/16]: B operand of the first product
+#pragma unroll
+  for (int q = 0; q < NS; ++q) xb[q] = Xp[(long)colc * NOPAD + 4 * q + lq] * (colok ? T(1) : T(0));
+  typename MF<T>::acc acc[NI];
+#pragma unroll
+  for (int it = 0; it < NI; ++it) zero(acc[it]);
+  const T *zb = zXv + (long)b * no * M;
+
+  const int st_beg = (s * rows_per_split) / ROWS;
+  const int st_end = min((M + ROWS - 1) / ROWS, ((s + 1) * rows_per_split) / ROWS);
+  auto fetch = [&](int st, int n) {
+#pragma unroll
+    for (int q = 0; q < ROWS / 4; ++q) {
+      const int r = 4 * q + wave;
+      if (lane < NL)
+        __builtin_amdgcn_global_load_lds(
+            (const void *)((const char *)(Xp + (long)(ROWS * st + r) * NOPAD) + 16 * lane),
+            (lds_ptr_t *)&xs[n][r * LD], 16, 0, 0);
+    }
+  };
+  if (st_beg < st_end) {
+    fetch(st_beg, 0);
+    int cur = 0;
+    for (int st = st_beg; st < st_end; ++st) {
+      const int nu0 = ROWS * st;
+      // zXv[b][j][nu0 + r] -> zs[r][j], coalesced along nu, zero past M and n_o
+      for (int e = tid; e < ROWS * NOPAD; e += 256) {
+        const int j = e / ROWS, r = e % ROWS, nu = nu0 + r;
+        zs[r * LDZ + j] = (j < no && nu < M) ? zb[(long)j * M + nu] : T(0);
+      }
+      __builtin_amdgcn_s_waitcnt(0);   // this wave's X_o stage landed
+      __syncthreads();                 // ... every wave's, and zs is written
+      if (st + 1 < st_end) fetch(st + 1, cur ^ 1);
+      const T *x = xs[cur];
+#pragma unroll
+      for (int ch = 0; ch < ROWS / 16; ++ch) {
+        const int r0 = ch * 16;
+        typename MF<T>::acc P;
+        zero(P);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) P = MF<T>::mma(zs[(r0 + lr) * LDZ + 4 * q + lq], xb[q], P);
+        T S[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int nu = min(nu0 + r0 + MF<T>::orow(lq, r), M - 1);   // rows past M: P = 0
+          S[r] = P[r] * W[(long)nu * M + colc];
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+          for (int it = 0; it < NI; ++it)
+            acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r], acc[it]);
+      }
+      __syncthreads();   // zs and xs[cur] are rewritten next stage
+      cur ^= 1;
+    }
+  }
+  if (colok) {
+#pragma unroll
+    for (int it = 0; it < NI; ++it)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        Upart[(((long)s * k + b) * NOPAD + 16 * it + MF<T>::orow(lq, r)) * M + col] =
+            acc[it][r];
+  }
+}
+
 // Xp (Mp x nopad) = X_o zero-padded, the exchange's staged copy
 template <typename T>
 __global__ void k_pack_xo(int M, int no, int Mp, int nopad, const T *__restrict__ Xo,
@@ -468,7 +566,7 @@ __global__ void k_tail_finish(int rows, int no, int nv, int nq, const T *__restr
     const int m = (int)(e / nv), a = (int)(e % nv), i = m % no;
     T s = Ep[e];
     for (int q = 1; q < nq; ++q) s += Ep[(long)q * n + e];
-    Az[e] = d[(long)i * nv + a] * z[e] - (double)s;
+    Az[e] = (d ? d[(long)i * nv + a] * z[e] : 0.0) - (double)s;
   }
 }
 
@@ -548,8 +646,8 @@ struct mbpt_sigma {
   int nvp32, Mp32;
   double *d;
   // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
-  void *ws[7];
-  size_t ws_cap[7];        // bytes
+  void *ws[8];
+  size_t ws_cap[8];        // bytes
   size_t held;
   int prof;                // stage timing on
   hipEvent_t ev[5];
@@ -564,7 +662,7 @@ constexpr int kTargetBlocks = 2048;
 
 int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
 
-enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_ZT, WS_N };
+enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_ZT, WS_UPART, WS_N };
 
 void free_ws(mbpt_sigma *h) {
   for (int q = 0; q < WS_N; ++q) {
@@ -599,14 +697,15 @@ int ensure_buf(mbpt_sigma *h, int q, size_t bytes) {
   return MBPT_OK;
 }
 
-int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu, int ldr) {
+int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu, int ldr, bool want_b) {
   const size_t e = h->esz, M = h->M, no = h->no, nv = h->nv;
   const bool km = h->esz == 4;   // K-major head/tail operands (FP32 working precision)
   const size_t sizes[WS_N] = {e * k * no * M, e * k * M, e * (size_t)nqu * k * M,
                               e * (size_t)nsplit * k * h->nopad * M,
                               km ? 0 : e * k * no * M,
                               e * (size_t)nq * k * no * nv,
-                              km ? 4 * (size_t)std::max(h->nvp32, h->Mp32) * ldr : 0};
+                              km ? 4 * (size_t)std::max(h->nvp32, h->Mp32) * ldr : 0,
+                              want_b ? e * (size_t)nsplit * k * h->nopad * M : 0};
   for (int q = 0; q < WS_N; ++q) {
     int rc = ensure_buf(h, q, sizes[q]);
     if (rc) return rc;
@@ -637,6 +736,16 @@ void launch_exchange(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_
   k_exchange<T, NOPAD><<<nblk, 256, 0, st>>>(h->M, h->no, k, (const T *)h->Xp,
                                              (const T *)h->W, (const T *)h->ws[WS_ZXV],
                                              (T *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
+}
+
+template <typename T, int NOPAD>
+void launch_exchange_b(const mbpt_sigma *h, int k, int nsplit, int rps, T *Upart,
+                       hipStream_t st) {
+  const int nmb = (h->M + 63) / 64;
+  const int nblk = pad_grid((long)nmb * k * nsplit);
+  k_exchange_b<T, NOPAD><<<nblk, 256, 0, st>>>(h->M, h->no, k, (const T *)h->Xp,
+                                               (const T *)h->W, (const T *)h->ws[WS_ZXV],
+                                               Upart, nsplit, rps, nmb, nblk);
 }
 
 template <int NP, int NIE, int TAIL, int NV>
@@ -677,7 +786,7 @@ void launch_hartree_u(int M, int k, int chunk, int nqu, const T *p, const T *V, 
 }
 
 template <typename T>
-int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st) {
+int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hipStream_t st) {
   const int M = h->M, no = h->no, nv = h->nv;
   const int nsplit = nsplit_for(h, k);
   const int rows = k * no;
@@ -706,7 +815,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
   int nqu = std::max(1, std::min((512 + (M + 255) / 256 - 1) / ((M + 255) / 256), (M + 63) / 64));
   const int hchunk = (M + nqu - 1) / nqu;
   nqu = (M + hchunk - 1) / hchunk;
-  int rc = ensure_ws(h, k, nsplit, nq, nqu, ldr);
+  int rc = ensure_ws(h, k, nsplit, nq, nqu, ldr, Bz != nullptr);
   if (rc) return rc;
   T *zXv = (T *)h->ws[WS_ZXV], *p = (T *)h->ws[WS_P], *u = (T *)h->ws[WS_U];
   T *Tpart = (T *)h->ws[WS_TPART], *Tc = (T *)h->ws[WS_TC], *Ep = (T *)h->ws[WS_EP];
@@ -782,47 +891,68 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     default: set_error("internal: nopad"); return MBPT_ERR_VALUE;
   }
   if ((rc = launch_status("k_exchange"))) return rc;
-  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
-  if (b3) {  // tail, split BF16: T' as hi / lo rows, times X_v, split over K
-    __bf16 *Th = (__bf16 *)h->ws[WS_ZT], *Tl = Th + (size_t)ldr * h->Mp32;
-    k_tail_prep_b3<<<grid1((long)ldr * h->Mp32), 256, 0, st>>>(
-        k, no, h->nopad, M, h->Mp32, ldr, nsplit_eff, (const float *)Tpart,
-        (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr,
-        (float)h->kappa, Th, Tl);
-    if ((rc = launch_status("k_tail_prep_b3"))) return rc;
-    const int ntm = ldr / 128, ntn = h->nvq / 128;
-    const int nblk = pad_grid((long)ntm * ntn * nq);
-    k_gemm_nt3<EpiStore><<<nblk, 256, 0, st>>>(h->Mp32, kchunk, Th, Tl, h->Mp32, h->XvTB[0],
-                                                h->XvTB[1], h->Mp32, ntm, ntn, nq,
-                                                EpiStore{(float *)Ep, rows, nv, nv}, nblk);
-    if ((rc = launch_status("k_gemm_nt3(tail)"))) return rc;
-  } else if (km) {  // tail, K-major: T'^T padded, then (T'^T)^T X_v split over K
-    float *Tct = (float *)h->ws[WS_ZT];
-    k_tail_prep_t<<<dim3(ldr / 64, (h->Mp + 63) / 64), 256, 0, st>>>(
-        k, no, h->nopad, M, h->Mp, ldr, nsplit_eff, (const float *)Tpart,
-        (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr, 1,
-        (float)h->kappa, Tct);
-    if ((rc = launch_status("k_tail_prep_t"))) return rc;
-    const int ntm = ldr / 128, ntn = (nv + 127) / 128;
-    const int nblk = pad_grid((long)ntm * ntn * nq);
-    k_gemm_km<EpiStore><<<nblk, 512, 0, st>>>(h->Mp, kchunk, Tct, ldr, h->XvP, h->nvq, ntm,
-                                               ntn, nq, EpiStore{(float *)Ep, rows, nv, nv},
-                                               nblk);
-    if ((rc = launch_status("k_gemm_km(tail)"))) return rc;
-  } else {  // tail
-    k_tail_prep<T><<<grid1((long)rows * M), 256, 0, st>>>(
-        k, no, h->nopad, M, nsplit_eff, Tpart, (const T *)h->XoT,
-        h->kappa != 0.0 ? u : nullptr, 1, (T)h->kappa, Tc);
-    if ((rc = launch_status("k_tail_prep"))) return rc;
-    const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
-    const int nblk = pad_grid((long)ntm * ntn * nq);
-    k_gemm_nn_splitk<T><<<nblk, 256, 0, st>>>(rows, nv, M, kchunk, Tc, M,
-                                              (const T *)h->Xv, nv, Ep, ntm, ntn, nq, nblk);
-    if ((rc = launch_status("k_gemm_nn_splitk(tail)"))) return rc;
+  T *Upart = (T *)h->ws[WS_UPART];
+  if (Bz) {  // the B block's exchange partials (in working precision; FP32 for split BF16)
+    switch (h->nopad) {
+      case 16: launch_exchange_b<T, 16>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 32: launch_exchange_b<T, 32>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 48: launch_exchange_b<T, 48>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 64: launch_exchange_b<T, 64>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 80: launch_exchange_b<T, 80>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 96: launch_exchange_b<T, 96>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 112: launch_exchange_b<T, 112>(h, k, nsplit_eff, rps, Upart, st); break;
+      case 128: launch_exchange_b<T, 128>(h, k, nsplit_eff, rps, Upart, st); break;
+      default: set_error("internal: nopad"); return MBPT_ERR_VALUE;
+    }
+    if ((rc = launch_status("k_exchange_b"))) return rc;
   }
-  k_tail_finish<T><<<grid1((long)rows * nv), 256, 0, st>>>(rows, no, nv, nq, Ep, h->d, z,
-                                                            Az);
-  if ((rc = launch_status("k_tail_finish"))) return rc;
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
+  // tail: out = dd o z - (sum_s partials_s - kappa X_o^T o u) X_v (dd NULL: no
+  // diagonal term, the B block)
+  auto tail = [&](const T *src, const double *dd, double *out) -> int {
+    if (b3) {  // tail, split BF16: T' as hi / lo rows, times X_v, split over K
+      __bf16 *Th = (__bf16 *)h->ws[WS_ZT], *Tl = Th + (size_t)ldr * h->Mp32;
+      k_tail_prep_b3<<<grid1((long)ldr * h->Mp32), 256, 0, st>>>(
+          k, no, h->nopad, M, h->Mp32, ldr, nsplit_eff, (const float *)src,
+          (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr,
+          (float)h->kappa, Th, Tl);
+      if ((rc = launch_status("k_tail_prep_b3"))) return rc;
+      const int ntm = ldr / 128, ntn = h->nvq / 128;
+      const int nblk = pad_grid((long)ntm * ntn * nq);
+      k_gemm_nt3<EpiStore><<<nblk, 256, 0, st>>>(h->Mp32, kchunk, Th, Tl, h->Mp32, h->XvTB[0],
+                                                  h->XvTB[1], h->Mp32, ntm, ntn, nq,
+                                                  EpiStore{(float *)Ep, rows, nv, nv}, nblk);
+      if ((rc = launch_status("k_gemm_nt3(tail)"))) return rc;
+    } else if (km) {  // tail, K-major: T'^T padded, then (T'^T)^T X_v split over K
+      float *Tct = (float *)h->ws[WS_ZT];
+      k_tail_prep_t<<<dim3(ldr / 64, (h->Mp + 63) / 64), 256, 0, st>>>(
+          k, no, h->nopad, M, h->Mp, ldr, nsplit_eff, (const float *)src,
+          (const float *)h->XoT, h->kappa != 0.0 ? (const float *)u : nullptr, 1,
+          (float)h->kappa, Tct);
+      if ((rc = launch_status("k_tail_prep_t"))) return rc;
+      const int ntm = ldr / 128, ntn = (nv + 127) / 128;
+      const int nblk = pad_grid((long)ntm * ntn * nq);
+      k_gemm_km<EpiStore><<<nblk, 512, 0, st>>>(h->Mp, kchunk, Tct, ldr, h->XvP, h->nvq, ntm,
+                                                 ntn, nq, EpiStore{(float *)Ep, rows, nv, nv},
+                                                 nblk);
+      if ((rc = launch_status("k_gemm_km(tail)"))) return rc;
+    } else {  // tail
+      k_tail_prep<T><<<grid1((long)rows * M), 256, 0, st>>>(
+          k, no, h->nopad, M, nsplit_eff, src, (const T *)h->XoT,
+          h->kappa != 0.0 ? u : nullptr, 1, (T)h->kappa, Tc);
+      if ((rc = launch_status("k_tail_prep"))) return rc;
+      const int ntm = (rows + 63) / 64, ntn = (nv + 63) / 64;
+      const int nblk = pad_grid((long)ntm * ntn * nq);
+      k_gemm_nn_splitk<T><<<nblk, 256, 0, st>>>(rows, nv, M, kchunk, Tc, M,
+                                                (const T *)h->Xv, nv, Ep, ntm, ntn, nq, nblk);
+      if ((rc = launch_status("k_gemm_nn_splitk(tail)"))) return rc;
+    }
+    k_tail_finish<T><<<grid1((long)rows * nv), 256, 0, st>>>(rows, no, nv, nq, Ep, dd, z,
+                                                              out);
+    if ((rc = launch_status("k_tail_finish"))) return rc;
+    return MBPT_OK;
+  };
+  if ((rc = tail(Tpart, h->d, Az))) return rc;
   if (h->prof) {
     MBPT_HIP_TRY(hipEventRecord(h->ev[4], st));
     MBPT_HIP_TRY(hipEventSynchronize(h->ev[4]));
@@ -834,6 +964,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, hipStream_t st
     h->prof_flops += (double)k * (4.0 * no * (double)M * M + (double)M * M);
     h->prof_launches += 1;
   }
+  if (Bz && (rc = tail(Upart, nullptr, Bz))) return rc;
   return MBPT_OK;
 }
 
@@ -960,7 +1091,7 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
                                                             h->Mp, h->nvq, h->XvP);
     if ((rc = launch_status("X_v padded copies"))) return fail(rc);
   }
-  if (precision != MBPT_BF16X3) {  // the exchange's zero-padded X_o
+  {  // the exchange's zero-padded X_o (the B block uses it in every precision)
     const int Mp32 = (M + 31) / 32 * 32;
     const size_t bytes = e * (size_t)Mp32 * h->nopad;
     he = hipMalloc(&h->Xp, bytes);
@@ -1000,8 +1131,21 @@ int mbpt_sigma_apply(mbpt_sigma *h, int k, const double *z_dev, double *Az_dev,
   MBPT_REQUIRE(z_dev && Az_dev, "mbpt_sigma_apply: NULL z or Az");
   MBPT_REQUIRE(z_dev != Az_dev, "mbpt_sigma_apply: z and Az must be distinct buffers");
   hipStream_t st = (hipStream_t)stream;
-  return h->prec == MBPT_FP64 ? apply_impl<double>(h, k, z_dev, Az_dev, st)
-                              : apply_impl<float>(h, k, z_dev, Az_dev, st);
+  return h->prec == MBPT_FP64 ? apply_impl<double>(h, k, z_dev, Az_dev, nullptr, st)
+                              : apply_impl<float>(h, k, z_dev, Az_dev, nullptr, st);
+}
+
+int mbpt_sigma_apply_ab(mbpt_sigma *h, int k, const double *z_dev, double *Az_dev,
+                        double *Bz_dev, void *stream) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_sigma_apply_ab: NULL handle");
+  MBPT_REQUIRE(k >= 0, "mbpt_sigma_apply_ab: k < 0");
+  if (k == 0) return MBPT_OK;
+  MBPT_REQUIRE(z_dev && Az_dev && Bz_dev, "mbpt_sigma_apply_ab: NULL z, Az or Bz");
+  MBPT_REQUIRE(z_dev != Az_dev && z_dev != Bz_dev && Az_dev != Bz_dev,
+               "mbpt_sigma_apply_ab: z, Az and Bz must be distinct buffers");
+  hipStream_t st = (hipStream_t)stream;
+  return h->prec == MBPT_FP64 ? apply_impl<double>(h, k, z_dev, Az_dev, Bz_dev, st)
+                              : apply_impl<float>(h, k, z_dev, Az_dev, Bz_dev, st);
 }
 
 int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream) {

@@ -16,13 +16,15 @@ Two kinds of functions live here:
    ``davidson_tda`` (the Jacobi-Davidson TDA iteration).
 2. Closed forms that pin them independently of the restatement:
    ``dense_tda_matrix`` (A from the ISDF factors, O(n_pair^2 M)),
+   ``dense_casida_b_matrix`` (B from the same factors),
    ``proj_tau_bruteforce`` (the pair sum of chi0(i tau)),
    ``chi0_omega_closed_form`` (the exact frequency-axis chi0),
    ``rpa_energy_casida`` (Furche's trace formula).
 
 Pinning status (what checks each function; see tests/test_oracle.py):
 
-* isdf_block_action       -- dense A (closed form), tiling invariance
+* isdf_block_action       -- dense A (closed form), tiling invariance; its
+  B block (tda=False) -- dense B (closed form), B symmetric
 * screened_diagonal       -- diagonal of the dense direct term
 * polarizability_projected_tau -- brute-force pair sum, tiling invariance
 * chi0_imaginary_frequency -- closed form chi0(i w) at the fit's error
@@ -69,8 +71,10 @@ def screened_rows(D, W_aux):
     return D @ (W_aux @ D.T)
 
 
-def isdf_block_action(eps, nocc, X, D, W_aux, kappa=2.0, tile_memory_gb=4.0):
-    """(apply_A, diag_d) of the Tamm-Dancoff ISDF action.
+def isdf_block_action(eps, nocc, X, D, W_aux, kappa=2.0, tile_memory_gb=4.0, tda=True):
+    """(apply_A, diag_d) of the Tamm-Dancoff ISDF action; with tda=False,
+    (apply_AB, diag_d) where apply_AB(z) -> (A z, B z) (the full Casida
+    action, block_actions.py:563-661 with the B block of :632-652).
 
     apply_A(z (k, nocc, nvir)) -> A z, following block_actions.py:563-661 with
     one grid-row block (serial) and tda=True:
@@ -92,16 +96,21 @@ def isdf_block_action(eps, nocc, X, D, W_aux, kappa=2.0, tile_memory_gb=4.0):
     Zt = screened_rows(D, W_aux)
     rows = max(1, min(npts, int(tile_memory_gb * BYTES_PER_GB / max(npts * 8, 1))))
 
-    def apply_A(z):
+    def apply_AB(z):
         z = np.asarray(z, float)
         Az = d[None] * z
+        Bz = None if tda else np.zeros(z.shape)
         for n, zn in enumerate(z):
             zXv = zn @ X_v.T                                    # (nocc, M)
             if kappa:
                 p = np.einsum('kj,jk->k', X_o, zXv)
                 u = D @ (p @ D)
-                Az[n] += kappa * (X_o.T @ (u[:, None] * X_v))
+                hartree = kappa * (X_o.T @ (u[:, None] * X_v))
+                Az[n] += hartree
+                if not tda:
+                    Bz[n] = hartree
             T = np.empty((nocc, npts))
+            U = None if tda else np.empty((npts, nocc))
             for p0 in range(0, npts, rows):
                 p1 = min(p0 + rows, npts)
                 blk = X_o[p0:p1] @ zXv                          # P's rows
@@ -111,8 +120,17 @@ def isdf_block_action(eps, nocc, X, D, W_aux, kappa=2.0, tile_memory_gb=4.0):
                     T[...] = Tb
                 else:
                     T += Tb
+                if not tda:
+                    # Zt symmetric: Zt o P^T = (Zt o P)^T, and its product
+                    # with X_o over the column index is these rows of U (:637-652)
+                    U[p0:p1] = blk @ X_o
             Az[n] -= T @ X_v
-        return Az
+            if not tda:
+                Bz[n] -= U.T @ X_v
+        return Az, Bz
+
+    def apply_A(z):
+        return apply_AB(z)[0]
 
     def screened_diagonal():
         """d - (ii|W|aa) with (ii|W|aa) = [D^T (X_o o X_o)]^T W_aux
@@ -122,6 +140,9 @@ def isdf_block_action(eps, nocc, X, D, W_aux, kappa=2.0, tile_memory_gb=4.0):
         return d - (B_o.T @ W_aux) @ B_v
 
     apply_A.screened_diagonal = screened_diagonal
+    if not tda:
+        apply_AB.screened_diagonal = screened_diagonal
+        return apply_AB, d
     return apply_A, d
 
 
@@ -150,6 +171,26 @@ def dense_tda_matrix(eps, nocc, X, D, W_aux, kappa=2.0):
     A = kappa * hartree - direct
     A[np.diag_indices(npair)] += pair_diagonal(eps, nocc).ravel()
     return A
+
+
+def dense_casida_b_matrix(nocc, X, D, W_aux, kappa=2.0):
+    """B[ia, jb] = kappa (ia|jb) - (ib|W|ja) built whole from the factors
+    (linear_response.py:291-409, the separable-RI form):
+
+        (ib|W|ja) = sum_{mu nu} C[mu, ib] Zt[mu, nu] C[nu, ja],  C = X_o o X_v
+
+    The independent check of the B block of the action."""
+    X = np.asarray(X, float)
+    X_o, X_v = X[:, :nocc], X[:, nocc:]
+    no, nv = X_o.shape[1], X_v.shape[1]
+    npair = no * nv
+    C = (X_o[:, :, None] * X_v[:, None, :]).reshape(-1, npair)
+    CD = D.T @ C
+    hartree = CD.T @ CD
+    Zt = D @ W_aux @ D.T
+    swap = (C.T @ Zt @ C).reshape(no, nv, no, nv)          # [i, b, j, a]
+    swap = swap.transpose(0, 3, 2, 1).reshape(npair, npair)  # [(i, a), (j, b)]
+    return kappa * hartree - swap
 
 
 # ---------------------------------------------------------------------------

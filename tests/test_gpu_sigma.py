@@ -153,8 +153,6 @@ def test_empty_batch_and_shape_errors():
     Wa[0, 1] = 1.0
     with pytest.raises(ValueError):
         isdf_block_action(LR, 3, True, Wa, (inst.X, D), tda=True)
-    with pytest.raises(NotImplementedError):
-        isdf_block_action(LR, 3, True, np.eye(20), (inst.X, D), tda=False)
     with pytest.raises(ValueError):
         isdf_block_action(LR, 3, True, np.eye(19), (inst.X, D), tda=True)
 

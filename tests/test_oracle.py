@@ -70,6 +70,43 @@ def test_dense_matrix_has_the_known_pieces():
     assert np.isclose(A2[i * nv + a, j * nv + b], 2 * hart - direct, rtol=1e-12)
 
 
+@pytest.mark.parametrize('spin_kappa', [2.0, 0.0])
+def test_full_action_b_block_equals_dense_b(spin_kappa):
+    eps, X, D, W, z = small_system()
+    apply_AB, d = O.isdf_block_action(eps, 4, X, D, W, kappa=spin_kappa, tda=False)
+    Az, Bz = apply_AB(z)
+    B = O.dense_casida_b_matrix(4, X, D, W, kappa=spin_kappa)
+    ref = (B @ z.reshape(3, -1).T).T.reshape(z.shape)
+    assert np.abs(Bz - ref).max() <= 1e-13 * np.abs(ref).max()
+    apply_A, _ = O.isdf_block_action(eps, 4, X, D, W, kappa=spin_kappa)
+    assert np.array_equal(Az, apply_A(z))
+    tiled = O.isdf_block_action(eps, 4, X, D, W, kappa=spin_kappa, tda=False,
+                                tile_memory_gb=6 * 37 * 8 / 1e9)[0](z)[1]
+    assert np.abs(tiled - Bz).max() <= 1e-13 * np.abs(Bz).max()
+
+
+def test_dense_b_has_the_swapped_exchange():
+    """B = kappa (ia|jb) - (ib|W|ja): symmetric, the Hartree part shared with
+    A, and one element by brute force against the swapped pair (ib)(ja) --
+    the direct pair (ij)(ab) of A would fail it."""
+    eps, X, D, W, z = small_system()
+    no, nv = 4, 5
+    B2 = O.dense_casida_b_matrix(no, X, D, W, kappa=2.0)
+    B0 = O.dense_casida_b_matrix(no, X, D, W, kappa=0.0)
+    A2 = O.dense_tda_matrix(eps, no, X, D, W, kappa=2.0)
+    A0 = O.dense_tda_matrix(eps, no, X, D, W, kappa=0.0)
+    assert np.allclose(B2, B2.T, atol=1e-13)
+    assert np.allclose(B2 - B0, A2 - A0, atol=1e-13)
+    i, a, j, b = 1, 2, 3, 0
+    C = lambda p, q: X[:, p] * X[:, no + q]
+    Zt = D @ W @ D.T
+    hart = C(i, a) @ (D @ D.T) @ C(j, b)
+    swap = C(i, b) @ Zt @ C(j, a)
+    direct = (X[:, i] * X[:, j]) @ Zt @ (X[:, no + a] * X[:, no + b])
+    assert np.isclose(B2[i * nv + a, j * nv + b], 2 * hart - swap, rtol=1e-12)
+    assert not np.isclose(B2[i * nv + a, j * nv + b], 2 * hart - direct, rtol=1e-6)
+
+
 def test_block_action_is_tiling_invariant():
     """tile_memory_gb sets the grid-row tile; the result must not move
     (tests/test_davidson_isdf_bse.py:165-177)."""
