@@ -10,8 +10,9 @@ Drop-ins for the reference's Hermitian Davidson stack:
   davidson_shared.py:19-54), tracked roots above the requested ones
   (`tracked_roots` :146-154), 'bare' / 'screened' preconditioner diagonal
   (:125-143), unconfirmed-root bound (:538-546);
-* `solve_casida_davidson` (LR/davidson.py:252) with `tda=True` and
-  `check_residual_floor` (:428-450).
+* `solve_casida_davidson` (LR/davidson.py:252) and `check_residual_floor`
+  (:428-450); with `tda=False` the full Casida roots come from the paired
+  loop in `casida.py` (`_davidson_root` :998, `casida_rows`).
 
 Everything the iteration does is FP64 and stays on the device: the trial
 space V and its image AV, the Rayleigh-Ritz matrix, residuals, corrections
@@ -427,7 +428,15 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     check).  Returns NumPy omega (nroots,), X (n_pair, nroots), Y = 0.
     """
     if not tda:
-        raise NotImplementedError('the full (non-TDA) Casida Davidson is not built')
+        if x_sym is not None:
+            raise NotImplementedError('pair-symmetry sectors of the Casida Davidson '
+                                      'are not built')
+        return _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
+                            guess_factor=guess_factor, stats=stats, comm=comm,
+                            timings=timings, refuse_unconverged=refuse_unconverged,
+                            preconditioner=preconditioner, guess=guess,
+                            tracked=tracked, extra_roots=extra_roots,
+                            max_space=max_space, refine_apply=refine_apply)
     if x_sym is not None:
         raise ValueError('the Tamm-Dancoff Davidson takes no pair symmetry')
     if comm is not None and comm.Get_size() > 1:
@@ -553,6 +562,119 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     return np.asarray(e[:nroots]), Xn, np.zeros_like(Xn)
 
 
+def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
+                 guess_factor=DAVIDSON_GUESS_FACTOR, stats=None, comm=None, timings=None,
+                 refuse_unconverged=False, preconditioner=DAVIDSON_PRECONDITIONER,
+                 guess=None, tracked=None, extra_roots=None, max_space=None,
+                 refine_apply=None):
+    """(omega, X, Y): the full Casida roots of a GPU block action, the
+    reference's `_davidson_root` (LR/davidson.py:998) with its paired loop
+    (`casida.casida_davidson`): the same guess pairs and tracked roots as the
+    Tamm-Dancoff solve, the unit guesses with Y = 0 and a warm `guess`
+    (X, Y) ahead of them.  refine_apply: an FP64 action of the same operator; the
+    requested roots' residuals are recomputed on it, and the solve continues
+    on it from those roots where one is above the tolerance."""
+    from .casida import casida_davidson
+    if comm is not None and comm.Get_size() > 1:
+        raise NotImplementedError('one GPU: no pair-row split')
+    check_preconditioner(preconditioner)
+    op = _operator_of(apply_AB)
+    no, nv = np.shape(diag_d)
+    npair = no * nv
+    dev = op.device
+    t_all = time.time()
+    counters = {}
+    if preconditioner == 'bare':
+        pdiag = torch.as_tensor(np.asarray(diag_d, float).ravel(), device=dev)
+    elif preconditioner == 'learned':
+        raise NotImplementedError('the learned correction is built for the '
+                                  'Tamm-Dancoff Davidson only')
+    else:
+        pdiag = op.screened_diagonal_device().reshape(-1)
+    order = guess_indices(diag_d, nroots, guess_factor, counters=counters)
+    track = tracked_roots(nroots, npair, extra_roots)
+    x0 = torch.zeros((npair, len(order)), dtype=torch.float64, device=dev)
+    x0[torch.as_tensor(order, device=dev), torch.arange(len(order), device=dev)] = 1.0
+    y0 = torch.zeros_like(x0)
+    if guess is not None:
+        gx = torch.as_tensor(np.asarray(guess[0], float).reshape(npair, -1), device=dev)
+        gy = (torch.zeros_like(gx) if guess[1] is None else
+              torch.as_tensor(np.asarray(guess[1], float).reshape(npair, -1), device=dev))
+        x0, y0 = torch.cat([gx, x0], dim=1), torch.cat([gy, y0], dim=1)
+
+    def ab_of(o):
+        def ab(Z):
+            k = Z.shape[0]
+            Az, Bz = o.apply_ab(Z.reshape(k, no, nv))
+            return Az.reshape(k, npair), Bz.reshape(k, npair)
+        return ab
+
+    res = casida_davidson(ab_of(op), pdiag, track, x0, tol=conv_tol, max_cycle=max_cycle,
+                          max_space=max_space, y0=y0)
+    cycles, vectors = res.cycles, res.vectors
+    e, X, Y, conv, rn = np.asarray(res.e, float), res.X, res.Y, res.conv, res.rnorm
+    refine = {}
+    if refine_apply is not None:
+        op64 = _operator_of(refine_apply)
+        ab64 = ab_of(op64)
+        Xr, Yr = X[:, :nroots].contiguous(), Y[:, :nroots].contiguous()
+        AX, BX = ab64(Xr.T.contiguous())
+        AY, BY = ab64(Yr.T.contiguous())
+        w = torch.as_tensor(e[:nroots], device=dev)
+        Rx = AX.T + BY.T - Xr * w
+        Ry = BX.T + AY.T + Yr * w
+        rn64 = torch.sqrt((Rx * Rx).sum(0) + (Ry * Ry).sum(0)).cpu().numpy()
+        refine['rnorm_before'] = rn64.copy()
+        if not (rn64 <= conv_tol).all():
+            res2 = casida_davidson(ab64, pdiag, track, X.contiguous(), tol=conv_tol,
+                                   max_cycle=max_cycle, max_space=max_space,
+                                   y0=Y.contiguous())
+            e, X, Y, conv, rn = np.asarray(res2.e, float), res2.X, res2.Y, res2.conv, res2.rnorm
+            refine['cycles'] = res2.cycles
+            refine['vectors'] = res2.vectors
+            cycles += res2.cycles
+            vectors += res2.vectors
+            rn64 = rn[:nroots]
+        else:
+            rn = np.array(rn, dtype=float)
+            rn[:nroots] = rn64
+            conv = rn <= conv_tol
+        refine['rnorm_after'] = np.asarray(rn64).copy()
+    conv = np.asarray(conv, bool)
+    if len(e) < nroots:
+        raise RuntimeError(f'Casida Davidson found {len(e)} of {nroots} roots')
+    unconfirmed = np.zeros(nroots, bool)
+    extra = np.flatnonzero(~conv[nroots:]) + nroots
+    if extra.size:
+        lower = np.min(e[extra] - rn[extra])
+        unconfirmed = e[:nroots] > lower
+    if stats is not None:
+        stats.update({'davidson_vind_calls': cycles, 'davidson_block_actions': vectors,
+                      'davidson_dense_completion': False})
+    if timings is not None:
+        timings.update({'davidson_s': time.time() - t_all, 'davidson_iterations': cycles,
+                        'davidson_vectors': vectors,
+                        'davidson_converged': conv[:nroots].tolist(),
+                        'davidson_residuals': np.asarray(rn[:nroots]).tolist(),
+                        'davidson_unconfirmed': unconfirmed.tolist(),
+                        'davidson_trace': res.trace,
+                        'davidson_refine': {k: (v.tolist() if hasattr(v, 'tolist') else v)
+                                            for k, v in refine.items()},
+                        'davidson_flags': counters})
+    Xn = X.cpu().numpy()
+    Yn = Y.cpu().numpy()
+    if tracked is not None:
+        tracked.update(omega=e.copy(), X=Xn.copy(), Y=Yn.copy())
+    bad = ~conv[:nroots]
+    if bad.any():
+        msg = (f'Casida Davidson left {int(bad.sum())} of {nroots} roots above '
+               f'|r| <= {conv_tol:g} after {cycles} cycles')
+        if refuse_unconverged:
+            raise DavidsonNotConverged(msg)
+        warnings.warn(msg, RuntimeWarning, stacklevel=3)
+    return e[:nroots], Xn[:, :nroots], Yn[:, :nroots]
+
+
 def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
                           W_aux=None, orbsym=None, isdf_factors=None, stats=None,
                           spin='singlet', comm=None, timings=None,
@@ -562,7 +684,9 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
                           conv_tol=1e-8, max_cycle=150, guess_factor=2,
                           preconditioner=DAVIDSON_PRECONDITIONER,
                           precision='fp32', refine=True, learned=None):
-    """The matrix-free Davidson entry (LR/davidson.py:252) on the GPU, TDA only.
+    """The matrix-free Davidson entry (LR/davidson.py:252) on the GPU: the
+    Tamm-Dancoff solve (tda=True) or the full Casida one (tda=False, the
+    paired loop of `casida.casida_davidson`).
 
     `numerics` may carry a `bse` section with nroots, conv_tol, max_cycle,
     guess_factor and preconditioner (numerics.py:241-286); otherwise the
@@ -579,6 +703,9 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
     if orbsym is not None and tda:
         raise ValueError('the Tamm-Dancoff Davidson takes no pair symmetry: its '
                          'trial space is one Hermitian block')
+    if orbsym is not None:
+        raise NotImplementedError('pair-symmetry sectors of the Casida Davidson '
+                                  'are not built')
     if one_body is not None:
         raise NotImplementedError('one_body coupling is not built')
     from .orbital_spaces import bse_pair_diagonal

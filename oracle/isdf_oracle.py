@@ -17,6 +17,7 @@ Two kinds of functions live here:
 2. Closed forms that pin them independently of the restatement:
    ``dense_tda_matrix`` (A from the ISDF factors, O(n_pair^2 M)),
    ``dense_casida_b_matrix`` (B from the same factors),
+   ``dense_casida_roots`` (the full Casida roots of dense A, B),
    ``proj_tau_bruteforce`` (the pair sum of chi0(i tau)),
    ``chi0_omega_closed_form`` (the exact frequency-axis chi0),
    ``rpa_energy_casida`` (Furche's trace formula).
@@ -25,6 +26,8 @@ Pinning status (what checks each function; see tests/test_oracle.py):
 
 * isdf_block_action       -- dense A (closed form), tiling invariance; its
   B block (tda=False) -- dense B (closed form), B symmetric
+* dense_casida_roots      -- the Casida equations' residuals, B = 0 giving
+  eig(A), the RPA trace formula
 * screened_diagonal       -- diagonal of the dense direct term
 * polarizability_projected_tau -- brute-force pair sum, tiling invariance
 * chi0_imaginary_frequency -- closed form chi0(i w) at the fit's error
@@ -191,6 +194,31 @@ def dense_casida_b_matrix(nocc, X, D, W_aux, kappa=2.0):
     swap = (C.T @ Zt @ C).reshape(no, nv, no, nv)          # [i, b, j, a]
     swap = swap.transpose(0, 3, 2, 1).reshape(npair, npair)  # [(i, a), (j, b)]
     return kappa * hartree - swap
+
+
+def dense_casida_roots(A, B):
+    """(omega, X, Y): every root of the Casida problem
+    [[A, B], [B, A]] [X; Y] = omega [[1, 0], [0, -1]] [X; Y], omega > 0,
+    ascending, from the Hermitian form of A - B > 0:
+    (A-B)^(1/2) (A+B) (A-B)^(1/2) T = omega^2 T,
+    X + Y = (A-B)^(1/2) T / sqrt(omega), X - Y = omega (A-B)^(-1) (X + Y),
+    normalised so that X^T X - Y^T Y = 1.  The closed form the Casida
+    Davidson is judged against (casida.py:156 solves the same problem
+    densely)."""
+    A, B = np.asarray(A, float), np.asarray(B, float)
+    e, U = np.linalg.eigh(A - B)
+    if e.min() <= 0:
+        raise np.linalg.LinAlgError('A - B is not positive definite')
+    h = (U * np.sqrt(e)) @ U.T
+    w2, T = np.linalg.eigh(h @ (A + B) @ h)
+    if w2.min() <= 0:
+        raise np.linalg.LinAlgError('A + B is not positive definite')
+    w = np.sqrt(w2)
+    xpy = h @ T / np.sqrt(w)
+    xmy = w * np.linalg.solve(A - B, xpy)
+    X, Y = 0.5 * (xpy + xmy), 0.5 * (xpy - xmy)
+    norm = np.sqrt(np.sum(X * X, axis=0) - np.sum(Y * Y, axis=0))
+    return w, X / norm, Y / norm
 
 
 # ---------------------------------------------------------------------------

@@ -1,0 +1,137 @@
+"""The paired Casida Davidson (mbptcode_amd.casida) on the CPU against the
+closed-form dense Casida roots of the oracle (LR/trial_space.py:454-786 is the
+reference loop; the roots, not the iteration, are judged)."""
+import numpy as np
+import pytest
+import torch
+
+from mbptcode_amd import grids, synth
+from mbptcode_amd.casida import casida_davidson, subspace_solve, symmetric_coefficients
+from oracle import isdf_oracle as O
+
+
+@pytest.fixture(scope='module')
+def cfg0_ab():
+    inst = synth.config(0)
+    D = O.grid_factors(inst.V)
+    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    W_aux = O.static_screening_matrix(inst.X, D, inst.eps, inst.nocc,
+                                      g.tau_points, g.cosft_wt[0])
+    A = O.dense_tda_matrix(inst.eps, inst.nocc, inst.X, D, W_aux)
+    B = O.dense_casida_b_matrix(inst.nocc, inst.X, D, W_aux)
+    return inst, A, B
+
+
+def test_subspace_solver_on_the_whole_space(cfg0_ab):
+    """With the whole pair space as trial pairs (V = I, W = 0) the projected
+    problem is the full one: its roots are the dense Casida roots."""
+    inst, A, B = cfg0_ab
+    n = A.shape[0]
+    w, x, y = subspace_solve(A, B, np.eye(n), np.zeros((n, n)), 8)
+    ref, Xr, Yr = O.dense_casida_roots(A, B)
+    assert np.allclose(w, ref[:8], rtol=1e-12)
+    assert np.allclose(np.sum(x * x, 0) - np.sum(y * y, 0), 1.0, atol=1e-12)
+    # X and Y up to a sign per root
+    sgn = np.sign(np.sum(x * Xr[:, :8], 0))
+    assert np.abs(x * sgn - Xr[:, :8]).max() < 1e-9
+    assert np.abs(y * sgn - Yr[:, :8]).max() < 1e-9
+
+
+def test_symmetric_coefficients_orthonormalise_pairs_and_partners():
+    rng = np.random.default_rng(2)
+    n, k = 50, 6
+    X, Y = rng.normal(size=(n, k)), 0.3 * rng.normal(size=(n, k))
+    X[:, 5] = X[:, 0] + 1e-9 * rng.normal(size=n)    # a nearly dependent one
+    Y[:, 5] = Y[:, 0]
+    nrm = np.sqrt((X * X).sum(0) + (Y * Y).sum(0))   # unit pairs, as the solver hands them
+    X, Y = X / nrm, Y / nrm
+    ca, cb = symmetric_coefficients(X.T @ X + Y.T @ Y, X.T @ Y + Y.T @ X, 1e-14)
+    xo, yo = X @ ca + Y @ cb, Y @ ca + X @ cb
+    Z = np.vstack([np.hstack([xo, yo]), np.hstack([yo, xo])])
+    assert ca.shape[1] == 5
+    assert np.abs(Z.T @ Z - np.eye(2 * ca.shape[1])).max() < 1e-10
+
+
+@pytest.mark.parametrize('tol, max_space', [(1e-6, 40), (1e-9, None)])
+def test_paired_davidson_matches_dense_roots(cfg0_ab, tol, max_space):
+    """Against the dense roots; max_space 40 of the 95 pairs restarts the
+    space every few cycles."""
+    inst, A, B = cfg0_ab
+    At, Bt = torch.as_tensor(A), torch.as_tensor(B)
+
+    def apply_ab(Z):
+        return (At @ Z.T).T, (Bt @ Z.T).T
+    d = torch.as_tensor(np.diag(A).copy())
+    nroots, ntrack = 5, 7
+    order = np.argsort(np.diag(A))[:10]
+    x0 = torch.zeros((A.shape[0], len(order)), dtype=torch.float64)
+    x0[torch.as_tensor(order), torch.arange(len(order))] = 1.0
+    res = casida_davidson(apply_ab, d, ntrack, x0, tol=tol, max_cycle=80, max_space=max_space)
+    ref, Xr, Yr = O.dense_casida_roots(A, B)
+    assert res.conv.all(), res.trace
+    assert np.abs(res.e[:nroots] - ref[:nroots]).max() < 10 * tol
+    X, Y = res.X.numpy(), res.Y.numpy()
+    assert np.allclose(np.sum(X * X, 0) - np.sum(Y * Y, 0), 1.0, atol=1e-8)
+    R1 = A @ X + B @ Y - X * res.e
+    R2 = B @ X + A @ Y + Y * res.e
+    assert np.sqrt((R1 ** 2).sum(0) + (R2 ** 2).sum(0)).max() <= tol * 1.0001
+
+
+class DenseOperator:
+    """The device operator's surface (apply_ab, screened_diagonal_device,
+    device) over dense CPU A and B, to drive run_davidson's Casida branch."""
+
+    def __init__(self, A, B, nocc, nvir):
+        self.A, self.B = torch.as_tensor(A), torch.as_tensor(B)
+        self.nocc, self.nvir, self.device = nocc, nvir, torch.device('cpu')
+
+    def apply_ab(self, z):
+        k = z.shape[0]
+        Z = z.reshape(k, -1)
+        return ((Z @ self.A).reshape(z.shape), (Z @ self.B).reshape(z.shape))
+
+    def screened_diagonal_device(self):
+        return torch.diagonal(self.A).reshape(self.nocc, self.nvir).clone()
+
+
+def operator_action(op):
+    def apply_AB(z):
+        return op.apply_ab(z)
+    apply_AB.operator = op
+    return apply_AB
+
+
+def test_run_davidson_full_casida_branch(cfg0_ab):
+    """run_davidson(tda=False): the guess, tracked roots, FP64 recheck and
+    warm start around the paired loop, on a dense stand-in operator."""
+    from mbptcode_amd.davidson import run_davidson
+    inst, A, B = cfg0_ab
+    op = operator_action(DenseOperator(A, B, inst.nocc, inst.nvir))
+    d = (inst.eps[inst.nocc:][None, :] - inst.eps[:inst.nocc][:, None])
+    ref = O.dense_casida_roots(A, B)[0][:inst.nroots]
+    timings, tracked = {}, {}
+    w, X, Y = run_davidson(op, d, inst.nroots, 1e-9, 100, tda=False, refine_apply=op,
+                           timings=timings, tracked=tracked)
+    assert np.abs(w - ref).max() < 1e-8
+    assert np.allclose(np.sum(X * X, 0) - np.sum(Y * Y, 0), 1.0, atol=1e-10)
+    assert max(timings['davidson_residuals']) <= 1e-9
+    assert timings['davidson_refine']['rnorm_before'] == pytest.approx(
+        timings['davidson_residuals'], rel=1e-6, abs=1e-14)
+    assert tracked['X'].shape[1] >= inst.nroots
+    warm = {}
+    w2, _, _ = run_davidson(op, d, inst.nroots, 1e-9, 100, tda=False, guess=(X, Y),
+                            timings=warm)
+    # the requested roots are converged at the first cycle (the tracked
+    # extras past them still iterate); without Y the guess is no root
+    assert warm['davidson_trace'][0]['converged'] >= inst.nroots
+    assert np.abs(w2 - w).max() < 1e-12
+    x_only = {}
+    run_davidson(op, d, inst.nroots, 1e-9, 100, tda=False, guess=(X, None), timings=x_only)
+    assert x_only['davidson_trace'][0]['converged'] == 0
+    # every tracked pair offered back: one cycle
+    whole = {}
+    run_davidson(op, d, inst.nroots, 1e-9, 100, tda=False,
+                 guess=(tracked['X'], tracked['Y']), timings=whole)
+    assert whole['davidson_iterations'] == 1
+    with pytest.raises(NotImplementedError):
+        run_davidson(op, d, inst.nroots, 1e-9, 100, tda=False, preconditioner='learned')

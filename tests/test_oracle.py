@@ -225,3 +225,27 @@ def test_guess_indices_extend_over_degeneracy():
     assert len(idx) == 6
     d2 = np.array([1.0] * 30)
     assert len(O.guess_indices(d2, 2)) == 20   # capped at max(4k, 20)
+
+
+def test_dense_casida_roots_solve_the_casida_equations(cfg0):
+    """The closed-form Casida roots: A X + B Y = w X, B X + A Y = -w Y to
+    round-off, X^T X - Y^T Y = 1, B = 0 gives eig(A), and the RPA trace
+    formula E_c = (sum w - Tr A)/2 agrees with rpa_energy_casida."""
+    inst, D, g, W_aux = cfg0
+    A = O.dense_tda_matrix(inst.eps, inst.nocc, inst.X, D, W_aux)
+    B = O.dense_casida_b_matrix(inst.nocc, inst.X, D, W_aux)
+    w, Xc, Yc = O.dense_casida_roots(A, B)
+    assert np.all(np.diff(w) >= 0) and w.min() > 0
+    assert np.abs(A @ Xc + B @ Yc - Xc * w).max() < 1e-11 * w.max()
+    assert np.abs(B @ Xc + A @ Yc + Yc * w).max() < 1e-11 * w.max()
+    assert np.allclose(np.sum(Xc * Xc, 0) - np.sum(Yc * Yc, 0), 1.0, atol=1e-12)
+    w0, X0, Y0 = O.dense_casida_roots(A, np.zeros_like(A))
+    assert np.allclose(w0, np.linalg.eigvalsh(A), rtol=1e-12)
+    assert np.abs(Y0).max() < 1e-12
+    # RPA: A = d + 2K, B = 2K (no W) -> Furche's trace
+    eps, X, D, W, z = small_system()
+    Arpa = O.dense_tda_matrix(eps, 4, X, D, np.zeros_like(W))
+    Brpa = O.dense_casida_b_matrix(4, X, D, np.zeros_like(W))
+    wr = O.dense_casida_roots(Arpa, Brpa)[0]
+    assert np.isclose(0.5 * (wr.sum() - np.trace(Arpa)),
+                      O.rpa_energy_casida(X, D, eps, 4), rtol=1e-10)
