@@ -35,7 +35,10 @@ Simplifications against the reference:
 * Tracked extra pairs must converge; the reference also settles them by their
   residual interval clearing the highest requested root.
 * There are no pair-symmetry sectors.
-* There is no dense completion or (A - B) probe.
+* There is no dense completion for a pair space the trial space exhausts.
+* A projected a - b that is not positive definite raises LinAlgError here;
+  `davidson.run_davidson` turns it into CasidaBreakdown or
+  UnstableReference by the (A - B) probe (`probe.py`).
 * Corrections keep their raw length.
 """
 import time

@@ -39,6 +39,11 @@ DAVIDSON_GUESS_FACTOR = 2               # :1886
 DAVIDSON_GUESS_DEG_TOL = 1e-3           # :2190
 DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
 
+# the (A - B) probe (constants.py:253-254, :1234)
+AMB_PROBE_TOL = 1e-6                    # residual bar of the converged probe, Ha
+AMB_PROBE_SIGN_TOLS = (1e-2, 1e-3, 1e-4)  # relative bars of the sign passes
+PROBE_START_MIX = 0.1                   # share of 1/d in the warm sign pass
+
 #: Hartree -> meV and -> eV.
 HARTREE_TO_MEV = 27211.386245988
 HARTREE_TO_EV = 27.211386245988

@@ -64,6 +64,85 @@ class DavidsonNotConverged(RuntimeError):
     (src/Base/step_failed.py:109)."""
 
 
+class CasidaInstability(RuntimeError):
+    """The Casida problem has no physical roots: (A - B) or omega^2 is not
+    positive on the pair space (the dense solver's refusal)."""
+
+
+class UnstableReference(CasidaInstability):
+    """The Casida problem refused for an unstable reference: min eig(A - B)
+    <= 0 measured (LR/davidson.py:232-241).  `min_eig_amb` is the measured
+    minimum, or None."""
+
+    def __init__(self, message, min_eig_amb=None):
+        super().__init__(message)
+        self.min_eig_amb = min_eig_amb
+
+
+class CasidaBreakdown(RuntimeError):
+    """A projected (A - B) block that is not positive definite, with the
+    min eig(A - B) the probe measured on the whole pair space, or None where
+    the probe did not converge (LR/davidson.py:221-229)."""
+
+    def __init__(self, message, min_eig_amb):
+        super().__init__(message)
+        self.min_eig_amb = min_eig_amb
+
+
+def unstable_reference(amb, when='before the solve returned its roots'):
+    """The UnstableReference for min eig(A - B) = amb <= 0
+    (LR/davidson.py:1282-1293)."""
+    return UnstableReference(
+        f'BSE refused {when}: min eig(A-B) = {amb:.6f} Ha <= 0 -- the mean-field '
+        'reference is singlet/triplet unstable and the Casida omega^2 reduction is '
+        'invalid there; stabilise the reference (check that the SCF is its lowest '
+        'solution) or solve the non-Hermitian problem', amb)
+
+
+def casida_breakdown(apply_AB, diag_d, conv_tol):
+    """The exception for a projected (A - B) block that is not positive
+    definite (LR/davidson.py:1243-1279): the cold (A - B) probe on the same
+    action tells a stable reference's numerical breakdown (CasidaBreakdown,
+    min eig > 0) from an unstable reference (UnstableReference); a probe that
+    does not converge gives a CasidaBreakdown with min_eig_amb None."""
+    from .probe import AmbProbeNotConverged, lowest_amb_from_action
+    try:
+        amb = float(lowest_amb_from_action(apply_AB, diag_d)[0])
+        detail = f'measured min eig(A-B) = {amb:.6f} Ha'
+    except AmbProbeNotConverged:
+        amb = None
+        detail = ('min eig(A-B) probe did not converge; smallest orbital-energy '
+                  f'difference {np.min(diag_d):.4f} Ha')
+    if amb is not None and amb > 0:
+        return CasidaBreakdown(
+            'Casida Davidson broke down numerically: the projected (A-B) block is '
+            f'not positive definite although (A-B) itself is ({detail}); the trial '
+            f'space lost accuracy as conv_tol ({conv_tol:g} Ha) neared round-off '
+            f'({residual_floor(diag_d):.1e} Ha here). Loosen conv_tol.', amb)
+    if amb is not None:
+        return unstable_reference(amb)
+    return CasidaBreakdown(
+        'Casida Davidson failed: the projected (A-B) block is not positive '
+        f'definite, the signature of an unstable mean-field reference ({detail}); '
+        'the omega^2 reduction is invalid in this regime.', amb)
+
+
+def probe_after_solve(apply_AB, diag_d, omega, X, Y, probe, stats=None, comm=None):
+    """min eig(A - B) on the action a Casida solve just ran on
+    (LR/davidson.py:356-373): the converged probe from 1/d for probe=True, the
+    sign alone for probe='sign', tried first on the span of the roots' X - Y.
+    A value <= 0 refuses the roots (UnstableReference).  Returns the value."""
+    from .probe import lowest_amb_from_action
+    sign_only = isinstance(probe, str) and probe.lower() == 'sign'
+    order = np.argsort(omega, kind='stable')
+    amb = float(lowest_amb_from_action(
+        apply_AB, diag_d, sign_only=sign_only, stats=stats, comm=comm,
+        start=(np.asarray(X) - np.asarray(Y))[:, order] if sign_only else None)[0])
+    if amb <= 0:
+        raise unstable_reference(amb)
+    return amb
+
+
 # ---------------------------------------------------------------------------
 # shared pieces (LR/davidson_shared.py)
 # ---------------------------------------------------------------------------
@@ -609,8 +688,12 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
             return Az.reshape(k, npair), Bz.reshape(k, npair)
         return ab
 
-    res = casida_davidson(ab_of(op), pdiag, track, x0, tol=conv_tol, max_cycle=max_cycle,
-                          max_space=max_space, y0=y0)
+    probe_on = apply_AB if refine_apply is None else refine_apply
+    try:
+        res = casida_davidson(ab_of(op), pdiag, track, x0, tol=conv_tol,
+                              max_cycle=max_cycle, max_space=max_space, y0=y0)
+    except np.linalg.LinAlgError as exc:
+        raise casida_breakdown(probe_on, diag_d, conv_tol) from exc
     cycles, vectors = res.cycles, res.vectors
     e, X, Y, conv, rn = np.asarray(res.e, float), res.X, res.Y, res.conv, res.rnorm
     refine = {}
@@ -626,9 +709,12 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
         rn64 = torch.sqrt((Rx * Rx).sum(0) + (Ry * Ry).sum(0)).cpu().numpy()
         refine['rnorm_before'] = rn64.copy()
         if not (rn64 <= conv_tol).all():
-            res2 = casida_davidson(ab64, pdiag, track, X.contiguous(), tol=conv_tol,
-                                   max_cycle=max_cycle, max_space=max_space,
-                                   y0=Y.contiguous())
+            try:
+                res2 = casida_davidson(ab64, pdiag, track, X.contiguous(), tol=conv_tol,
+                                       max_cycle=max_cycle, max_space=max_space,
+                                       y0=Y.contiguous())
+            except np.linalg.LinAlgError as exc:
+                raise casida_breakdown(refine_apply, diag_d, conv_tol) from exc
             e, X, Y, conv, rn = np.asarray(res2.e, float), res2.X, res2.Y, res2.conv, res2.rnorm
             refine['cycles'] = res2.cycles
             refine['vectors'] = res2.vectors
@@ -692,7 +778,10 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
     guess_factor and preconditioner (numerics.py:241-286); otherwise the
     keyword values (the reference defaults) are used.  precision: the block
     action's; with refine (default) a lower-precision solve is re-checked and
-    polished on an FP64 action.  Returns (omega, X, Y).
+    polished on an FP64 action.  probe (full solve only): True or 'sign' runs
+    the (A - B) probe after the solve on the FP64 action (`probe_after_solve`),
+    puts min eig(A - B) in stats['min_eig_amb'] and refuses the roots of an
+    unstable reference (UnstableReference).  Returns (omega, X, Y).
     """
     from .block_actions import block_action
     bse = getattr(numerics, 'bse', None)
@@ -718,8 +807,15 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
         refine_apply = block_action(lr_solver, nocc, polarizability, W_aux,
                                     isdf_factors, spin=spin, comm=comm, tda=tda,
                                     precision='fp64')[0]
-    return run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle,
-                        guess_factor=guess_factor, stats=stats, comm=comm,
-                        timings=timings, refuse_unconverged=refuse_unconverged,
-                        preconditioner=preconditioner, guess=guess, tda=tda,
-                        tracked=tracked, refine_apply=refine_apply, learned=learned)
+    omega, X, Y = run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle,
+                               guess_factor=guess_factor, stats=stats, comm=comm,
+                               timings=timings, refuse_unconverged=refuse_unconverged,
+                               preconditioner=preconditioner, guess=guess, tda=tda,
+                               tracked=tracked, refine_apply=refine_apply,
+                               learned=learned)
+    if probe and not tda:
+        amb = probe_after_solve(apply_AB if refine_apply is None else refine_apply,
+                                diag_d, omega, X, Y, probe, stats=stats, comm=comm)
+        if stats is not None:
+            stats['min_eig_amb'] = amb
+    return omega, X, Y

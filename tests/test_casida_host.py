@@ -135,3 +135,84 @@ def test_run_davidson_full_casida_branch(cfg0_ab):
     assert whole['davidson_iterations'] == 1
     with pytest.raises(NotImplementedError):
         run_davidson(op, d, inst.nroots, 1e-9, 100, tda=False, preconditioner='learned')
+
+
+def planted_amb(A, B, lowest):
+    """B' with eig(A - B') = eig(A - B) shifted so that its minimum is
+    `lowest` (an unstable reference for lowest < 0)."""
+    M = A - B
+    M = M - (np.linalg.eigvalsh(M)[0] - lowest) * np.eye(len(A))
+    return A - M
+
+
+@pytest.mark.parametrize('k', [1, 3])
+def test_amb_probe_lowest_values(cfg0_ab, k):
+    """The converged probe from 1/d against eigvalsh(A - B)
+    (probe.py:222-228; a Ritz value is good to |r|^2 / gap)."""
+    from mbptcode_amd.probe import lowest_amb_from_action
+    inst, A, B = cfg0_ab
+    op = operator_action(DenseOperator(A, B, inst.nocc, inst.nvir))
+    d = (inst.eps[inst.nocc:][None, :] - inst.eps[:inst.nocc][:, None])
+    stats = {}
+    got = lowest_amb_from_action(op, d, k=k, stats=stats)
+    ref = np.linalg.eigvalsh(A - B)[:k]
+    assert got.shape == (k,) and np.abs(got - ref).max() < 1e-9
+    assert stats['probe_source'] == 'davidson-cold' and stats['probe_matvecs'] > 0
+
+
+def test_amb_probe_sign_certificates(cfg0_ab):
+    """sign_only: the roots' span certifies a stable reference without a
+    Davidson; a planted negative minimum is certified negative from 1/d and
+    from a warm start; a minimum inside its own error bar is not called."""
+    from mbptcode_amd.probe import lowest_amb_from_action
+    inst, A, B = cfg0_ab
+    d = (inst.eps[inst.nocc:][None, :] - inst.eps[:inst.nocc][:, None])
+    op = operator_action(DenseOperator(A, B, inst.nocc, inst.nvir))
+    w, Xc, Yc = O.dense_casida_roots(A, B)
+    stats = {}
+    theta = lowest_amb_from_action(op, d, sign_only=True, start=(Xc - Yc)[:, :5],
+                                   stats=stats)[0]
+    assert stats['probe_source'] == 'roots' and stats['probe_sign_proven']
+    assert theta - stats['probe_residual'] > 0
+    assert theta >= np.linalg.eigvalsh(A - B)[0] - 1e-12   # a Ritz value bounds from above
+    cold = {}
+    lowest_amb_from_action(op, d, sign_only=True, stats=cold)
+    assert cold['probe_source'] == 'davidson-cold' and cold['probe_sign_proven']
+
+    bad = operator_action(DenseOperator(A, planted_amb(A, B, -0.01), inst.nocc, inst.nvir))
+    for start in (None, (Xc - Yc)[:, :5]):
+        st = {}
+        theta = lowest_amb_from_action(bad, d, sign_only=True, start=start, stats=st)[0]
+        assert st['probe_sign_proven'] and theta + st['probe_residual'] < 0
+        assert st['probe_source'] == ('davidson-cold' if start is None else 'roots')
+
+    flat = operator_action(DenseOperator(A, planted_amb(A, B, 1e-13), inst.nocc, inst.nvir))
+    for start in (None, (Xc - Yc)[:, :5]):
+        st = {}
+        theta = lowest_amb_from_action(flat, d, sign_only=True, start=start, stats=st)[0]
+        assert not st['probe_sign_proven'] and abs(theta) <= st['probe_residual']
+        assert st['probe_tol_used'] == 1e-6
+        assert st['probe_passes'][0] == ('davidson-cold' if start is None else 'davidson-warm')
+        assert len(st['probe_passes']) == 4
+
+
+def test_unstable_reference_and_breakdown(cfg0_ab):
+    """probe_after_solve refuses roots on an unstable reference; a projected
+    breakdown is told apart by the cold probe (LR/davidson.py:1243-1293)."""
+    from mbptcode_amd.davidson import (CasidaBreakdown, UnstableReference,
+                                       casida_breakdown, probe_after_solve)
+    inst, A, B = cfg0_ab
+    d = (inst.eps[inst.nocc:][None, :] - inst.eps[:inst.nocc][:, None])
+    w, Xc, Yc = O.dense_casida_roots(A, B)
+    good = operator_action(DenseOperator(A, B, inst.nocc, inst.nvir))
+    amb = probe_after_solve(good, d, w[:5], Xc[:, :5], Yc[:, :5], True)
+    assert abs(amb - np.linalg.eigvalsh(A - B)[0]) < 1e-9
+    assert probe_after_solve(good, d, w[:5], Xc[:, :5], Yc[:, :5], 'sign') > 0
+    bad = operator_action(DenseOperator(A, planted_amb(A, B, -0.01), inst.nocc, inst.nvir))
+    with pytest.raises(UnstableReference) as err:
+        probe_after_solve(bad, d, w[:5], Xc[:, :5], Yc[:, :5], 'sign')
+    assert err.value.min_eig_amb < 0
+    exc = casida_breakdown(good, d, 1e-12)
+    assert isinstance(exc, CasidaBreakdown) and exc.min_eig_amb > 0
+    exc = casida_breakdown(bad, d, 1e-8)
+    assert isinstance(exc, UnstableReference) and abs(exc.min_eig_amb + 0.01) < 1e-9

@@ -117,3 +117,62 @@ def test_learned_preconditioner_is_refused():
     _, _, W_grid = screened(inst)
     with pytest.raises(NotImplementedError):
         solve(inst, W_grid, 'fp64', 1e-6, preconditioner='learned')
+
+
+class LR:
+    """The two fields of the reference's solver object the entry reads."""
+
+    def __init__(self, eps):
+        self.eps = eps
+        self.spin_mode = 'restricted'
+
+
+def test_amb_probe_on_the_device_action():
+    """min eig(A - B) of the FP64 device action against eigvalsh(A - B): the
+    converged probe (k = 2), its spin invariance (kappa cancels), and the
+    sign certificate from the Casida roots (probe.py:23-228)."""
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.probe import lowest_amb_from_action
+    inst = synth.config(1)
+    D, W_aux, W_grid = screened(inst)
+    A, B = dense(inst, D, W_aux)
+    ref = np.linalg.eigvalsh(A - B)[:2]
+    print('amb probe cfg1 k=2', flush=True)
+    vals = {}
+    for kappa in (2.0, 0.0):
+        act, d = grid_block_action(inst.eps, inst.nocc, inst.X, W_grid, inst.V, kappa=kappa,
+                                   precision='fp64', tda=False)
+        vals[kappa] = lowest_amb_from_action(act, d, k=2)
+        assert np.abs(vals[kappa] - ref).max() < 1e-9, (vals[kappa], ref)
+    w, X, Y = solve(inst, W_grid, 'fp64', 1e-6)
+    act, d = grid_block_action(inst.eps, inst.nocc, inst.X, W_grid, inst.V,
+                               precision='fp64', tda=False)
+    stats = {}
+    theta = lowest_amb_from_action(act, d, sign_only=True, start=X - Y, stats=stats)[0]
+    assert stats['probe_sign_proven'] and theta - stats['probe_residual'] > 0
+    assert theta >= ref[0] - 1e-12
+
+
+@pytest.mark.parametrize('tda, probe', [(True, False), (False, 'sign'), (False, True)])
+def test_solve_casida_davidson_entry(tda, probe):
+    """The reference's entry (LR/davidson.py:252) with factor-gauge W_aux
+    and isdf_factors (X, D), fp32 with the FP64 recheck, against the dense
+    roots; probe= runs the (A - B) probe after the full solve."""
+    from mbptcode_amd.davidson import solve_casida_davidson
+    inst = synth.config(1)
+    D, W_aux, _ = screened(inst)
+    A, B = dense(inst, D, W_aux)
+    ref = (np.linalg.eigvalsh(A) if tda else O.dense_casida_roots(A, B)[0])[:5]
+    stats = {}
+    print(f'solve_casida_davidson cfg1 tda={tda} probe={probe}', flush=True)
+    w, X, Y = solve_casida_davidson(LR(inst.eps), inst.nocc, polarizability='BSE',
+                                    W_aux=W_aux, isdf_factors=(inst.X, D), tda=tda,
+                                    probe=probe, nroots=5, conv_tol=1e-6, stats=stats)
+    assert np.abs(w - ref).max() * HA_MEV < 1e-3
+    if tda:
+        assert not Y.any() and 'min_eig_amb' not in stats
+    else:
+        amb = np.linalg.eigvalsh(A - B)[0]
+        assert stats['min_eig_amb'] > 0
+        if probe is True:
+            assert abs(stats['min_eig_amb'] - amb) < 1e-9
