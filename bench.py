@@ -376,8 +376,39 @@ def leg_full_action(inst, W, k=20, reps=5):
     log(f'full action (k={k}): fp64 A {res["fp64"]["A_ms"]:.2f} / A+B {res["fp64"]["AB_ms"]:.2f} ms, '
         f'bf16x3 A {res["bf16x3"]["A_ms"]:.2f} / A+B {res["bf16x3"]["AB_ms"]:.2f} ms; '
         f'B rel err bf16x3 vs fp64 {err:.1e}')
+    # the full Casida Davidson on the same instance: split-BF16 action with the
+    # FP64 recheck against the plain FP64 solve
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.davidson import run_davidson
+    solves, roots = {}, {}
+    for name, prec in (('bf16x3+fp64_recheck', 'bf16x3'), ('plain_fp64', 'fp64')):
+        act, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision=prec,
+                                   tda=False)
+        ref64 = None
+        if prec != 'fp64':
+            ref64 = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V,
+                                      precision='fp64', tda=False)[0]
+        tm = {}
+        sync()
+        t0 = time.perf_counter()
+        w, _, _ = run_davidson(act, d, inst.nroots, inst.tol, 200, tda=False,
+                               refine_apply=ref64, timings=tm)
+        sync()
+        dt = time.perf_counter() - t0
+        roots[name] = w
+        solves[name] = {'s': dt, 'roots_per_s': inst.nroots / dt,
+                        'cycles': tm['davidson_iterations'], 'vectors': tm['davidson_vectors'],
+                        'max_residual_fp64': max(tm['davidson_residuals'])}
+        log(f'full Casida Davidson {name}: {dt * 1e3:.1f} ms, {inst.nroots / dt:.1f} roots/s, '
+            f'{tm["davidson_iterations"]} cycles, {tm["davidson_vectors"]} vectors')
+        del act, ref64
+    derr = float(np.abs(roots['bf16x3+fp64_recheck'] - roots['plain_fp64']).max() * HA_MEV)
     return {'workload': f'headline instance, k = {k} trial vectors per apply',
-            'precisions': res, 'B_rel_err_bf16x3_vs_fp64': err}
+            'precisions': res, 'B_rel_err_bf16x3_vs_fp64': err,
+            'davidson': {'nroots': inst.nroots, 'tol': inst.tol, 'solvers': solves,
+                         'max_root_diff_meV': derr,
+                         'speedup_vs_plain_fp64': solves['bf16x3+fp64_recheck']['roots_per_s']
+                         / solves['plain_fp64']['roots_per_s']}}
 
 
 def leg_batched():
