@@ -134,6 +134,28 @@ int mbpt_chi0_profile(mbpt_chi0 *h, int enable);
 int mbpt_chi0_profile_read(const mbpt_chi0 *h, double *ms, long *launches,
                            double *flops);
 
+/* The space-time GW self-energy's branch sums on the grid (FP32 handles):
+ *
+ *   out[0][t][j] = + sum_PQ X[P][p] (Zt(t) o Go(t))_PQ X[Q][p]
+ *   out[1][t][j] = - sum_PQ X[P][p] (Zt(t) o Gv(t))_PQ X[Q][p],  p = states[j]
+ *
+ * with Go, Gv the handle's Green's functions at tau_host[t] (as in
+ * mbpt_chi0_build) and Zt_dev (ntau, M, M) FP32 row-major, the correlation
+ * part of the screened interaction on the grid at each time point.
+ * out_dev (2, ntau, nstates) FP64.  Replaces `self_energy_branch_sums` /
+ * `SigmaPairs.add` (src/SingleReference/GW/imaginary_time.py:564-578,
+ * :75-92) in the grid gauge, where Zt = D Wt D^T is formed once per time. */
+#define MBPT_GW_MAX_STATES 4096
+int mbpt_chi0_sigma(mbpt_chi0 *h, int ntau, const double *tau_host, const float *Zt_dev,
+                    int nstates, const int *states_host, double *out_dev, void *stream);
+
+/* Device timing of mbpt_chi0_sigma (enabled and reset by mbpt_chi0_profile):
+ * milliseconds, calls and algorithmic flops, ntau M(M+1)(nocc + nvir + 1 +
+ * 2 nstates) per call (the triangle's Green's function products, the
+ * Hadamard product and the two branches' state contractions). */
+int mbpt_chi0_sigma_profile_read(const mbpt_chi0 *h, double *ms, long *launches,
+                                 double *flops);
+
 int mbpt_chi0_destroy(mbpt_chi0 *h);
 
 /* ------------------------------------------------------------------------

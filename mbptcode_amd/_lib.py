@@ -17,6 +17,7 @@ FP64, FP32, BF16X3 = 0, 1, 2
 PRECISIONS = {'fp64': FP64, 'fp32': FP32, 'bf16x3': BF16X3}
 OK, ERR_VALUE, ERR_HIP, ERR_UNSUPPORTED = 0, 1, 2, 3
 CHI0_MAX_TAU = 64
+GW_MAX_STATES = 4096
 
 #: Every symbol include/mbpt_hip.h declares.
 EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
@@ -24,7 +25,7 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_sigma_held_bytes', 'mbpt_sigma_profile',
            'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
            'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
-           'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
+           'mbpt_chi0_sigma', 'mbpt_chi0_sigma_profile_read', 'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
            'mbpt_mlp_nparams', 'mbpt_mlp_create', 'mbpt_mlp_get_params',
            'mbpt_mlp_forward', 'mbpt_mlp_gradient', 'mbpt_mlp_train_step',
            'mbpt_mlp_correction', 'mbpt_mlp_correction_batched', 'mbpt_mlp_destroy',
@@ -55,6 +56,8 @@ _SIGNATURES = {
     'mbpt_chi0_build': (_i, [_vp, _i, _dp, _dp, _vp, _i, _vp]),
     'mbpt_chi0_profile': (_i, [_vp, _i]),
     'mbpt_chi0_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
+    'mbpt_chi0_sigma': (_i, [_vp, _i, _dp, _vp, _i, _ip, _vp, _vp]),
+    'mbpt_chi0_sigma_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
     'mbpt_chi0_destroy': (_i, [_vp]),
     'mbpt_gemm_tn_f64_work': (ctypes.c_long, [_i, _i, _i]),
     'mbpt_gemm_tn_f64': (_i, [_i, _i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, ctypes.c_long,

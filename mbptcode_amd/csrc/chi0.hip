@@ -11,6 +11,7 @@
 // is written once; off-diagonal tiles are mirrored into the lower triangle.
 #include "common.h"
 #include "chi0_f32.h"
+#include "gw_sigma_f32.h"
 
 #include <algorithm>
 #include <cmath>
@@ -142,6 +143,12 @@ struct mbpt_chi0 {
   float *Yt, *part, *XT;         // FP32 path: scaled branches, per-chunk sums, X K-major
   size_t cap_yt, cap_part;       // bytes
   long prof_launches;
+  // GW self-energy: packed state columns, their column indices, tile partials
+  float *gw_xs, *gw_part;
+  int *gw_cols;
+  size_t cap_xs, cap_gwpart, cap_cols;
+  double gw_ms, gw_flops;
+  long gw_launches;
 };
 
 namespace {
@@ -244,6 +251,66 @@ int build_f32(mbpt_chi0 *h, int ntau, const double *tau, const double *w, double
   return MBPT_OK;
 }
 
+template <typename B>
+int grow_bytes(B *&buf, size_t &cap, size_t bytes) {
+  if (bytes <= cap) return MBPT_OK;
+  (void)hipFree(buf);
+  buf = nullptr;
+  cap = 0;
+  MBPT_HIP_TRY(hipMalloc(&buf, bytes));
+  cap = bytes;
+  return MBPT_OK;
+}
+
+int sigma_f32(mbpt_chi0 *h, int ntau, const double *tau, const float *Zt, int ns,
+              const int *states, double *out, hipStream_t st) {
+  const int M = h->M, Mp = (M + kCT - 1) / kCT * kCT;
+  const int ko = (h->no + 15) / 16 * 16, kv = (h->nv + 15) / 16 * 16;
+  const int nt = Mp / kCT, ntiles = nt * (nt + 1) / 2;
+  const int nsp = (ns + 15) / 16 * 16;
+  int rc = grow(h->Yt, h->cap_yt, sizeof(float) * (size_t)ntau * (ko + kv) * Mp);
+  if (!rc) rc = grow_bytes(h->gw_xs, h->cap_xs, sizeof(float) * (size_t)Mp * nsp);
+  if (!rc) rc = grow_bytes(h->gw_part, h->cap_gwpart,
+                           sizeof(float) * (size_t)ntau * ntiles * 2 * nsp);
+  if (!rc) rc = grow_bytes(h->gw_cols, h->cap_cols, sizeof(int) * (size_t)ns);
+  if (rc) return rc;
+  std::vector<int> cols(ns);
+  for (int j = 0; j < ns; ++j) cols[j] = states[j] < h->no ? states[j] : ko + (states[j] - h->no);
+  MBPT_HIP_TRY(hipMemcpyAsync(h->gw_cols, cols.data(), sizeof(int) * ns, hipMemcpyHostToDevice, st));
+  MBPT_HIP_TRY(hipStreamSynchronize(st));   // cols is a host temporary
+  TauW tw;
+  tw.n = ntau;
+  for (int t = 0; t < ntau; ++t) {
+    tw.tau_half[t] = 0.5 * tau[t];
+    tw.w[t] = 0.f;
+  }
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->e0, st));
+  k_chi0_prep<<<ntau * (ko + kv), 256, 0, st>>>(M, Mp, h->no, h->nv, ko, kv, h->XT, h->eo,
+                                                  h->ev, tw, h->Yt);
+  if ((rc = launch_status("k_chi0_prep"))) return rc;
+  const long nx = (long)Mp * nsp;
+  k_gw_pack_states<<<(int)std::min<long>((nx + 255) / 256, 4096L), 256, 0, st>>>(
+      M, Mp, ns, nsp, h->gw_cols, h->XT, h->gw_xs);
+  if ((rc = launch_status("k_gw_pack_states"))) return rc;
+  const int nblk = pad_grid((long)ntiles * ntau);
+  k_gw_sigma_f32<<<nblk, 512, 0, st>>>(M, Mp, ko, kv, nt, ntau, h->Yt, Zt, h->gw_xs, nsp,
+                                       h->gw_part, nblk);
+  if ((rc = launch_status("k_gw_sigma_f32"))) return rc;
+  k_gw_sigma_reduce<<<(2 * ntau * ns + 255) / 256, 256, 0, st>>>(ntau, ntiles, ns, nsp,
+                                                                 h->gw_part, out);
+  if ((rc = launch_status("k_gw_sigma_reduce"))) return rc;
+  if (h->prof) {
+    MBPT_HIP_TRY(hipEventRecord(h->e1, st));
+    MBPT_HIP_TRY(hipEventSynchronize(h->e1));
+    float ms = 0.f;
+    MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->e0, h->e1));
+    h->gw_ms += ms;
+    h->gw_flops += (double)ntau * (double)M * (M + 1.0) * (h->no + h->nv + 1.0 + 2.0 * ns);
+    h->gw_launches += 1;
+  }
+  return MBPT_OK;
+}
+
 template <typename T> int split_impl(mbpt_chi0 *h, const double *X, hipStream_t st) {
   const long n = (long)h->M * (h->no + h->nv);
   k_split_X2<T><<<(int)std::min<long>((n + 255) / 256, 65535L), 256, 0, st>>>(
@@ -333,6 +400,35 @@ int mbpt_chi0_build(mbpt_chi0 *h, int ntau, const double *tau_host,
              : build_f32(h, ntau, tau_host, weight_host, out_dev, accumulate, st);
 }
 
+int mbpt_chi0_sigma(mbpt_chi0 *h, int ntau, const double *tau_host, const float *Zt_dev,
+                    int nstates, const int *states_host, double *out_dev, void *stream) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_chi0_sigma: NULL handle");
+  if (h->prec != MBPT_FP32) {
+    set_error("mbpt_chi0_sigma: the self-energy sums are built for MBPT_FP32 handles");
+    return MBPT_ERR_UNSUPPORTED;
+  }
+  MBPT_REQUIRE(ntau >= 1 && ntau <= MBPT_CHI0_MAX_TAU, "mbpt_chi0_sigma: ntau must be in [1, 64]");
+  MBPT_REQUIRE(nstates >= 1 && nstates <= MBPT_GW_MAX_STATES,
+               "mbpt_chi0_sigma: nstates must be in [1, MBPT_GW_MAX_STATES]");
+  MBPT_REQUIRE(tau_host && Zt_dev && states_host && out_dev, "mbpt_chi0_sigma: NULL argument");
+  for (int t = 0; t < ntau; ++t)
+    MBPT_REQUIRE(tau_host[t] >= 0.0, "mbpt_chi0_sigma: time points must be non-negative");
+  for (int j = 0; j < nstates; ++j)
+    MBPT_REQUIRE(states_host[j] >= 0 && states_host[j] < h->no + h->nv,
+                 "mbpt_chi0_sigma: a state index is outside the orbitals");
+  return sigma_f32(h, ntau, tau_host, Zt_dev, nstates, states_host, out_dev,
+                   (hipStream_t)stream);
+}
+
+int mbpt_chi0_sigma_profile_read(const mbpt_chi0 *h, double *ms, long *launches,
+                                 double *flops) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_chi0_sigma_profile_read: NULL handle");
+  if (ms) *ms = h->gw_ms;
+  if (launches) *launches = h->gw_launches;
+  if (flops) *flops = h->gw_flops;
+  return MBPT_OK;
+}
+
 int mbpt_chi0_profile(mbpt_chi0 *h, int enable) {
   MBPT_REQUIRE(h != nullptr, "mbpt_chi0_profile: NULL handle");
   if (enable && !h->prof) {
@@ -346,6 +442,8 @@ int mbpt_chi0_profile(mbpt_chi0 *h, int enable) {
   h->prof = enable ? 1 : 0;
   h->prof_ms = h->prof_flops = 0.0;
   h->prof_launches = 0;
+  h->gw_ms = h->gw_flops = 0.0;
+  h->gw_launches = 0;
   return MBPT_OK;
 }
 
@@ -367,6 +465,7 @@ int mbpt_chi0_destroy(mbpt_chi0 *h) {
   (void)hipFree(h->Xo); (void)hipFree(h->Xv); (void)hipFree(h->so); (void)hipFree(h->sv);
   (void)hipFree(h->eo); (void)hipFree(h->ev);
   (void)hipFree(h->Yt); (void)hipFree(h->part); (void)hipFree(h->XT);
+  (void)hipFree(h->gw_xs); (void)hipFree(h->gw_part); (void)hipFree(h->gw_cols);
   delete h;
   return MBPT_OK;
 }

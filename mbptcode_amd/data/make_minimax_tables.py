@@ -1,12 +1,15 @@
-"""Regenerate ``minimax_tau.json`` from the GreenX minimax imaginary-time tables.
+"""Regenerate ``minimax_tau.json`` and ``minimax_omega.json`` from the GreenX
+minimax tables.
 
-The numbers are the GreenX "Laplace" coefficients (GX-TimeFrequency,
-minimax_tau.F90; Apache-2.0, see ``NOTICE`` beside this file) as the reference
-ships them in ``src/Base/utils/grids/minimax_tau_data.json``.  They are data,
-not code: a minimax grid is a table lookup, so the product needs the same
-numbers.  This script only re-serialises them into this package's layout
+The numbers are the GreenX coefficients (GX-TimeFrequency, minimax_tau.F90 for
+the imaginary-time "Laplace" grids and minimax_omega.F90 for the
+imaginary-frequency grids; Apache-2.0, see ``NOTICE`` beside this file) as the
+reference ships them in ``src/Base/utils/grids/minimax_tau_data.json`` and
+``minimax_omega_data.json``.  They are data, not code: a minimax grid is a
+table lookup, so the product needs the same numbers.  This script only
+re-serialises them into this package's layout
 
-    {"<ntau>": {"ranges": [R_1, ..., R_n], "rows": [[tau_1..tau_n, w_1..w_n], ...]}}
+    {"<n>": {"ranges": [R_1, ..., R_n], "rows": [[x_1..x_n, w_1..w_n], ...]}}
 
 where row j serves e_max/e_min < ranges[j] and the last row everything wider.
 It runs only where the reference tree is mounted; the JSON it writes is
@@ -16,11 +19,15 @@ import json
 import os
 import sys
 
-SRC = '/root/reference/src/Base/utils/grids/minimax_tau_data.json'
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'minimax_tau.json')
+SRC_DIR = '/root/reference/src/Base/utils/grids'
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLES = (('minimax_tau_data.json', 'minimax_tau.json',
+           'GreenX GX-TimeFrequency minimax_tau (Apache-2.0)'),
+          ('minimax_omega_data.json', 'minimax_omega.json',
+           'GreenX GX-TimeFrequency minimax_omega (Apache-2.0)'))
 
 
-def main(src=SRC, out=OUT):
+def pack(src, out, origin):
     with open(src) as f:
         table = json.load(f)
     packed = {}
@@ -30,9 +37,13 @@ def main(src=SRC, out=OUT):
                         'rows': [[float(x) for x in row]
                                  for row in entry['aw_erange_matrix']]}
     with open(out, 'w') as f:
-        json.dump({'origin': 'GreenX GX-TimeFrequency minimax_tau (Apache-2.0)',
-                   'tables': packed}, f, separators=(',', ':'))
+        json.dump({'origin': origin, 'tables': packed}, f, separators=(',', ':'))
     return out
+
+
+def main(src_dir=SRC_DIR, out_dir=HERE):
+    return [pack(os.path.join(src_dir, s), os.path.join(out_dir, o), origin)
+            for s, o, origin in TABLES]
 
 
 if __name__ == '__main__':

@@ -8,7 +8,11 @@
   time scaling of the time-frequency pair;
 * the Laplace test and the automatic point count (time_frequency.py:151-215);
 * Gauss-Legendre frequencies on [0, inf) and the gap-scaled w0
-  (grids.py:161-185).
+  (grids.py:161-185);
+* minimax imaginary-frequency grids (grids.py:107-126), the four cosine /
+  sine transforms (time_frequency.py:257-380), the self-energy's fit ranges
+  (:387-427) and the space-time GW axes (GW/space_time.py:559-618,
+  GW/imaginary_time.py:95-138).
 
 These are set-up computations of a few microseconds to milliseconds; they run
 on the host in FP64 and hand plain arrays (tau points, transform weights) to
@@ -23,9 +27,19 @@ import numpy as np
 from .constants import DEFAULT_TAU_TARGET, TRANSFORM_FIT_RCOND
 from .orbital_spaces import particle_hole_window
 
-_TABLE_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data',
-                           'minimax_tau.json')
-_tables = None
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data')
+_TABLE_FILES = {'tau': 'minimax_tau.json', 'omega': 'minimax_omega.json'}
+_tables = {}
+
+#: GreenX transformation-type codes (minimax_utils.F90, time_frequency.py:95):
+#: cosine tau -> omega, cosine omega -> tau, sine tau -> omega, sine omega -> tau
+COSINE_TW, COSINE_WT, SINE_TW, SINE_WT = 1, 2, 3, 4
+#: the self-energy fit ranges' padding (time_frequency.py:230)
+SELF_ENERGY_PAD = (0.3, 3.0)
+#: Pade points of the space-time self-energy and the scale of their
+#: Gauss-Legendre axis, Ha (constants.py:1849-1851)
+DEFAULT_NPADE = 16
+GL_FREQUENCY_SCALE = 1.0
 
 #: Fits at or above this many input points get a Tikhonov term
 #: (time_frequency.py:109-110).
@@ -35,17 +49,21 @@ DEFAULT_REGULARIZATION = 1e-8
 FIT_ERROR_WARN = 1e-3
 
 
-def _load_tables():
-    global _tables
-    if _tables is None:
-        with open(_TABLE_PATH) as f:
-            _tables = json.load(f)['tables']
-    return _tables
+def _load_tables(axis='tau'):
+    if axis not in _tables:
+        with open(os.path.join(_DATA, _TABLE_FILES[axis])) as f:
+            _tables[axis] = json.load(f)['tables']
+    return _tables[axis]
 
 
 def minimax_tau_supported_sizes():
     """Tabulated time-grid sizes (even, 6..34)."""
     return sorted(int(k) for k in _load_tables())
+
+
+def minimax_supported_sizes():
+    """Tabulated frequency-grid sizes (grids.py:42-44)."""
+    return sorted(int(k) for k in _load_tables('omega'))
 
 
 def _require_window(e_min, e_max, who):
@@ -57,7 +75,7 @@ def _require_window(e_min, e_max, who):
                          f'e_min = {e_min:.6g}')
 
 
-def table_row(ntau, e_min, e_max):
+def table_row(ntau, e_min, e_max, axis='tau'):
     """(row, e_ratio): the coefficient row for this size and range, and the
     narrow-range stretch factor (grids.py:78-104).
 
@@ -66,10 +84,11 @@ def table_row(ntau, e_min, e_max):
     than 20 points are stretched by e_ratio = R_0 / R (divided by 1.5 when that
     exceeds 1.5); time points then MULTIPLY by it.
     """
-    entry = _load_tables().get(str(int(ntau)))
+    entry = _load_tables(axis).get(str(int(ntau)))
     if entry is None:
-        raise ValueError(f'ntau={ntau} has no tabulated minimax time grid; '
-                         f'supported: {minimax_tau_supported_sizes()}')
+        sizes = sorted(int(k) for k in _load_tables(axis))
+        raise ValueError(f'{ntau} points have no tabulated minimax {axis} grid; '
+                         f'supported: {sizes}')
     ranges = entry['ranges']
     ratio = e_max / e_min
     pick = len(ranges)
@@ -83,6 +102,15 @@ def table_row(ntau, e_min, e_max):
         if stretch > 1.5:
             stretch /= 1.5
     return np.asarray(entry['rows'][pick], float), stretch
+
+
+def minimax_frequency_grid(nfreq, e_min, e_max):
+    """(omega, w): the minimax imaginary-frequency grid on [e_min, e_max]
+    (grids.py:107-126): table row / e_ratio * e_min (a frequency carries
+    energy, so the narrow-range stretch divides)."""
+    _require_window(e_min, e_max, 'minimax_frequency_grid')
+    row, stretch = table_row(nfreq, e_min, e_max, axis='omega')
+    return row[:nfreq] / stretch * e_min, row[nfreq:] / stretch * e_min
 
 
 def minimax_time_grid(ntau, e_min, e_max):
@@ -140,44 +168,125 @@ def gap_scaled_w0(eps, nocc, scale=0.5):
     return w0
 
 
-def cosine_transform_weights(tau, omega, e_min, e_max, regularization=None,
-                             nodes_factor=200, warn=True):
-    """(C, max_fit_error): the tau -> omega cosine weights, C[k, j] including the
-    cos(omega_k tau_j) factor, fitted row by row so that
-    sum_j C[k, j] e^{-x tau_j} = 2x / (x^2 + omega_k^2) on a log-spaced node set
-    over [e_min, e_max] (time_frequency.py:282-380, kind COSINE_TW).
+def _fit_target(kind, tau, omega, i, x):
+    """(psi, A): the fitted function on the node set x and the design matrix
+    for output point i of transform `kind` (time_frequency.py:257-279):
 
-    Node count: max((floor(log10 R) + 1) * nodes_factor, ntau).  Each row is a
+    * COSINE_TW: sum_j c_j cos(w tau_j) e^{-x tau_j} = 2x / (x^2 + w^2);
+    * COSINE_WT: sum_k c_k cos(t w_k) 2x / (x^2 + w_k^2) = e^{-x t};
+    * SINE_TW: sum_j c_j sin(w tau_j) e^{-x tau_j} = 2w / (x^2 + w^2);
+    * SINE_WT: sum_k c_k sin(t w_k) 2w_k / (x^2 + w_k^2) = e^{-x t}."""
+    if kind in (COSINE_TW, SINE_TW):
+        w = omega[i]
+        trig = np.cos if kind == COSINE_TW else np.sin
+        psi = 2 * (x if kind == COSINE_TW else w) / (x ** 2 + w ** 2)
+        return psi, trig(w * tau)[None, :] * np.exp(-np.outer(x, tau))
+    if kind in (COSINE_WT, SINE_WT):
+        t = tau[i]
+        lor = 1.0 / (x[:, None] ** 2 + omega[None, :] ** 2)
+        if kind == COSINE_WT:
+            return np.exp(-x * t), np.cos(t * omega)[None, :] * 2 * x[:, None] * lor
+        return np.exp(-x * t), np.sin(t * omega)[None, :] * 2 * omega[None, :] * lor
+    raise ValueError(f'unknown transform kind {kind}')
+
+
+def transform_weights(kind, tau, omega, e_min, e_max, regularization=None,
+                      nodes_factor=200, warn=True):
+    """(T, max_fit_error): one transform matrix, fitted row by row on a
+    log-spaced node set over [e_min, e_max] (time_frequency.py:282-380).
+    Rows are the output axis (omega for *_TW, tau for *_WT); the cos / sin
+    factor is included.
+
+    Node count: max((floor(log10 R) + 1) * nodes_factor, n_in).  Each row is a
     filtered SVD solve, filter S / (reg^2 + S^2) on S > rcond * S_max, with
-    reg = 0 below 20 input points and 1e-8 at or above.
-    """
+    reg = 0 below 20 input points and 1e-8 at or above."""
     tau = np.asarray(tau, float)
     omega = np.asarray(omega, float)
-    n = len(tau)
+    to_omega = kind in (COSINE_TW, SINE_TW)
+    n_out = len(omega) if to_omega else len(tau)
+    n = len(tau) if to_omega else len(omega)
     if regularization is None:
         regularization = 0.0 if n < REGULARIZATION_ABOVE else DEFAULT_REGULARIZATION
     nx = max((int(np.log10(e_max / e_min)) + 1) * nodes_factor, n)
     x = e_min * (e_max / e_min) ** (np.arange(nx) / (nx - 1.0))
-    decay = np.exp(-np.outer(x, tau))                    # (nx, ntau)
-    C = np.zeros((len(omega), n))
+    T = np.zeros((n_out, n))
     worst = 0.0
-    for k, w in enumerate(omega):
-        psi = 2 * x / (x ** 2 + w ** 2)
-        A = np.cos(w * tau)[None, :] * decay
+    for i in range(n_out):
+        psi, A = _fit_target(kind, tau, omega, i, x)
         U, S, Vt = np.linalg.svd(A, full_matrices=False)
         filt = np.zeros_like(S)
         np.divide(S, regularization ** 2 + S ** 2, out=filt,
                   where=S > TRANSFORM_FIT_RCOND * S[0])
-        C[k] = Vt.T @ (filt * (U.T @ psi))
-        err = float(np.abs(A @ C[k] - psi).max())
-        if not (np.isfinite(err) and np.isfinite(C[k]).all()):
-            raise FloatingPointError(f'cosine transform row {k} is not finite')
+        T[i] = Vt.T @ (filt * (U.T @ psi))
+        err = float(np.abs(A @ T[i] - psi).max())
+        if not (np.isfinite(err) and np.isfinite(T[i]).all()):
+            raise FloatingPointError(f'transform kind {kind}: row {i} is not finite')
         worst = max(worst, err)
     if warn and not worst <= FIT_ERROR_WARN:
-        warnings.warn(f'minimax cosine transform fit reached only {worst:.2e} '
+        warnings.warn(f'minimax transform (kind {kind}) fit reached only {worst:.2e} '
                       f'at {n} points over e_max/e_min = {e_max / e_min:.3g}',
                       RuntimeWarning, stacklevel=2)
-    return C * np.cos(np.outer(omega, tau)), worst
+    trig = np.cos if kind in (COSINE_TW, COSINE_WT) else np.sin
+    phase = trig(np.outer(omega, tau))
+    return (T * phase if to_omega else T * phase.T), worst
+
+
+def cosine_transform_weights(tau, omega, e_min, e_max, regularization=None,
+                             nodes_factor=200, warn=True):
+    """(C, max_fit_error): the tau -> omega cosine weights (kind COSINE_TW),
+    C[k, j] including the cos(omega_k tau_j) factor, so that
+    sum_j C[k, j] e^{-x tau_j} ~ 2x / (x^2 + omega_k^2) on [e_min, e_max]."""
+    return transform_weights(COSINE_TW, tau, omega, e_min, e_max, regularization,
+                             nodes_factor, warn)
+
+
+def self_energy_fit_ranges_from_window(eps, mu, w_lo, w_hi, pad=SELF_ENERGY_PAD):
+    """((rW_lo, rW_hi), (rS_lo, rS_hi)) (time_frequency.py:387-398): the
+    screening window [w_lo, w_hi] padded, and the self-energy's decay rates
+    |eps - mu| + Omega, sums of both factors' rates, padded alike."""
+    lo, hi = pad
+    dG = np.abs(np.asarray(eps, float) - mu)
+    return ((lo * w_lo, hi * w_hi),
+            (lo * (float(dG.min()) + w_lo), hi * (float(dG.max()) + w_hi)))
+
+
+def sigma_fit_ranges(eps, nocc, mu=None):
+    """(rW, rS): the ranges the W(i omega) -> tau and Sigma(tau) -> i omega
+    transforms are fitted on (time_frequency.py:401-427), mu at mid-gap by
+    default."""
+    eps = np.asarray(eps, float)
+    occ, virt = eps[:nocc], eps[nocc:]
+    if mu is None:
+        mu = 0.5 * (occ.max() + virt.min())
+    return self_energy_fit_ranges_from_window(eps, mu, virt.min() - occ.max(),
+                                              virt.max() - occ.min())
+
+
+def minimax_points_for_ranges(ratios, target=DEFAULT_TAU_TARGET, npoints_max=34):
+    """(n, worst error): the smallest tabulated size resolving every range
+    ratio; the widest binds (time_frequency.py:233-254)."""
+    npoints, worst = 0, 0.0
+    for R in ratios:
+        n, err = minimax_points_for_accuracy(1.0, R, target=target,
+                                             npoints_max=npoints_max)
+        npoints, worst = max(npoints, n), max(worst, err)
+    return npoints, worst
+
+
+def minimax_points_for_gw(eps, nocc, mu=None, target=DEFAULT_TAU_TARGET, npoints_max=34):
+    """(ntau, worst error) over the three ranges of the space-time route:
+    chi0's [gap, span], rW and rS (GW/imaginary_time.py:108-138)."""
+    gap, span, mid = particle_hole_window(eps, nocc)
+    rW, rS = sigma_fit_ranges(eps, nocc, mid if mu is None else mu)
+    return minimax_points_for_ranges((span / gap, rW[1] / rW[0], rS[1] / rS[0]),
+                                     target=target, npoints_max=npoints_max)
+
+
+def screening_frequency_grid(ntau, eps, nocc, mu=None):
+    """(omega, w): the minimax frequencies W(i omega) is sampled on, over rW
+    (GW/imaginary_time.py:95-105)."""
+    rW, _ = sigma_fit_ranges(eps, nocc, mu=mu)
+    return minimax_frequency_grid(ntau, *rW)
 
 
 class TimeFrequencyGrid:
@@ -230,6 +339,34 @@ def static_screening_grid(eps, nocc, ntau=None, tau_target=DEFAULT_TAU_TARGET):
     if ntau is None or (isinstance(ntau, str) and ntau.lower() == 'auto'):
         ntau, _ = minimax_points_for_accuracy(gap, span, target=tau_target)
     return TimeFrequencyGrid.minimax_split(int(ntau), gap, span, [0.0], [1.0])
+
+
+def space_time_gw_grids(eps, nocc, ntau=None, npade=DEFAULT_NPADE, w0=GL_FREQUENCY_SCALE,
+                        tau_target=DEFAULT_TAU_TARGET):
+    """The axes of one restricted space-time self-energy (GW/space_time.py:
+    559-618) as a dict:
+
+    * ntau: given, or the smallest size `minimax_points_for_gw` accepts;
+    * grid: chi0's split grid, its minimax time axis on [gap, span] and the
+      cosine transform onto the screening frequencies;
+    * freq_points, freq_weights: the screening frequencies (minimax over rW);
+    * tau_points: the self-energy's time axis, half the minimax points over rS;
+    * pade_freq: `npade` Gauss-Legendre frequencies of scale w0, where the
+      self-energy is returned;
+    * ranges: (rW, rS); mu: mid-gap."""
+    gap, span, mu = particle_hole_window(eps, nocc)
+    if ntau is None or (isinstance(ntau, str) and ntau.lower() == 'auto'):
+        ntau, _ = minimax_points_for_gw(eps, nocc, mu=mu, target=tau_target)
+    ntau = int(ntau)
+    freq_points, freq_weights = screening_frequency_grid(ntau, eps, nocc, mu=mu)
+    ranges = sigma_fit_ranges(eps, nocc, mu=mu)
+    return {'ntau': ntau, 'mu': mu,
+            'grid': TimeFrequencyGrid.minimax_split(ntau, gap, span, freq_points,
+                                                    freq_weights),
+            'freq_points': freq_points, 'freq_weights': freq_weights,
+            'tau_points': 0.5 * minimax_time_grid(ntau, *ranges[1])[0],
+            'pade_freq': gauss_legendre_grid(int(npade), w0)[0],
+            'ranges': ranges}
 
 
 def rpa_energy_grid(eps, nocc, ntau, nfreq=40, w0_scale=0.5):

@@ -109,6 +109,43 @@ class Chi0Builder:
                    'mbpt_chi0_profile_read')
         return {'ms': float(ms.value), 'launches': int(n.value), 'flops': float(fl.value)}
 
+    def sigma_sums(self, tau, Zt, states):
+        """(2, ntau, nstates) FP64 device: the space-time self-energy's branch
+        sums (`mbpt_chi0_sigma`), + sum X_s (Zt o Go) X_s and
+        - sum X_s (Zt o Gv) X_s per time point, Zt (ntau, M, M) FP32 device."""
+        if self.precision != 'fp32':
+            raise NotImplementedError('the self-energy sums run on FP32 handles')
+        tau = np.atleast_1d(np.asarray(tau, float))
+        states = np.atleast_1d(np.asarray(states, dtype=np.int32))
+        if Zt.dtype != torch.float32 or tuple(Zt.shape) != (len(tau), self.M, self.M):
+            raise ValueError(f'Zt must be FP32 ({len(tau)}, {self.M}, {self.M})')
+        if not Zt.is_contiguous():
+            raise ValueError('Zt must be contiguous')
+        out = torch.empty((2, len(tau), len(states)), dtype=torch.float64,
+                          device=self.device)
+        for t0 in range(0, len(tau), _lib.CHI0_MAX_TAU):
+            t1 = min(len(tau), t0 + _lib.CHI0_MAX_TAU)
+            tt, tp = _lib.host_doubles(tau[t0:t1])
+            part = out if (t0 == 0 and t1 == len(tau)) else torch.empty(
+                (2, t1 - t0, len(states)), dtype=torch.float64, device=self.device)
+            st = np.ascontiguousarray(states)
+            _lib.check(self._lib.mbpt_chi0_sigma(
+                self._h, t1 - t0, tp, _lib.dptr(Zt[t0:t1]), len(st),
+                st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _lib.dptr(part),
+                _lib.stream_ptr()), 'mbpt_chi0_sigma')
+            if part is not out:
+                out[:, t0:t1] = part
+        return out
+
+    def sigma_profile_read(self):
+        """{'ms', 'launches', 'flops'} of the self-energy sums since `profile()`."""
+        ms, fl = ctypes.c_double(), ctypes.c_double()
+        n = ctypes.c_long()
+        _lib.check(self._lib.mbpt_chi0_sigma_profile_read(
+            self._h, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(fl)),
+            'mbpt_chi0_sigma_profile_read')
+        return {'ms': float(ms.value), 'launches': int(n.value), 'flops': float(fl.value)}
+
     def build(self, tau, weights, out=None, accumulate=False):
         """out (M, M) FP64 = [out] + sum_t weights[t] (-2) Go(t) o Gv(t)."""
         tau = np.atleast_1d(np.asarray(tau, float))

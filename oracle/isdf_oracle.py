@@ -319,6 +319,63 @@ def grid_screened_interaction(W_aux, D):
 
 
 # ---------------------------------------------------------------------------
+# the space-time GW self-energy (GW/space_time.py:789-849,
+# GW/imaginary_time.py:39-92, :181-195, :536-578)
+# ---------------------------------------------------------------------------
+
+def screened_interaction_tau(chi0_omega, Ctw):
+    """Wt(tau_t) = sum_k Ctw[t, k] ([I - chi0_k]^-1 - I), (ntau, naux, naux)
+    (imaginary_time.py:181-195 with the Dyson step of space_time.py:382-390)."""
+    eye = np.eye(chi0_omega.shape[-1])
+    Wc = np.stack([dyson_screening(c) - eye for c in chi0_omega])
+    return np.tensordot(np.asarray(Ctw, float), Wc, axes=(1, 0))
+
+
+def self_energy_branch_sums(X, D, eps, nocc, states, Wt, tau_points, mu):
+    """(Sigma^<_pp(tau), Sigma^>_pp(tau)), (2, ntau, nstates)
+    (imaginary_time.py:75-92, one block pair):
+
+        Sigma^<_pp(tau) = + sum_PQ X_s[P,p] (Zt o G^<)_PQ X_s[Q,p]
+        Sigma^>_pp(tau) = - sum_PQ X_s[P,p] (Zt o G^>)_PQ X_s[Q,p]
+
+    with Zt = D Wt(tau) D^T, G^< = X_o e^{e_o tau} X_o^T and
+    G^> = X_v e^{-e_v tau} X_v^T, e = eps - mu."""
+    X = np.asarray(X, float)
+    X_o, X_v, e_o, e_v, _ = split_branches(X, eps, nocc, mu)
+    X_s = X[:, np.atleast_1d(states)]
+    out = np.zeros((2, len(tau_points), X_s.shape[1]))
+    for k, tau in enumerate(np.asarray(tau_points, float)):
+        Zt = D @ Wt[k] @ D.T
+        Gl = (X_o * np.exp(e_o * tau)) @ X_o.T
+        Gg = (X_v * np.exp(-e_v * tau)) @ X_v.T
+        out[0, k] = np.einsum('Pp,Pp->p', X_s, (Zt * Gl) @ X_s)
+        out[1, k] = -np.einsum('Pp,Pp->p', X_s, (Zt * Gg) @ X_s)
+    return out
+
+
+def self_energy_diagonal(branch_sums, C, S):
+    """Sigma^c_pp(i omega_out) = -1/2 [(S^> + S^<)^T C^T + i (S^> - S^<)^T S^T],
+    (nstates, nfreq) (imaginary_time.py:557-561); C, S the cosine and sine
+    tau -> omega weights."""
+    sl, sg = branch_sums
+    return -0.5 * ((sg + sl).T @ np.asarray(C).T + 1j * ((sg - sl).T @ np.asarray(S).T))
+
+
+def gw_sigma_space_time(X, D, eps, nocc, states, grid, freq_points, tau_points,
+                        omega_out, ranges_weights, mu):
+    """Sigma^c_pp(i omega_out) of the serial space-time route
+    (space_time.py:789-849): chi0 on the screening frequencies, Dyson, the
+    cosine transform of W - I to the self-energy's time axis, the branch sums
+    and the cosine / sine transforms back.  ranges_weights = (Ctw, C, S), the
+    transforms built on the fit ranges (rW, rS)."""
+    Ctw, C, S = ranges_weights
+    chi = chi0_imaginary_frequency(X, D, eps, nocc, grid.tau_points, grid.cosft_wt, mu)
+    Wt = screened_interaction_tau(chi, Ctw)
+    sums = self_energy_branch_sums(X, D, eps, nocc, states, Wt, tau_points, mu)
+    return self_energy_diagonal(sums, C, S)
+
+
+# ---------------------------------------------------------------------------
 # closed forms
 # ---------------------------------------------------------------------------
 
@@ -355,6 +412,46 @@ def rpa_energy_casida(X, D, eps, nocc):
     H = (sq[:, None] * (np.diag(d) + 4.0 * K)) * sq[None, :]
     omega = np.sqrt(np.linalg.eigvalsh(H))
     return 0.5 * (omega.sum() - (d.sum() + 2.0 * np.trace(K)))
+
+
+def rpa_poles(X, D, eps, nocc):
+    """(Omega_S, rho_S): the RPA excitation energies and the auxiliary-basis
+    transition densities rho_S = CD (X + Y)_S, from (A - B) = d,
+    (A + B) = d + 4K: Omega^2 = eig(d^1/2 (d + 4K) d^1/2) with eigenvectors
+    F_S, (X + Y)_S = d^1/2 F_S / Omega_S^1/2.  Then, in the gauge where the
+    bare interaction is I, W - I = chi0 (I - chi0)^-1
+    = -4 sum_S Omega_S / (Omega_S^2 + w^2) rho_S rho_S^T."""
+    CD = _pair_densities(X, nocc, D)
+    K = CD.T @ CD
+    d = pair_diagonal(eps, nocc).ravel()
+    sq = np.sqrt(d)
+    H = (sq[:, None] * (np.diag(d) + 4.0 * K)) * sq[None, :]
+    w2, F = np.linalg.eigh(H)
+    om = np.sqrt(w2)
+    XpY = sq[:, None] * F / np.sqrt(om)[None, :]
+    return om, CD @ XpY
+
+
+def gw_sigma_sum_over_poles(X, D, eps, nocc, states, omega, mu):
+    """The exact GW@RPA correlation self-energy at z = mu + i omega:
+
+        Sigma^c_pp = sum_m sum_S 2 (B_pm . rho_S)^2 / (z - eps_m + s_m Omega_S),
+
+    s_m = +1 for occupied m, -1 for virtual m, B_pm = D^T (X_p o X_m): the
+    pole expansion of -1/(2 pi) int G(z + i w') W_c(i w') dw' with W_c from
+    `rpa_poles`.  (nstates, nfreq) complex."""
+    X = np.asarray(X, float)
+    eps = np.asarray(eps, float)
+    om, rho = rpa_poles(X, D, eps, nocc)
+    sgn = np.where(np.arange(len(eps)) < nocc, 1.0, -1.0)
+    z = mu + 1j * np.asarray(omega, float)
+    out = np.zeros((len(np.atleast_1d(states)), len(z)), complex)
+    for r, p in enumerate(np.atleast_1d(states)):
+        Bp = D.T @ (X[:, p:p + 1] * X)                     # (naux, nmo): B_pm
+        amp = 2.0 * (Bp.T @ rho) ** 2                       # (nmo, nS)
+        den = z[None, None, :] - eps[:, None, None] + (sgn[:, None] * om[None, :])[:, :, None]
+        out[r] = (amp[:, :, None] / den).sum((0, 1))
+    return out
 
 
 # ---------------------------------------------------------------------------
