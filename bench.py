@@ -58,6 +58,8 @@ def parse():
                    help='learned correction while the largest residual exceeds this x tol')
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     p.add_argument('--no-legs', action='store_true', help='headline only')
+    p.add_argument('--leg', choices=['benzene', 'large', 'gw', 'batched'],
+                   help='run this side leg alone and print its JSON (no headline line)')
     p.add_argument('--kernel-probe', action='store_true',
                    help='only a few sigma applies of the headline instance (k = 20, the '
                         'mixed precision) and one 256 MiB device copy, for counter runs')
@@ -411,6 +413,44 @@ def leg_full_action(inst, W, k=20, reps=5):
                          / solves['plain_fp64']['roots_per_s']}}
 
 
+def leg_gw():
+    """The space-time GW correlation self-energy Sigma^c_pp(i w) on the 16
+    Pade frequencies: stage times and the fused branch-sum kernel's FP32 MFMA
+    rate, for a 10-state gap window and for every state, at the large and the
+    learned-preconditioner instances."""
+    from mbptcode_amd import synth
+    from mbptcode_amd.gw import self_energy_diagonal_space_time
+    res = {}
+    for cfg in (2, 4):
+        inst = synth.config(cfg)
+        window = list(range(inst.nocc - 5, inst.nocc + 5))
+        for name, states in (('window10', window),
+                             ('all', list(range(inst.nocc + inst.nvir)))):
+            self_energy_diagonal_space_time(inst.X, inst.V, inst.eps, inst.nocc, states[:2])
+            sync()
+            tm = {}
+            t0 = time.perf_counter()
+            sig, ax = self_energy_diagonal_space_time(inst.X, inst.V, inst.eps, inst.nocc,
+                                                      states, timings=tm)
+            sync()
+            wall = time.perf_counter() - t0
+            k = tm['sigma_kernel']
+            tf = k['flops'] / (k['ms'] * 1e-3) / 1e12
+            key = f'cfg{cfg}_{name}'
+            res[key] = {'workload': f'nocc={inst.nocc}, nvir={inst.nvir}, M={inst.M}, '
+                                    f'{len(states)} states, ntau={ax["ntau"]}',
+                        'seconds': wall,
+                        'stages_s': {n: tm[n] for n in ('transforms', 'screening', 'to_tau',
+                                                        'sigma')},
+                        'sigma_kernel_ms': k['ms'], 'sigma_kernel_tflops': tf,
+                        'sigma_kernel_frac_fp32_peak': tf / PEAK_TFLOPS['fp32'],
+                        'max_abs_sigma_Ha': float(np.abs(sig).max())}
+            log(f'gw {key}: {wall * 1e3:.0f} ms (screening {tm["screening"] * 1e3:.0f} ms, '
+                f'sigma kernel {k["ms"]:.2f} ms, {tf:.1f} TF = '
+                f'{100 * tf / PEAK_TFLOPS["fp32"]:.0f}% FP32 peak), ntau {ax["ntau"]}')
+    return res
+
+
 def leg_batched():
     """The batched workload: 256 instances, 5 roots each, tol 1e-5."""
     import torch
@@ -498,6 +538,11 @@ def main():
     if args.kernel_probe:
         kernel_probe(args)
         return
+    if args.leg:
+        legs = {'benzene': leg_benzene, 'large': leg_large, 'gw': leg_gw,
+                'batched': leg_batched}
+        print(json.dumps({args.leg: legs[args.leg]()}), flush=True)
+        return
     inst, W, out, w_s = headline(args)
     lm = out[f'{args.precision}+learned']
     pl = out['plain_fp64']
@@ -527,6 +572,8 @@ def main():
         result['large_instance'] = leg_large()
         log('leg: full Casida action (B block)')
         result['full_casida_action'] = leg_full_action(inst, W)
+        log('leg: space-time GW self-energy')
+        result['gw_self_energy'] = leg_gw()
         log('leg: batched workload')
         result['batched'] = leg_batched()
     if not args.no_cpu:
