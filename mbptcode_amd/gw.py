@@ -49,36 +49,59 @@ def _solve_lower_blocked(C, B, block=1024):
     return out
 
 
-def screened_correlation_grid(L, chi0):
+def screened_correlation_grid(L, chi0, V=None, infos=None):
     """W - V on the grid for one chi0 (M, M) FP64: W = L A^-1 L^T with
-    A = I - L^T chi0 L = C C^T, as (L C^-T)(L C^-T)^T, then minus V = L L^T."""
+    A = I - L^T chi0 L = C C^T, as (L C^-T)(L C^-T)^T, then minus V = L L^T
+    (V itself when given, saving a product).  With `infos` (a list) the
+    factorization's status tensor is appended for the caller to check, so
+    that no host synchronisation happens here."""
     K = L.T @ (chi0 @ L)
     n = K.shape[0]
     A = torch.eye(n, dtype=K.dtype, device=K.device) - 0.5 * (K + K.T)
     C, info = torch.linalg.cholesky_ex(A)
-    if int(info) != 0:
-        raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
+    if infos is None:
+        if int(info) != 0:
+            raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
+    else:
+        infos.append(info)
     Y = _solve_lower_blocked(C, L.T.contiguous()).T            # L C^-T
-    Wc = Y @ Y.T - L @ L.T
+    Wc = Y @ Y.T - (L @ L.T if V is None else V)
     return 0.5 * (Wc + Wc.T)
 
 
-def screened_correlation_frequencies(builder, V, grid):
+def screened_correlation_frequencies(builder, V, grid, streams=4):
     """(W - V)(i w_k) on the grid for every frequency of `grid`, (nfreq, M, M)
     FP64: proj(tau) on grid's time axis once (an FP64 builder), its cosine
-    transform, Dyson."""
+    transform, Dyson.  The Dyson solves of different frequencies are
+    independent and run on `streams` side streams (default priority; four
+    match the device's hardware queues): the Cholesky factorizations are
+    latency-bound (small panel kernels), so several in flight fill the
+    device.  The factorizations' status is checked once, after all of them."""
     if builder.precision != 'fp64':
         raise ValueError('the screening frequencies need FP64 proj(tau): the '
                          'cosine weights onto them cancel over many digits')
     dev = builder.device
-    L = torch.linalg.cholesky(_dev(V, dev))
+    Vd = _dev(V, dev)
+    L = torch.linalg.cholesky(Vd)
     proj = torch.stack([builder.build([t], [1.0]) for t in grid.tau_points])
     C = torch.as_tensor(np.asarray(grid.cosft_wt, float), device=dev)
     chi = torch.tensordot(C, proj, dims=([1], [0]))
     del proj
     out = torch.empty_like(chi)
+    main = torch.cuda.current_stream(dev)
+    side = [torch.cuda.Stream(device=dev) for _ in range(max(1, min(streams, chi.shape[0])))]
+    for st in side:
+        st.wait_stream(main)
+        for t in (chi, out, L, Vd):
+            t.record_stream(st)
+    infos = []
     for k in range(chi.shape[0]):
-        out[k] = screened_correlation_grid(L, chi[k])
+        with torch.cuda.stream(side[k % len(side)]):
+            out[k] = screened_correlation_grid(L, chi[k], Vd, infos)
+    for st in side:
+        main.wait_stream(st)
+    if int(torch.stack(infos).abs().sum()) != 0:
+        raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
     return out
 
 
