@@ -296,8 +296,8 @@ int sigma_f32(mbpt_chi0 *h, int ntau, const double *tau, const float *Zt, int ns
   k_gw_sigma_f32<<<nblk, 512, 0, st>>>(M, Mp, ko, kv, nt, ntau, h->Yt, Zt, h->gw_xs, nsp,
                                        h->gw_part, nblk);
   if ((rc = launch_status("k_gw_sigma_f32"))) return rc;
-  k_gw_sigma_reduce<<<(2 * ntau * ns + 255) / 256, 256, 0, st>>>(ntau, ntiles, ns, nsp,
-                                                                 h->gw_part, out);
+  k_gw_sigma_reduce<<<2 * ntau * (nsp / 16), 256, 0, st>>>(ntau, ntiles, ns, nsp, h->gw_part,
+                                                            out);
   if ((rc = launch_status("k_gw_sigma_reduce"))) return rc;
   if (h->prof) {
     MBPT_HIP_TRY(hipEventRecord(h->e1, st));

@@ -128,15 +128,29 @@ __global__ __launch_bounds__(512) void k_gw_sigma_f32(int M, int Mp, int ko, int
   }
 }
 
-// out[(b * ntau + t) * ns + s] = (b ? -1 : 1) sum_tiles part[...], tiles in order
-__global__ void k_gw_sigma_reduce(int ntau, int ntiles, int ns, int nsp,
-                                  const float *__restrict__ part, double *__restrict__ out) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= 2 * ntau * ns) return;
-  const int s = idx % ns, t = (idx / ns) % ntau, b = idx / (ns * ntau);
+// out[(b * ntau + t) * ns + s] = (b ? -1 : 1) sum_tiles part[...]: one
+// workgroup per (branch, time, 16-state group); thread (q, s) sums the tiles
+// q, q + 16, ... of state s (16 consecutive states: one 64-byte read per
+// tile), then the 16 partial sums of a state meet in a fixed-order tree.
+__global__ __launch_bounds__(256) void k_gw_sigma_reduce(int ntau, int ntiles, int ns, int nsp,
+                                                         const float *__restrict__ part,
+                                                         double *__restrict__ out) {
+  const int ngrp = nsp / 16;
+  const int g = blockIdx.x % ngrp, t = (blockIdx.x / ngrp) % ntau, b = blockIdx.x / (ngrp * ntau);
+  const int s = threadIdx.x & 15, q = threadIdx.x >> 4;
   double v = 0.0;
-  for (int tl = 0; tl < ntiles; ++tl) v += part[(((long)t * ntiles + tl) * 2 + b) * nsp + s];
-  out[idx] = b ? -v : v;
+  for (int tl = q; tl < ntiles; tl += 16)
+    v += part[(((long)t * ntiles + tl) * 2 + b) * nsp + 16 * g + s];
+  __shared__ double red[16][17];
+  red[q][s] = v;
+  __syncthreads();
+  if (q == 0) {
+    double tot = 0.0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) tot += red[i][s];
+    const int st = 16 * g + s;
+    if (st < ns) out[((long)b * ntau + t) * ns + st] = b ? -tot : tot;
+  }
 }
 
 // Xs[mu][j] (Mp x nsp): collocation column cols[j] of the K-major XT, zero
