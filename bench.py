@@ -58,7 +58,7 @@ def parse():
                    help='learned correction while the largest residual exceeds this x tol')
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     p.add_argument('--no-legs', action='store_true', help='headline only')
-    p.add_argument('--leg', choices=['benzene', 'large', 'gw', 'batched'],
+    p.add_argument('--leg', choices=['benzene', 'large', 'gw', 'fit', 'batched'],
                    help='run this side leg alone and print its JSON (no headline line)')
     p.add_argument('--kernel-probe', action='store_true',
                    help='only a few sigma applies of the headline instance (k = 20, the '
@@ -451,6 +451,38 @@ def leg_gw():
     return res
 
 
+def leg_fit():
+    """The separable fit at the learned-preconditioner instance's size: 6000
+    points, a 60000-column test set, 2000 auxiliary functions (synthetic D and
+    F: the integrals behind them need pyscf).  FP64 throughout."""
+    import torch
+    from mbptcode_amd import isdf_fit as F
+    nk, nrho, naux = 6000, 60000, 2000
+    g = torch.Generator(device='cuda').manual_seed(0)
+    D = torch.randn((nk, nrho), generator=g, dtype=torch.float64, device='cuda')
+    Ff = torch.randn((naux, nrho), generator=g, dtype=torch.float64, device='cuda')
+    B = torch.randn((naux, naux), generator=g, dtype=torch.float64, device='cuda')
+    V = B @ B.T / naux + 0.05 * torch.eye(naux, dtype=torch.float64, device='cuda')
+    F.fit_M_stable(D[:500], Ff)
+    sync()
+    t0 = time.perf_counter()
+    M = F.fit_M_stable(D, Ff)
+    sync()
+    t_fit = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    Dsep = F.separable_factor(M, V)
+    sync()
+    t_gauge = time.perf_counter() - t0
+    flops = 2.0 * nk * nk * nrho + 2.0 * naux * nk * nrho
+    res = {'workload': f'nk={nk}, nrho={nrho}, naux={naux}, FP64',
+           'fit_s': t_fit, 'gauge_s': t_gauge,
+           'fit_gemm_tflops': flops / t_fit / 1e12,
+           'factor_shape': list(Dsep.shape)}
+    log(f'fit: {t_fit * 1e3:.0f} ms ({res["fit_gemm_tflops"]:.1f} TF FP64 over its two '
+        f'products), gauge {t_gauge * 1e3:.0f} ms')
+    return res
+
+
 def leg_batched():
     """The batched workload: 256 instances, 5 roots each, tol 1e-5."""
     import torch
@@ -540,7 +572,7 @@ def main():
         return
     if args.leg:
         legs = {'benzene': leg_benzene, 'large': leg_large, 'gw': leg_gw,
-                'batched': leg_batched}
+                'fit': leg_fit, 'batched': leg_batched}
         print(json.dumps({args.leg: legs[args.leg]()}), flush=True)
         return
     inst, W, out, w_s = headline(args)
@@ -574,6 +606,8 @@ def main():
         result['full_casida_action'] = leg_full_action(inst, W)
         log('leg: space-time GW self-energy')
         result['gw_self_energy'] = leg_gw()
+        log('leg: separable fit')
+        result['isdf_fit'] = leg_fit()
         log('leg: batched workload')
         result['batched'] = leg_batched()
     if not args.no_cpu:

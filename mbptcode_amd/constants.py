@@ -39,6 +39,11 @@ DAVIDSON_GUESS_FACTOR = 2               # :1886
 DAVIDSON_GUESS_DEG_TOL = 1e-3           # :2190
 DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
 
+# the ISDF fit (constants.py:36, :1755): the metric root's relative eigenvalue
+# floor and the Tikhonov floor of the balanced Gram matrix
+AUX_METRIC_ROOT_FLOOR = 1e-12
+ISDF_FIT_REGULARIZATION = 4e-7
+
 # the (A - B) probe (constants.py:253-254, :1234)
 AMB_PROBE_TOL = 1e-6                    # residual bar of the converged probe, Ha
 AMB_PROBE_SIGN_TOLS = (1e-2, 1e-3, 1e-4)  # relative bars of the sign passes

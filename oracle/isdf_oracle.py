@@ -10,17 +10,23 @@ Two kinds of functions live here:
 
 1. Restatements of the reference's own algorithm, step for step, in FP64
    (file:line given per function):
-   ``isdf_block_action`` (serial, Tamm-Dancoff branch), ``split_branches``,
-   ``polarizability_projected_tau``, ``static_screening_matrix``,
-   ``chi0_imaginary_frequency``, ``rpa_correlation_energy_space_time``,
-   ``davidson_tda`` (the Jacobi-Davidson TDA iteration).
+   ``isdf_block_action`` (serial; the Tamm-Dancoff action and the B block),
+   ``split_branches``, ``polarizability_projected_tau``,
+   ``static_screening_matrix``, ``chi0_imaginary_frequency``,
+   ``rpa_correlation_energy_space_time``, ``davidson_tda`` (the
+   Jacobi-Davidson TDA iteration), the space-time GW self-energy
+   (``screened_interaction_tau``, ``self_energy_branch_sums``,
+   ``self_energy_diagonal``, ``gw_sigma_space_time``) and the separable fit
+   (``balanced_gram``, ``fit_M``, ``fit_M_stable``, ``aux_metric_sqrt``).
 2. Closed forms that pin them independently of the restatement:
    ``dense_tda_matrix`` (A from the ISDF factors, O(n_pair^2 M)),
    ``dense_casida_b_matrix`` (B from the same factors),
    ``dense_casida_roots`` (the full Casida roots of dense A, B),
    ``proj_tau_bruteforce`` (the pair sum of chi0(i tau)),
    ``chi0_omega_closed_form`` (the exact frequency-axis chi0),
-   ``rpa_energy_casida`` (Furche's trace formula).
+   ``rpa_energy_casida`` (Furche's trace formula),
+   ``rpa_poles`` and ``gw_sigma_sum_over_poles`` (the exact GW@RPA
+   self-energy by its pole expansion).
 
 Pinning status (what checks each function; see tests/test_oracle.py):
 
@@ -36,6 +42,13 @@ Pinning status (what checks each function; see tests/test_oracle.py):
 * davidson_tda            -- dense eigh of the same A
 * the cosine-transform weights' exact bits -- parity unpinned (only an import
   of the reference could pin them; pinned here to the fit's model error)
+* gw_sigma_space_time and its parts -- the exact GW@RPA pole sum
+  (gw_sigma_sum_over_poles, its W - I expansion against the matrix inverse),
+  planted slips, a brute-force branch sum (tests/test_gw_oracle.py)
+* balanced_gram, fit_M, fit_M_stable, aux_metric_sqrt -- recovery of a
+  planted interpolation matrix, invariance under row scaling, the two
+  estimators against each other, root^2 = V and the dropped null space
+  (tests/test_isdf_fit.py)
 """
 import numpy as np
 
@@ -373,6 +386,46 @@ def gw_sigma_space_time(X, D, eps, nocc, states, grid, freq_points, tau_points,
     Wt = screened_interaction_tau(chi, Ctw)
     sums = self_energy_branch_sums(X, D, eps, nocc, states, Wt, tau_points, mu)
     return self_energy_diagonal(sums, C, S)
+
+
+# ---------------------------------------------------------------------------
+# the separable fit (Base/factorization/separable_ri.py:1616-1653, :1802-1930)
+# ---------------------------------------------------------------------------
+
+def balanced_gram(D, regularization=4e-7):
+    """(Dt, d, G): rows of D at unit length, their Gram matrix plus the
+    Tikhonov floor (separable_ri.py:1616-1627)."""
+    D = np.asarray(D, float)
+    scale = np.sqrt(np.einsum('kr,kr->k', D, D))
+    scale[scale == 0] = 1.0
+    d = 1.0 / scale
+    Dt = D * d[:, None]
+    G = Dt @ Dt.T
+    G[np.diag_indices_from(G)] += regularization
+    return Dt, d, G
+
+
+def fit_M(D, F, regularization=4e-7):
+    """((F Dt^T) G^-1) diag(d) (separable_ri.py:1630-1637)."""
+    Dt, d, G = balanced_gram(D, regularization)
+    return ((np.asarray(F, float) @ Dt.T) @ np.linalg.inv(G)) * d[None, :]
+
+
+def fit_M_stable(D, F, regularization=4e-7):
+    """The same through a Cholesky solve (separable_ri.py:1640-1653)."""
+    Dt, d, G = balanced_gram(D, regularization)
+    L = np.linalg.cholesky(G)
+    R = (np.asarray(F, float) @ Dt.T).T
+    X = np.linalg.solve(L.T, np.linalg.solve(L, R))
+    return X.T * d[None, :]
+
+
+def aux_metric_sqrt(V, floor=1e-12):
+    """V^1/2 with eigenvalues below floor x max dropped
+    (separable_ri.py:1802-1836, :1915-1929)."""
+    w, v = np.linalg.eigh(np.asarray(V, float))
+    keep = w > floor * w.max()
+    return (v[:, keep] * np.sqrt(w[keep])) @ v[:, keep].T
 
 
 # ---------------------------------------------------------------------------

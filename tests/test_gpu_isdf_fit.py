@@ -1,0 +1,44 @@
+"""The separable fit on the device (FP64 GEMMs, Cholesky and eigh through
+torch on ROCm) against the FP64 oracle (separable_ri.py:1616-1712,
+:1802-1930): relative 1e-10 on M, the metric root and the factor, and
+Z = D D^T = M^T V M."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import isdf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(got, ref):
+    got = got.cpu().numpy() if isinstance(got, torch.Tensor) else got
+    return float(np.abs(got - ref).max() / np.abs(ref).max())
+
+
+@pytest.mark.parametrize('nk, nrho, naux', [(50, 300, 20), (700, 4000, 300)])
+def test_fit_against_oracle(nk, nrho, naux):
+    from mbptcode_amd import isdf_fit as F
+    rng = np.random.default_rng(nk)
+    D = rng.normal(size=(nk, nrho)) * rng.uniform(0.5, 2.0, size=(nk, 1))
+    Ff = rng.normal(size=(naux, nrho))
+    print(f'isdf fit {nk} {nrho} {naux}', flush=True)
+    assert rel(F.fit_M_stable(D, Ff), O.fit_M_stable(D, Ff)) < 1e-10
+    assert rel(F.fit_M(D, Ff), O.fit_M(D, Ff)) < 1e-9
+    B = rng.normal(size=(naux, naux))
+    V = B @ B.T / naux + 0.05 * np.eye(naux)
+    assert rel(F.aux_metric_sqrt(V), O.aux_metric_sqrt(V)) < 1e-10
+    M = O.fit_M_stable(D, Ff)
+    Dsep = F.separable_factor(M, V)
+    Z = F.screened_points(M, V)
+    assert rel(Dsep, M.T @ O.aux_metric_sqrt(V)) < 1e-10
+    assert rel(Dsep @ Dsep.T, Z.cpu().numpy()) < 1e-10
+
+
+def test_planted_matrix_on_the_device():
+    from mbptcode_amd import isdf_fit as F
+    rng = np.random.default_rng(7)
+    D = rng.normal(size=(120, 500))
+    M = rng.normal(size=(40, 120))
+    got = F.fit_M_stable(D, M @ D, regularization=0.0)
+    assert rel(got, M) < 1e-9
