@@ -69,14 +69,16 @@ def screened_correlation_grid(L, chi0, V=None, infos=None):
     return 0.5 * (Wc + Wc.T)
 
 
-def screened_correlation_frequencies(builder, V, grid, streams=4):
+def screened_correlation_frequencies(builder, V, grid):
     """(W - V)(i w_k) on the grid for every frequency of `grid`, (nfreq, M, M)
     FP64: proj(tau) on grid's time axis once (an FP64 builder), its cosine
-    transform, Dyson.  The Dyson solves of different frequencies are
-    independent and run on `streams` side streams (default priority; four
-    match the device's hardware queues): the Cholesky factorizations are
-    latency-bound (small panel kernels), so several in flight fill the
-    device.  The factorizations' status is checked once, after all of them."""
+    transform, Dyson.  Everything stays on the caller's stream and the
+    factorizations' status is checked once, after all of them (a host
+    synchronisation per frequency would leave the device idle between the
+    latency-bound Cholesky factorizations).  The solves are not run
+    concurrently: on several streams, from one host thread or from one
+    thread per stream, torch's rocBLAS / rocSOLVER calls corrupted some
+    frequencies' results (1e-4 relative, at random)."""
     if builder.precision != 'fp64':
         raise ValueError('the screening frequencies need FP64 proj(tau): the '
                          'cosine weights onto them cancel over many digits')
@@ -88,18 +90,9 @@ def screened_correlation_frequencies(builder, V, grid, streams=4):
     chi = torch.tensordot(C, proj, dims=([1], [0]))
     del proj
     out = torch.empty_like(chi)
-    main = torch.cuda.current_stream(dev)
-    side = [torch.cuda.Stream(device=dev) for _ in range(max(1, min(streams, chi.shape[0])))]
-    for st in side:
-        st.wait_stream(main)
-        for t in (chi, out, L, Vd):
-            t.record_stream(st)
     infos = []
     for k in range(chi.shape[0]):
-        with torch.cuda.stream(side[k % len(side)]):
-            out[k] = screened_correlation_grid(L, chi[k], Vd, infos)
-    for st in side:
-        main.wait_stream(st)
+        out[k] = screened_correlation_grid(L, chi[k], Vd, infos)
     if int(torch.stack(infos).abs().sum()) != 0:
         raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
     return out

@@ -359,17 +359,17 @@ def rpa_correlation_energy_space_time(X, D, eps, nocc, grid, mu=None,
 
 
 def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32',
-                                flags=None, streams=4):
+                                flags=None):
     """E_c with the interaction on the grid: log det(I - L^T chi0 L) with
     L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); one fused chi0 launch per
     frequency, FP64 GEMMs and Cholesky log-dets.
 
-    The chi0 builds run on the caller's stream into a ring of `streams`
-    buffers; each frequency's projection and Cholesky run on a side stream of
-    default priority (the factorizations are latency-bound, so several in
-    flight fill the device), and the log-dets stay on the device until the
-    end.  A frequency whose I - L^T chi0 L is not positive definite is redone
-    through `dielectric_log_determinant` (slogdet, warning, flags)."""
+    The log-dets and traces stay on the device until the end (no host
+    synchronisation per frequency); a frequency whose I - L^T chi0 L is not
+    positive definite is redone through `dielectric_log_determinant`
+    (slogdet, warning, flags).  One stream: concurrent factorizations through
+    torch's rocBLAS / rocSOLVER (several streams, one or several host
+    threads) corrupted results at random."""
     dev = _device()
     b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
     Vd = _dev(V, dev)
@@ -377,39 +377,22 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     n = L.shape[0]
     nf = grid.nfreq
     eye = torch.eye(n, dtype=torch.float64, device=dev)
-    main = torch.cuda.current_stream(dev)
-    side = [torch.cuda.Stream(device=dev) for _ in range(max(1, min(streams, nf)))]
-    bufs = [torch.empty((n, n), dtype=torch.float64, device=dev) for _ in side]
+    chi = torch.empty((n, n), dtype=torch.float64, device=dev)
     logdet = torch.zeros(nf, dtype=torch.float64, device=dev)
     trace = torch.zeros(nf, dtype=torch.float64, device=dev)
     info = torch.zeros(nf, dtype=torch.int32, device=dev)
-    for st, buf in zip(side, bufs):
-        for t in (L, eye, buf, logdet, trace, info):
-            t.record_stream(st)
-    free = [None] * len(side)
     for k in range(nf):
-        r = k % len(side)
-        if free[r] is not None:
-            main.wait_event(free[r])
-        b.build(grid.tau_points, grid.cosft_wt[k], out=bufs[r])
-        built = torch.cuda.Event()
-        built.record(main)
-        side[r].wait_event(built)
-        with torch.cuda.stream(side[r]):
-            K = L.T @ (bufs[r] @ L)
-            A = eye - K
-            C, inf = torch.linalg.cholesky_ex(0.5 * (A + A.T))
-            logdet[k] = 2.0 * torch.log(torch.diagonal(C)).sum()
-            trace[k] = torch.trace(K)
-            info[k] = inf
-            free[r] = torch.cuda.Event()
-            free[r].record(side[r])
-    for st in side:
-        main.wait_stream(st)
+        b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
+        K = L.T @ (chi @ L)
+        A = eye - K
+        C, inf = torch.linalg.cholesky_ex(0.5 * (A + A.T))
+        logdet[k] = 2.0 * torch.log(torch.diagonal(C)).sum()
+        trace[k] = torch.trace(K)
+        info[k] = inf
     logdet_h, trace_h = logdet.cpu().numpy(), trace.cpu().numpy()
     mine = {}
     for k in np.flatnonzero(info.cpu().numpy()):
-        chi = b.build(grid.tau_points, grid.cosft_wt[k])
+        b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
         K = L.T @ (chi @ L)
         logdet_h[k] = dielectric_log_determinant(eye - K, int(k), mine)
     e_c = float(np.dot(np.asarray(grid.omega_weights, float), logdet_h + trace_h))
