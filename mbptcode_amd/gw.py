@@ -26,9 +26,9 @@ Everything is in the grid gauge (the auxiliary basis is the interpolation grid
 and V = L L^T), so the screened interaction is formed on the grid once per
 frequency and no per-time projection is needed.  The frequency axes, transform
 weights and fit ranges are `grids.space_time_gw_grids`'.  Sigma^c(i w) is
-returned on the Pade frequencies; the Pade continuation and the quasiparticle
-equation need the mean field's exchange and xc terms (`finish_qp`, pyscf) and
-are not part of this module.
+returned on the Pade frequencies; `quasiparticle_energies_space_time` adds
+the continuation and the quasiparticle equation (`qp`) with the static term
+<Sigma_x - v_xc> from the caller (its build needs the mean field: pyscf).
 """
 import time
 
@@ -171,6 +171,22 @@ def self_energy_diagonal_space_time(X, V, eps, nocc, states, ntau=None,
         b.profile(False)
         timings.update(clock)
     return diagonal_from_branches(sums, C, S), axes
+
+
+def quasiparticle_energies_space_time(X, V, eps, nocc, states, xc_diagonal, ntau=None,
+                                      npade=grids.DEFAULT_NPADE,
+                                      w0=grids.GL_FREQUENCY_SCALE, with_z=False,
+                                      timings=None):
+    """G0W0 quasiparticle energies of `states` by the space-time route with the
+    caller's <Sigma_x - v_xc>_pp (the serial core of
+    `solve_qp_energy_space_time`, GW/space_time.py:631-849, with
+    `finish_qp(xc_diagonal=...)`): Sigma^c(i w) on the Pade frequencies on the
+    device, then the continuation and the largest-weight root (`qp`)."""
+    from .qp import quasiparticle_energies
+    sig, axes = self_energy_diagonal_space_time(X, V, eps, nocc, states, ntau=ntau,
+                                                npade=npade, w0=w0, timings=timings)
+    return quasiparticle_energies(sig, eps, nocc, states, axes['mu'], axes['pade_freq'],
+                                  xc_diagonal, with_z=with_z)
 
 
 def self_energy_diagonal_imaginary_time(X_mo, D, W_omega, eps, nocc, states, tau_points,

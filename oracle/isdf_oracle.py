@@ -485,19 +485,20 @@ def rpa_poles(X, D, eps, nocc):
     return om, CD @ XpY
 
 
-def gw_sigma_sum_over_poles(X, D, eps, nocc, states, omega, mu):
+def gw_sigma_sum_over_poles(X, D, eps, nocc, states, omega, mu, z=None):
     """The exact GW@RPA correlation self-energy at z = mu + i omega:
 
         Sigma^c_pp = sum_m sum_S 2 (B_pm . rho_S)^2 / (z - eps_m + s_m Omega_S),
 
     s_m = +1 for occupied m, -1 for virtual m, B_pm = D^T (X_p o X_m): the
     pole expansion of -1/(2 pi) int G(z + i w') W_c(i w') dw' with W_c from
-    `rpa_poles`.  (nstates, nfreq) complex."""
+    `rpa_poles`.  (nstates, nfreq) complex; `z` (complex or real points)
+    replaces mu + i omega when given (the real axis for the QP equation)."""
     X = np.asarray(X, float)
     eps = np.asarray(eps, float)
     om, rho = rpa_poles(X, D, eps, nocc)
     sgn = np.where(np.arange(len(eps)) < nocc, 1.0, -1.0)
-    z = mu + 1j * np.asarray(omega, float)
+    z = mu + 1j * np.asarray(omega, float) if z is None else np.atleast_1d(np.asarray(z, complex))
     out = np.zeros((len(np.atleast_1d(states)), len(z)), complex)
     for r, p in enumerate(np.atleast_1d(states)):
         Bp = D.T @ (X[:, p:p + 1] * X)                     # (naux, nmo): B_pm

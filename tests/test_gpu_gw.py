@@ -126,3 +126,21 @@ def test_refusals():
         b.sigma_sums([0.1], Z.double(), [0])
     with pytest.raises(ValueError):
         b.sigma_sums([0.1, 0.2], Z, [0])
+
+
+def test_quasiparticle_energies_on_the_device_chain():
+    """G0W0 energies from the device Sigma^c(i w) against the exact GW@RPA
+    roots (water size) within 1 meV, with the quasiparticle weights."""
+    from mbptcode_amd.gw import quasiparticle_energies_space_time
+    from test_gw_oracle import HA_MEV, exact_qp
+    inst = synth.config(0)
+    D = O.grid_factors(inst.V)
+    no = inst.nocc
+    states = [no - 2, no - 1, no, no + 1]
+    xc = np.array([-0.05, -0.05, 0.02, 0.02])
+    print('gw qp energies cfg0', flush=True)
+    e, z = quasiparticle_energies_space_time(inst.X, inst.V, inst.eps, no, states, xc,
+                                             with_z=True)
+    exact = exact_qp(inst, D, states, xc)
+    assert np.abs(e - exact[:, 0]).max() * HA_MEV < 1.0
+    assert np.abs(z - exact[:, 1]).max() < 1e-3
