@@ -58,7 +58,7 @@ def parse():
                    help='learned correction while the largest residual exceeds this x tol')
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     p.add_argument('--no-legs', action='store_true', help='headline only')
-    p.add_argument('--leg', choices=['benzene', 'large', 'gw', 'fit', 'batched'],
+    p.add_argument('--leg', choices=['benzene', 'large', 'casida', 'gw', 'fit', 'batched'],
                    help='run this side leg alone and print its JSON (no headline line)')
     p.add_argument('--kernel-probe', action='store_true',
                    help='only a few sigma applies of the headline instance (k = 20, the '
@@ -573,6 +573,10 @@ def main():
     if args.leg:
         legs = {'benzene': leg_benzene, 'large': leg_large, 'gw': leg_gw,
                 'fit': leg_fit, 'batched': leg_batched}
+        if args.leg == 'casida':
+            inst, W, _ = build_problem(4)
+            print(json.dumps({'casida': leg_full_action(inst, W)}), flush=True)
+            return
         print(json.dumps({args.leg: legs[args.leg]()}), flush=True)
         return
     inst, W, out, w_s = headline(args)

@@ -318,4 +318,231 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   }
 }
 
+// ---- the B block's exchange (full Casida) ---------------------------------------
+// U^T[b][i][mu] = sum_nu X_o[nu][i] W[nu][mu] P^T[nu][mu],
+// P^T[nu][mu] = sum_j zXv[b][j][nu] X_o[mu][j]   (W symmetric: W o P^T = (W o P)^T).
+// The exchange above with the operands of the first product swapped: the
+// per-vector zXv rows nu are the streamed A operand, packed per apply in the
+// fragment order of X_o's P part (k_pack_bf16_zxv), and a wave's own columns
+// mu of X_o are the B operand, split once into registers.  The second
+// product is the exchange's own (X_o^T fragments, the same LDS stream).  One
+// vector and NC 16-column tiles per wave, so every zXv and X_o^T fragment
+// read from LDS serves NC tiles.
+
+// zfrag[(b * nchunk + c) * spp + slot]: the P-part fragments of zXv[b] with
+// rows nu = 32 c + 16 tt + l%16 and the K index j laid out as X_o's (hi / lo
+// pairs, the last block's split terms side by side)
+__global__ void k_pack_bf16_zxv(int M, int no, int k, const float *__restrict__ zXv,
+                                bf16x8 *__restrict__ zfrag) {
+  const Bf16Shape sh = bf16_shape(no);
+  const int j0 = 32 * (sh.nkb - 1), r = no - j0;
+  const int nch = (M + 31) / 32;
+  const long n = (long)k * nch * sh.spp;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const int b = (int)(e / ((long)nch * sh.spp));
+    const int rem2 = (int)(e % ((long)nch * sh.spp));
+    const int chunk = rem2 / sh.spp, rem = rem2 % sh.spp;
+    const int lane = rem & 63, q = lane >> 4, g = rem >> 6;
+    const int tt = g / sh.fp, f = g % sh.fp;
+    const int row = 32 * chunk + 16 * tt + (lane & 15);
+    const float *zb = zXv + (long)b * no * M;
+    bf16x8 out;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      int j, lo;
+      if (f < 2 * (sh.nkb - 1) || sh.tail == 3) {
+        j = 32 * (f >> 1) + 8 * q + s;
+        lo = f & 1;
+      } else {
+        const int kap = 32 * (f - 2 * (sh.nkb - 1)) + 8 * q + s;
+        const int term = kap < r ? 0 : (kap < 2 * r ? 1 : (kap < 3 * r ? 2 : 3));
+        j = term == 3 ? no : j0 + kap - term * r;
+        lo = term == 1;
+      }
+      const float x = (row < M && j < no) ? zb[(long)j * M + row] : 0.f;
+      const __bf16 h = (__bf16)x;
+      out[s] = lo ? (__bf16)(x - (float)h) : h;
+    }
+    zfrag[e] = out;
+  }
+}
+
+template <int NP, int NIE, int TAIL, int NC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_exchange_b_bf16(
+    int M, int no, int k, const bf16x8 *__restrict__ frag, const bf16x8 *__restrict__ zfrag,
+    const float *__restrict__ Xo, const float *__restrict__ W, float *__restrict__ Upart,
+    int nsplit, int rows_per_split, int nvb, int nblk) {
+  constexpr int NKB = NP / 32, NFB = TAIL == 3 ? NKB : NKB - 1;
+  constexpr int TP = TAIL == 3 ? 2 : TAIL, FP = 2 * (NKB - 1) + TP;
+  constexpr int SPP = 128 * FP, SPC = SPP + 128 * NIE, COLS = 64 * NC;
+  __shared__ bf16x8 buf[2][SPC];   // zXv[b]'s P part, then X_o's T part
+  const int nch = (M + 31) / 32;
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  if (lid >= nvb * k * nsplit) return;
+  const int b = lid % k, t = lid / k, vb = t % nvb, s = t / nvb;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lq = lane >> 4;
+  int colv[NC];
+  unsigned woff[NC];
+#pragma unroll
+  for (int ct = 0; ct < NC; ++ct) {
+    colv[ct] = vb * COLS + wave * 16 * NC + 16 * ct + lr;
+    woff[ct] = (unsigned)(4 * lq * M + min(colv[ct], M - 1));
+  }
+
+  // X_o[mu][:] of this lane's column as the B operands of the P^T products
+  bf16x8 bxh[NC][NFB > 0 ? NFB : 1], bxl[NC][NFB > 0 ? NFB : 1], bxt[NC][TAIL == 3 ? 1 : TAIL];
+  const int j0 = 32 * (NKB - 1), r = no - j0;
+#pragma unroll
+  for (int ct = 0; ct < NC; ++ct) {
+    const int col = colv[ct];
+    const bool ok = col < M;
+    const float *xr = Xo + (long)min(col, M - 1) * no;
+#pragma unroll
+    for (int kb = 0; kb < NFB; ++kb) {
+      float x[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = 32 * kb + 8 * lq + q;
+        x[q] = (ok && j < no) ? xr[j] : 0.f;
+      }
+      split8(x, bxh[ct][kb], bxl[ct][kb]);
+    }
+    if (TAIL < 3) {
+#pragma unroll
+      for (int tf = 0; tf < (TAIL == 3 ? 1 : TAIL); ++tf)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int kap = 32 * tf + 8 * lq + q;
+          const int term = kap < r ? 0 : (kap < 2 * r ? 1 : (kap < 3 * r ? 2 : 3));
+          const int j = j0 + kap - term * r;
+          const float x = (ok && term < 3) ? xr[j] : 0.f;
+          const __bf16 h = (__bf16)x;
+          bxt[ct][tf][q] = term == 2 ? (__bf16)(x - (float)h) : (term == 3 ? (__bf16)0.f : h);
+        }
+    }
+  }
+  f32x4 acc[NIE][NC];
+#pragma unroll
+  for (int it = 0; it < NIE; ++it)
+#pragma unroll
+    for (int u = 0; u < NC; ++u) zero(acc[it][u]);
+
+  const int c_beg = (s * rows_per_split) / 32;
+  const int c_end = min(nch, ((s + 1) * rows_per_split) / 32);
+  auto load_w = [&](int c, float (&w)[NC][2][4]) {
+    if (32 * c + 32 <= M) {
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const float *base = W + (long)(32 * c + 16 * tt + rr) * M;
+#pragma unroll
+          for (int ct = 0; ct < NC; ++ct) w[ct][tt][rr] = base[woff[ct]];
+        }
+    } else {
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct)
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr)
+            w[ct][tt][rr] = W[(long)min(32 * c + 16 * tt + 4 * lq + rr, M - 1) * M +
+                              min(colv[ct], M - 1)];
+    }
+  };
+  // stage c: 64-slot groups, each wholly in zXv[b]'s P part or X_o's T part
+  auto fetch = [&](int c, int n) {
+    const bf16x8 *zsrc = zfrag + ((long)b * nch + c) * SPP;
+    const bf16x8 *xsrc = frag + (long)c * SPC + SPP;
+#pragma unroll
+    for (int q = 0; q < (SPC + 255) / 256; ++q) {
+      const int g0 = 256 * q + 64 * wave;
+      if (SPC % 256 == 0 || g0 < SPC) {
+        const bf16x8 *src = g0 < SPP ? zsrc + g0 : xsrc + (g0 - SPP);
+        __builtin_amdgcn_global_load_lds((const void *)(src + lane), (lds_void *)&buf[n][g0], 16,
+                                         0, 0);
+      }
+    }
+  };
+  if (c_beg < c_end) fetch(c_beg, 0);
+  float w[NC][2][4];
+  load_w(c_beg, w);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int cur = 0;
+  for (int c = c_beg; c < c_end; ++c) {
+    const bool more = c + 1 < c_end;
+    float wn[NC][2][4];
+    if (more) {
+      fetch(c + 1, cur ^ 1);
+      load_w(c + 1, wn);
+    }
+    const bf16x8 *fb = buf[cur];
+    f32x4 P[2][NC];
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+      for (int u = 0; u < NC; ++u) zero(P[tt][u]);
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+#pragma unroll
+      for (int kb = 0; kb < NFB; ++kb) {
+        const bf16x8 ah = fb[(tt * FP + 2 * kb + 0) * 64 + lane];
+        const bf16x8 al = fb[(tt * FP + 2 * kb + 1) * 64 + lane];
+#pragma unroll
+        for (int u = 0; u < NC; ++u) P[tt][u] = mma3(ah, al, bxh[u][kb], bxl[u][kb], P[tt][u]);
+      }
+      if (TAIL < 3) {
+#pragma unroll
+        for (int tf = 0; tf < (TAIL == 3 ? 1 : TAIL); ++tf) {
+          const bf16x8 a = fb[(tt * FP + 2 * (NKB - 1) + tf) * 64 + lane];
+#pragma unroll
+          for (int u = 0; u < NC; ++u) P[tt][u] = mma_bf16(a, bxt[u][tf], P[tt][u]);
+        }
+      }
+    }
+    bf16x8 sh[NC], sl[NC];
+#pragma unroll
+    for (int ct = 0; ct < NC; ++ct) {
+      float sv[8];
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) sv[4 * tt + rr] = P[tt][ct][rr] * w[ct][tt][rr];
+      split8(sv, sh[ct], sl[ct]);
+    }
+#pragma unroll
+    for (int it = 0; it < NIE; ++it) {
+      const bf16x8 ah = fb[SPP + (it * 2 + 0) * 64 + lane];
+      const bf16x8 al = fb[SPP + (it * 2 + 1) * 64 + lane];
+#pragma unroll
+      for (int u = 0; u < NC; ++u) acc[it][u] = mma3(ah, al, sh[u], sl[u], acc[it][u]);
+    }
+    if (more) {
+#pragma unroll
+      for (int ct = 0; ct < NC; ++ct)
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+          for (int rr = 0; rr < 4; ++rr) w[ct][tt][rr] = wn[ct][tt][rr];
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    cur ^= 1;
+  }
+#pragma unroll
+  for (int ct = 0; ct < NC; ++ct) {
+    const int col = colv[ct];
+    if (col >= M) continue;
+#pragma unroll
+    for (int it = 0; it < NIE; ++it)
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr)
+        Upart[(((long)s * k + b) * NP + 16 * it + 4 * lq + rr) * M + col] = acc[it][ct][rr];
+  }
+}
+
 }  // namespace mbpt

@@ -629,6 +629,9 @@ __global__ __launch_bounds__(256) void k_sdiag(int M, int no, int nv,
 // ===================================================================================
 using namespace mbpt;
 
+// per-apply workspace slots of the handle
+enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_ZT, WS_UPART, WS_ZFRAG, WS_N };
+
 struct mbpt_sigma {
   int M, no, nv, nopad, prec;
   double kappa;
@@ -645,9 +648,9 @@ struct mbpt_sigma {
   __bf16 *XvB[2], *XvTB[2];
   int nvp32, Mp32;
   double *d;
-  // per-apply workspace, grown on demand: zXv, p, u, Tpart, Tc, Ep
-  void *ws[8];
-  size_t ws_cap[8];        // bytes
+  // per-apply workspace, grown on demand (slots WS_*)
+  void *ws[WS_N];
+  size_t ws_cap[WS_N];     // bytes
   size_t held;
   int prof;                // stage timing on
   hipEvent_t ev[5];
@@ -661,8 +664,6 @@ constexpr int kMaxNocc = 128;
 constexpr int kTargetBlocks = 2048;
 
 int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
-
-enum { WS_ZXV, WS_P, WS_U, WS_TPART, WS_TC, WS_EP, WS_ZT, WS_UPART, WS_N };
 
 void free_ws(mbpt_sigma *h) {
   for (int q = 0; q < WS_N; ++q) {
@@ -705,7 +706,11 @@ int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu, int ldr, bool w
                               km ? 0 : e * k * no * M,
                               e * (size_t)nq * k * no * nv,
                               km ? 4 * (size_t)std::max(h->nvp32, h->Mp32) * ldr : 0,
-                              want_b ? e * (size_t)nsplit * k * h->nopad * M : 0};
+                              want_b ? e * (size_t)nsplit * k * h->nopad * M : 0,
+                              // split BF16, B block: zXv in fragment order
+                              want_b && h->prec == MBPT_BF16X3
+                                  ? 16 * (size_t)k * ((M + 31) / 32) * bf16_shape(h->no).spp
+                                  : 0};
   for (int q = 0; q < WS_N; ++q) {
     int rc = ensure_buf(h, q, sizes[q]);
     if (rc) return rc;
@@ -756,6 +761,38 @@ void launch_exchange_bf16_nv(const mbpt_sigma *h, int k, int nsplit, int rps, hi
   k_exchange_bf16<NP, NIE, TAIL, NV, NC><<<nblk, 256, 0, st>>>(
       h->M, h->no, k, h->frag, (const float *)h->W, (const float *)h->ws[WS_ZXV],
       (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
+}
+
+template <int NP, int NIE, int TAIL>
+void launch_exchange_b_bf16_k(const mbpt_sigma *h, int k, int nsplit, int rps, float *Upart,
+                              hipStream_t st) {
+  constexpr int NC = 2;
+  const int nvb = (h->M + 64 * NC - 1) / (64 * NC);
+  const int nblk = pad_grid((long)nvb * k * nsplit);
+  k_exchange_b_bf16<NP, NIE, TAIL, NC><<<nblk, 256, 0, st>>>(
+      h->M, h->no, k, h->frag, (const bf16x8 *)h->ws[WS_ZFRAG], (const float *)h->Xo,
+      (const float *)h->W, Upart, nsplit, rps, nvb, nblk);
+}
+
+// the B block's split-BF16 exchange: zXv packed, then the kernel of this
+// n_o's (row tiles, tail MFMAs)
+template <int NP>
+int launch_exchange_b_bf16(const mbpt_sigma *h, int k, int nsplit, int rps, float *Upart,
+                           hipStream_t st) {
+  constexpr int NI = NP / 16;
+  const Bf16Shape sh = bf16_shape(h->no);
+  if (sh.np != NP) { set_error("internal: bf16 padding"); return MBPT_ERR_VALUE; }
+  const long n = (long)k * ((h->M + 31) / 32) * sh.spp;
+  k_pack_bf16_zxv<<<(int)std::min<long>((n + 255) / 256, 16384L), 256, 0, st>>>(
+      h->M, h->no, k, (const float *)h->ws[WS_ZXV], (bf16x8 *)h->ws[WS_ZFRAG]);
+  int rc = launch_status("k_pack_bf16_zxv");
+  if (rc) return rc;
+  if (sh.nie == NI - 1 && sh.tail == 1) launch_exchange_b_bf16_k<NP, NI - 1, 1>(h, k, nsplit, rps, Upart, st);
+  else if (sh.nie == NI - 1 && sh.tail == 2) launch_exchange_b_bf16_k<NP, NI - 1, 2>(h, k, nsplit, rps, Upart, st);
+  else if (sh.nie == NI && sh.tail == 2) launch_exchange_b_bf16_k<NP, NI, 2>(h, k, nsplit, rps, Upart, st);
+  else if (sh.nie == NI && sh.tail == 3) launch_exchange_b_bf16_k<NP, NI, 3>(h, k, nsplit, rps, Upart, st);
+  else { set_error("internal: bf16 shape"); return MBPT_ERR_VALUE; }
+  return launch_status("k_exchange_b_bf16");
 }
 
 // two vectors per wave whenever there are two
@@ -892,7 +929,16 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hi
   }
   if ((rc = launch_status("k_exchange"))) return rc;
   T *Upart = (T *)h->ws[WS_UPART];
-  if (Bz) {  // the B block's exchange partials (in working precision; FP32 for split BF16)
+  if (Bz && h->prec == MBPT_BF16X3) {  // the B block's exchange partials, split BF16
+    switch (h->nopad) {
+      case 32: rc = launch_exchange_b_bf16<32>(h, k, nsplit_eff, rps, (float *)Upart, st); break;
+      case 64: rc = launch_exchange_b_bf16<64>(h, k, nsplit_eff, rps, (float *)Upart, st); break;
+      case 96: rc = launch_exchange_b_bf16<96>(h, k, nsplit_eff, rps, (float *)Upart, st); break;
+      case 128: rc = launch_exchange_b_bf16<128>(h, k, nsplit_eff, rps, (float *)Upart, st); break;
+      default: set_error("internal: nopad"); return MBPT_ERR_VALUE;
+    }
+    if (rc) return rc;
+  } else if (Bz) {  // the B block's exchange partials, FP32 / FP64
     switch (h->nopad) {
       case 16: launch_exchange_b<T, 16>(h, k, nsplit_eff, rps, Upart, st); break;
       case 32: launch_exchange_b<T, 32>(h, k, nsplit_eff, rps, Upart, st); break;
