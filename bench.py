@@ -350,8 +350,8 @@ def leg_large():
 def leg_full_action(inst, W, k=20, reps=5):
     """The full Casida action (A z, B z) against the Tamm-Dancoff A z alone on
     the headline instance: per-apply times of k vectors and the B block's
-    extra cost (its exchange is 4 n_o M^2 flops per vector, computed in FP32
-    by the split-BF16 handle), with B z's error against FP64."""
+    extra cost (its exchange is 4 n_o M^2 flops per vector, in split BF16 on
+    the split-BF16 handle), with B z's error against FP64."""
     import torch
     from mbptcode_amd.block_actions import SigmaOperator
     z = torch.randn((k, inst.nocc, inst.nvir), dtype=torch.float64, device='cuda')

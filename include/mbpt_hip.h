@@ -73,8 +73,7 @@ int mbpt_sigma_apply(mbpt_sigma *h, int k, const double *z_dev,
 /* (Az, Bz) = (A z, B z) of the full Casida action for k trial vectors, all
  * (k, nocc, nvir) FP64 and distinct: `apply_AB(z)` with tda=False,
  * block_actions.py:563-661 (the B block, kappa (ia|jb) - (ib|W|ja), from
- * U = (W o P) X_o and U^T X_v, :632-652).  The split-BF16 handle computes the
- * B block's exchange in FP32. */
+ * U = (W o P) X_o and U^T X_v, :632-652), in the handle's precision. */
 int mbpt_sigma_apply_ab(mbpt_sigma *h, int k, const double *z_dev, double *Az_dev,
                         double *Bz_dev, void *stream);
 

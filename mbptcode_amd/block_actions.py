@@ -243,8 +243,7 @@ def isdf_block_action(lr_solver, nocc, lBSE, W_aux, isdf_factors,
     or (X_mo, D, ...).  W_aux: screened interaction in D's gauge, None for
     TDHF (bare exchange), unused when lBSE is False (RPA).  tda=False gives
     the full Casida action, apply_AB(z) -> (A z, B z) (the B block,
-    block_actions.py:632-652; the split-BF16 handle computes its exchange in
-    FP32).  comm: one GPU only, so None or a one-rank communicator.
+    block_actions.py:632-652, in the handle's precision).  comm: one GPU only, so None or a one-rank communicator.
     precision: 'fp64', 'fp32' or 'bf16x3' for the bulk GEMMs.
     """
     if getattr(lr_solver, 'spin_mode', 'restricted') == 'unrestricted':
