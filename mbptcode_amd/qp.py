@@ -1,7 +1,8 @@
 """The quasiparticle equation from Sigma^c on the imaginary axis: Thiele-Pade
 continuation and the largest-weight root (GW/qp_solve.py:529-558, :575-711;
 Base/utils/grids/analytic_continuation.py:15-89; Solvers/qp_equation.py:
-1319-1389), host-side FP64 (a few hundred scalar evaluations per state).
+1319-1389), host-side FP64: per state, an O(n^2) greedy node ordering, one
+vectorised scan per window and a scalar bisection per bracket (about 1 ms).
 
     w = eps_p + <Sigma_x - v_xc>_pp + Re Sigma^c_pp(w)
 
@@ -55,24 +56,34 @@ def pade_eval(z_query, z, a):
 
 def greedy_pade_order(z, f):
     """Node order that keeps each next Thiele coefficient smallest, from the
-    point nearest the origin (analytic_continuation.py:34-61)."""
+    point nearest the origin (analytic_continuation.py:34-61).
+
+    A candidate's coefficient if appended next is the next entry of its own
+    row of the reciprocal-difference table, which depends only on the nodes
+    already chosen; every candidate's row is advanced once per chosen node,
+    so the ordering costs O(n^2) rather than a table per trial."""
     z = np.asarray(z, complex)
     f = np.asarray(f, complex)
-    left = list(range(len(z)))
-    order = [int(np.argmin(np.abs(z)))]
-    left.remove(order[0])
-    while left:
-        best, score = None, np.inf
-        for j in left:
-            trial = order + [j]
-            s = abs(thiele_coefficients(z[trial], f[trial])[-1])
-            if np.isfinite(s) and s < score:
-                best, score = j, s
-        if best is None:
-            raise FloatingPointError('greedy Pade ordering: every next coefficient '
-                                     'is not finite')
-        order.append(best)
-        left.remove(best)
+    n = len(z)
+    start = int(np.argmin(np.abs(z)))
+    order = [start]
+    left = np.ones(n, bool)
+    left[start] = False
+    row = f.copy()            # each candidate's current table entry g_x[c]
+    a_prev, z_prev = f[start], z[start]
+    with np.errstate(all='ignore'):
+        while left.any():
+            nxt = (a_prev - row) / ((z - z_prev) * row)
+            score = np.where(left & np.isfinite(nxt), np.abs(nxt), np.inf)
+            j = int(np.argmin(score))
+            if not np.isfinite(score[j]):
+                raise FloatingPointError(
+                    f'greedy Pade ordering: every next coefficient of the '
+                    f'{int(left.sum())} points left is not finite after {len(order)} nodes')
+            order.append(j)
+            left[j] = False
+            row = nxt
+            a_prev, z_prev = nxt[j], z[j]
     return np.array(order)
 
 
@@ -122,14 +133,16 @@ def _bisect(f, lo, hi, flo, tol, steps):
 
 def largest_weight_root(f, a, windows=QP_WINDOWS, n_omega=QP_GRAPHICAL_N_OMEGA,
                         tol=QP_GRAPHICAL_TOL, steps=QP_GRAPHICAL_MAX_BISECTION,
-                        z_min=QP_WINDOW_Z_MIN):
+                        z_min=QP_WINDOW_Z_MIN, vectorized=False):
     """(w, Z) of f(w) = 0 as described in the module docstring; (nan, nan)
-    when no window holds a root."""
+    when no window holds a root.  vectorized: f takes an array of points (the
+    scan is then one call)."""
     best = None
     nearest = None
     for half in windows:
         grid = np.linspace(a - half, a + half, n_omega)
-        vals = np.array([f(w) for w in grid])
+        vals = (np.asarray(f(grid), float) if vectorized
+                else np.array([f(w) for w in grid]))
         for i in range(n_omega - 1):
             if not (np.isfinite(vals[i]) and np.isfinite(vals[i + 1])):
                 continue
@@ -155,8 +168,10 @@ def solve_qp_from_imaginary_axis(eps_p, xc_p, z_fit, sigma_iw, greedy=True, with
     a = float(eps_p) + float(xc_p)
 
     def f(w):
-        return w - a - fit(np.array([w], complex))[0].real
-    w, z = largest_weight_root(f, a)
+        wa = np.asarray(w, float)
+        out = wa - a - fit(wa.astype(complex)).real
+        return out if out.ndim else float(out)
+    w, z = largest_weight_root(f, a, vectorized=True)
     return (w, z) if with_z else w
 
 

@@ -218,6 +218,25 @@ def test_pade_reproduces_a_rational_function():
         assert np.abs(fit(zq) - g(zq)).max() < 1e-9
 
 
+def test_greedy_order_is_the_brute_force_order():
+    """The incremental greedy ordering picks, at every step, the node whose
+    Thiele coefficient is smallest when a whole table is rebuilt for each
+    candidate (the definition, analytic_continuation.py:34-61)."""
+    from mbptcode_amd.qp import greedy_pade_order, thiele_coefficients
+    rng = np.random.default_rng(17)
+    for _ in range(12):
+        z = 0.05 + 1j * np.sort(rng.uniform(0.01, 15.0, 12)) * rng.choice([-1, 1])
+        poles = rng.uniform(-1, 1, 4) - 1j * rng.uniform(0.02, 0.4, 4)
+        amp = rng.uniform(0.05, 0.3, 4)
+        f = (amp[None, :] / (z[:, None] - poles[None, :])).sum(1)
+        order = [int(np.argmin(np.abs(z)))]
+        rest = [j for j in range(len(z)) if j != order[0]]
+        while rest:
+            sc = [abs(thiele_coefficients(z[order + [j]], f[order + [j]])[-1]) for j in rest]
+            order.append(rest.pop(int(np.argmin(sc))))
+        assert greedy_pade_order(z, f).tolist() == order
+
+
 def test_largest_weight_root_prefers_the_quasiparticle():
     """f(w) = w - a - s/(w - pole): two roots; the one of larger Z = 1/f' is
     kept, not the one nearer a."""
