@@ -36,17 +36,7 @@ import numpy as np
 import torch
 
 from . import grids
-from .space_time import Chi0Builder, _dev, _device
-
-
-def _solve_lower_blocked(C, B, block=1024):
-    """C^-1 B for lower-triangular C, in column blocks of B (rocBLAS's trsm
-    workspace under torch is bounded)."""
-    out = torch.empty_like(B)
-    for j in range(0, B.shape[1], block):
-        out[:, j:j + block] = torch.linalg.solve_triangular(C, B[:, j:j + block].contiguous(),
-                                                            upper=False)
-    return out
+from .space_time import Chi0Builder, _dev, _device, factor_batch_size, solve_lower_blocked
 
 
 def screened_correlation_grid(L, chi0, V=None, infos=None):
@@ -64,21 +54,22 @@ def screened_correlation_grid(L, chi0, V=None, infos=None):
             raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
     else:
         infos.append(info)
-    Y = _solve_lower_blocked(C, L.T.contiguous()).T            # L C^-T
-    Wc = Y @ Y.T - (L @ L.T if V is None else V)
+    Y = solve_lower_blocked(C, L.T.contiguous())               # C^-1 L^T
+    Wc = Y.T @ Y - (L @ L.T if V is None else V)
     return 0.5 * (Wc + Wc.T)
 
 
 def screened_correlation_frequencies(builder, V, grid):
     """(W - V)(i w_k) on the grid for every frequency of `grid`, (nfreq, M, M)
     FP64: proj(tau) on grid's time axis once (an FP64 builder), its cosine
-    transform, Dyson.  Everything stays on the caller's stream and the
-    factorizations' status is checked once, after all of them (a host
-    synchronisation per frequency would leave the device idle between the
-    latency-bound Cholesky factorizations).  The solves are not run
-    concurrently: on several streams, from one host thread or from one
-    thread per stream, torch's rocBLAS / rocSOLVER calls corrupted some
-    frequencies' results (1e-4 relative, at random)."""
+    transform, Dyson.  The frequencies' I - L^T chi0 L are factorized in
+    batches of `factor_batch_size` (rocSOLVER's batched Cholesky keeps the
+    device busy where one factorization alone is latency-bound), and each
+    W - V is written over its chi0.  Everything stays on the caller's stream
+    and the factorizations' status is checked once, after all of them.  The
+    solves are not run concurrently: on several streams, from one host
+    thread or from one thread per stream, torch's rocBLAS / rocSOLVER calls
+    corrupted some frequencies' results (1e-4 relative, at random)."""
     if builder.precision != 'fp64':
         raise ValueError('the screening frequencies need FP64 proj(tau): the '
                          'cosine weights onto them cancel over many digits')
@@ -89,13 +80,27 @@ def screened_correlation_frequencies(builder, V, grid):
     C = torch.as_tensor(np.asarray(grid.cosft_wt, float), device=dev)
     chi = torch.tensordot(C, proj, dims=([1], [0]))
     del proj
-    out = torch.empty_like(chi)
+    nf, n = chi.shape[0], chi.shape[-1]
+    g = factor_batch_size(n, nf)
+    eye = torch.eye(n, dtype=chi.dtype, device=dev)
+    Lt = L.T.contiguous()
+    A = torch.empty((g, n, n), dtype=chi.dtype, device=dev)
     infos = []
-    for k in range(chi.shape[0]):
-        out[k] = screened_correlation_grid(L, chi[k], Vd, infos)
-    if int(torch.stack(infos).abs().sum()) != 0:
+    for k0 in range(0, nf, g):
+        ks = range(k0, min(nf, k0 + g))
+        for j, k in enumerate(ks):
+            K = L.T @ (chi[k] @ L)
+            torch.sub(eye, 0.5 * (K + K.T), out=A[j])
+        C, info = torch.linalg.cholesky_ex(A[:len(ks)])
+        infos.append(info)
+        for j, k in enumerate(ks):
+            Y = solve_lower_blocked(C[j], Lt)                  # C^-1 L^T
+            Wc = Y.T @ Y - Vd
+            chi[k] = 0.5 * (Wc + Wc.T)                         # W - V over chi0
+        del C
+    if int(torch.cat(infos).abs().sum()) != 0:
         raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
-    return out
+    return chi
 
 
 def screened_interaction_tau(Wc, Ctw):

@@ -298,17 +298,43 @@ def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
     C, info = torch.linalg.cholesky_ex(A)
     if int(info) != 0:
         raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
-    # W = L A^-1 L^T = (L C^-T)(L C^-T)^T.  The triangular solve runs on
-    # column blocks of L^T: rocBLAS's trsm workspace under torch is bounded
-    # and a single solve with thousands of right-hand sides fails to get it.
-    Lt = L.T.contiguous()
-    Yt = torch.empty_like(Lt)
-    for j in range(0, n, 1024):
-        Yt[:, j:j + 1024] = torch.linalg.solve_triangular(C, Lt[:, j:j + 1024].contiguous(),
-                                                          upper=False)
-    Y = Yt.T                                                   # L C^-T
-    W = Y @ Y.T
+    # W = L A^-1 L^T = (C^-1 L^T)^T (C^-1 L^T)
+    Y = solve_lower_blocked(C, L.T.contiguous())
+    W = Y.T @ Y
     return 0.5 * (W + W.T), chi0
+
+
+#: device memory for one batched Cholesky factorization: the stacked FP64
+#: matrices and their factors (16 matrices of M = 6000 take 9.2 GB)
+FACTOR_BATCH_BYTES = 16 * 2 ** 30
+
+
+def factor_batch_size(n, count, budget=FACTOR_BATCH_BYTES):
+    """How many n x n FP64 matrices (with their factors) one batched Cholesky
+    takes, at most `count`.  rocSOLVER's factorization of one matrix of a few
+    thousand is latency-bound (22 ms at n = 6000); sixteen at once take
+    3.9 ms each."""
+    return int(max(1, min(count, budget // (2 * 8 * n * n))))
+
+
+def solve_lower_blocked(C, B, block=1024):
+    """C^-1 B for a lower-triangular C (n, n) and B (n, m): block forward
+    substitution with each diagonal block inverted by one triangular solve
+    against the identity, everything else GEMMs (a triangular solve with
+    thousands of right-hand sides runs at a third of the GEMM rate, and
+    rocBLAS's trsm workspace under torch is bounded)."""
+    n = C.shape[0]
+    out = torch.empty_like(B)
+    for k in range(0, n, block):
+        kb = min(block, n - k)
+        eye = torch.eye(kb, dtype=C.dtype, device=C.device)
+        dinv = torch.linalg.solve_triangular(C[k:k + kb, k:k + kb].contiguous(), eye,
+                                             upper=False)
+        rhs = B[k:k + kb]
+        if k:
+            rhs = torch.addmm(rhs, C[k:k + kb, :k], out[:k], alpha=-1.0)
+        out[k:k + kb] = dinv @ rhs
+    return out
 
 
 def dielectric_log_determinant(one_minus_c, where, flags=None):
@@ -364,12 +390,13 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); one fused chi0 launch per
     frequency, FP64 GEMMs and Cholesky log-dets.
 
-    The log-dets and traces stay on the device until the end (no host
-    synchronisation per frequency); a frequency whose I - L^T chi0 L is not
-    positive definite is redone through `dielectric_log_determinant`
-    (slogdet, warning, flags).  One stream: concurrent factorizations through
-    torch's rocBLAS / rocSOLVER (several streams, one or several host
-    threads) corrupted results at random."""
+    The matrices of up to `factor_batch_size` frequencies are factorized in
+    one batched Cholesky; the log-dets and traces stay on the device until
+    the end (no host synchronisation per frequency); a frequency whose
+    I - L^T chi0 L is not positive definite is redone through
+    `dielectric_log_determinant` (slogdet, warning, flags).  One stream:
+    concurrent factorizations through torch's rocBLAS / rocSOLVER (several
+    streams, one or several host threads) corrupted results at random."""
     dev = _device()
     b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
     Vd = _dev(V, dev)
@@ -381,14 +408,20 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     logdet = torch.zeros(nf, dtype=torch.float64, device=dev)
     trace = torch.zeros(nf, dtype=torch.float64, device=dev)
     info = torch.zeros(nf, dtype=torch.int32, device=dev)
-    for k in range(nf):
-        b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
-        K = L.T @ (chi @ L)
-        A = eye - K
-        C, inf = torch.linalg.cholesky_ex(0.5 * (A + A.T))
-        logdet[k] = 2.0 * torch.log(torch.diagonal(C)).sum()
-        trace[k] = torch.trace(K)
-        info[k] = inf
+    g = factor_batch_size(n, nf)
+    A = torch.empty((g, n, n), dtype=torch.float64, device=dev)
+    for k0 in range(0, nf, g):
+        ks = range(k0, min(nf, k0 + g))
+        for j, k in enumerate(ks):
+            b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
+            K = L.T @ (chi @ L)
+            trace[k] = torch.trace(K)
+            torch.sub(eye, 0.5 * (K + K.T), out=A[j])
+        C, inf = torch.linalg.cholesky_ex(A[:len(ks)])
+        logdet[k0:k0 + len(ks)] = 2.0 * torch.log(torch.diagonal(C, dim1=-2, dim2=-1)).sum(-1)
+        info[k0:k0 + len(ks)] = inf
+        del C
+    del A
     logdet_h, trace_h = logdet.cpu().numpy(), trace.cpu().numpy()
     mine = {}
     for k in np.flatnonzero(info.cpu().numpy()):
