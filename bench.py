@@ -473,13 +473,23 @@ def leg_fit():
     Dsep = F.separable_factor(M, V)
     sync()
     t_gauge = time.perf_counter() - t0
+    # the streaming form over 6000-column blocks of the same test set
+    blk = 6000
+    t0 = time.perf_counter()
+    Ms = F.fit_M_streaming((D[:, j:j + blk], Ff[:, j:j + blk]) for j in range(0, nrho, blk))
+    sync()
+    t_stream = time.perf_counter() - t0
+    stream_diff = float((Ms - M).abs().max() / M.abs().max())
     flops = 2.0 * nk * nk * nrho + 2.0 * naux * nk * nrho
     res = {'workload': f'nk={nk}, nrho={nrho}, naux={naux}, FP64',
            'fit_s': t_fit, 'gauge_s': t_gauge,
            'fit_gemm_tflops': flops / t_fit / 1e12,
+           'streaming_fit_s': t_stream, 'streaming_block': blk,
+           'streaming_vs_whole_rel': stream_diff,
            'factor_shape': list(Dsep.shape)}
     log(f'fit: {t_fit * 1e3:.0f} ms ({res["fit_gemm_tflops"]:.1f} TF FP64 over its two '
-        f'products), gauge {t_gauge * 1e3:.0f} ms')
+        f'products), streaming {t_stream * 1e3:.0f} ms ({stream_diff:.1e} from the whole '
+        f'fit), gauge {t_gauge * 1e3:.0f} ms')
     return res
 
 

@@ -72,6 +72,67 @@ def fit_M_stable(D, F, regularization=ISDF_FIT_REGULARIZATION):
     return torch.cholesky_solve(R, C).T * d[None, :]
 
 
+def fit_M_streaming(blocks, regularization=ISDF_FIT_REGULARIZATION, timings=None):
+    """M (naux, nk) without holding D or F whole: the pyscf-free core of the
+    reference's streaming fit (separable_ri.py:664-934).  `blocks` yields
+    (D_blk (nk, nb), F_blk (naux, nb)) column blocks of the test set (the
+    caller's three-centre pass, with the auxiliary block D = the auxiliary
+    functions on the points, F = I among them); only
+
+        S = sum D_blk D_blk^T  (nk, nk),   FD = sum F_blk D_blk^T  (naux, nk)
+
+    are kept.  The row norms of D are diag(S), so the balancing is
+    G = d S d + lambda I and the solve is one Cholesky: M = ((FD d) G^-1) d,
+    fit_M_stable's estimator up to the order of the sums.  `fit='rows'`
+    (grid tiles over ranks) is a multi-rank layout and not built.  timings,
+    a dict: 'fit_blocks', 'fit_gram' and 'fit_cholesky' seconds (device
+    synchronised)."""
+    import time
+    S = FD = None
+    nblk = 0
+    t0 = time.perf_counter()
+    for D_blk, F_blk in blocks:
+        Db, Fb = _dev(D_blk), _dev(F_blk)
+        if Db.shape[1] != Fb.shape[1]:
+            raise ValueError(f'block {nblk}: D has {Db.shape[1]} columns, F {Fb.shape[1]}')
+        if S is None:
+            S = torch.zeros((Db.shape[0], Db.shape[0]), dtype=torch.float64, device=Db.device)
+            FD = torch.zeros((Fb.shape[0], Db.shape[0]), dtype=torch.float64, device=Db.device)
+        elif Db.shape[0] != S.shape[0] or Fb.shape[0] != FD.shape[0]:
+            raise ValueError(f'block {nblk} changes nk or naux')
+        S.addmm_(Db, Db.T)
+        FD.addmm_(Fb, Db.T)
+        nblk += 1
+    if S is None:
+        raise ValueError('no test-set blocks')
+    if timings is not None:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        timings['fit_blocks'] = t1 - t0
+        timings['fit_blocks_total'] = nblk
+        t0 = t1
+    scale = torch.sqrt(torch.clamp(torch.diagonal(S), min=0.0))
+    scale = torch.where(scale == 0, torch.ones_like(scale), scale)
+    d = 1.0 / scale
+    G = S.mul_(d[:, None]).mul_(d[None, :])      # balanced in place
+    G.diagonal().add_(float(regularization))
+    if timings is not None:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        timings['fit_gram'] = t1 - t0
+        t0 = t1
+    C, info = torch.linalg.cholesky_ex(G)
+    if int(info) != 0:
+        raise np.linalg.LinAlgError(
+            f'Cholesky of the balanced Gram matrix failed at leading minor {int(info)}: '
+            'the points are degenerate')
+    M = torch.cholesky_solve((FD * d[None, :]).T, C).T * d[None, :]
+    if timings is not None:
+        torch.cuda.synchronize()
+        timings['fit_cholesky'] = time.perf_counter() - t0
+    return M
+
+
 def aux_metric_sqrt(V, floor=AUX_METRIC_ROOT_FLOOR):
     """V^1/2 of a symmetric metric, directions below `floor` x the largest
     eigenvalue dropped (its numerical null space; separable_ri.py:1802-1836,
