@@ -71,9 +71,9 @@ def subspace_solve(a, b, sigma, pi, k):
     LinAlgError where a - b is not positive definite on the space."""
     P, Q, T = a + b, a - b, sigma + pi
     L = np.linalg.cholesky(0.5 * (Q + Q.T))
-    C = sla.solve_triangular(L, T, lower=True)             # L^-1 T
-    G = np.linalg.solve(C.T, P)                             # C^-T P
-    H = np.linalg.solve(C.T, G.T).T                         # C^-T P C^-1
+    # C = L^-1 T, so C^-1 = T^-1 L: one LU of T serves H and the vectors
+    U = sla.lu_solve(sla.lu_factor(T, check_finite=False), L, check_finite=False)
+    H = U.T @ P @ U                                         # C^-T P C^-1
     H = 0.5 * (H + H.T)
     m = H.shape[0]
     top = min(m, 2 * k)
@@ -83,7 +83,7 @@ def subspace_solve(a, b, sigma, pi, k):
     keep = np.flatnonzero(w2 > 0)[:k]
     w = np.sqrt(w2[keep])
     Wv = Wv[:, keep] / np.sqrt(w)                           # |w|^2 = 1 / omega
-    u = np.linalg.solve(C, Wv)
+    u = U @ Wv                                              # C^-1 Wv
     v = sla.solve_triangular(L.T, Wv, lower=False) * w
     return w, 0.5 * (u + v), 0.5 * (u - v)
 
