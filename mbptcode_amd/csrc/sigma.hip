@@ -287,19 +287,55 @@ __global__ __launch_bounds__(256) void k_exchange_b(
             (lds_ptr_t *)&xs[n][r * LD], 16, 0, 0);
     }
   };
+  // The next stage's operands are requested at the top of each stage, W
+  // first: its values, then the X_o rows (direct to LDS), then zXv[b]'s rows
+  // (into registers, stored to LDS after the stage's closing barrier).  The
+  // stage itself reads only LDS and registers loaded a stage earlier, so no
+  // wait inside it stalls on the loads in flight.
+  constexpr int ZPT = ROWS * NOPAD / 256;   // zXv values per thread and stage
+  T zr[ZPT];
+  T wc[ROWS / 16][4], wn[ROWS / 16][4];
+  // zXv[b][j][nu0 + r] for e = tid + 256 i = j ROWS + r: coalesced along nu,
+  // zero past M and n_o
+  auto zload = [&](int st) {
+#pragma unroll
+    for (int i = 0; i < ZPT; ++i) {
+      const int e = tid + 256 * i, j = e / ROWS, r = e % ROWS, nu = ROWS * st + r;
+      zr[i] = (j < no && nu < M) ? zb[(long)j * M + nu] : T(0);
+    }
+  };
+  auto zstore = [&]() {
+#pragma unroll
+    for (int i = 0; i < ZPT; ++i) {
+      const int e = tid + 256 * i;
+      zs[(e % ROWS) * LDZ + e / ROWS] = zr[i];
+    }
+  };
+  // W[nu][mu] at the rows of P^T's accumulator (rows past M clamped: P = 0)
+  auto wload = [&](int st, T (&w)[ROWS / 16][4]) {
+#pragma unroll
+    for (int ch = 0; ch < ROWS / 16; ++ch)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int nu = min(ROWS * st + 16 * ch + MF<T>::orow(lq, r), M - 1);
+        w[ch][r] = W[(long)nu * M + colc];
+      }
+  };
   if (st_beg < st_end) {
+    wload(st_beg, wc);
     fetch(st_beg, 0);
+    zload(st_beg);
+    zstore();
     int cur = 0;
     for (int st = st_beg; st < st_end; ++st) {
-      const int nu0 = ROWS * st;
-      // zXv[b][j][nu0 + r] -> zs[r][j], coalesced along nu, zero past M and n_o
-      for (int e = tid; e < ROWS * NOPAD; e += 256) {
-        const int j = e / ROWS, r = e % ROWS, nu = nu0 + r;
-        zs[r * LDZ + j] = (j < no && nu < M) ? zb[(long)j * M + nu] : T(0);
-      }
       __builtin_amdgcn_s_waitcnt(0);   // this wave's X_o stage landed
       __syncthreads();                 // ... every wave's, and zs is written
-      if (st + 1 < st_end) fetch(st + 1, cur ^ 1);
+      const bool more = st + 1 < st_end;
+      if (more) {
+        wload(st + 1, wn);
+        fetch(st + 1, cur ^ 1);
+        zload(st + 1);
+      }
       const T *x = xs[cur];
 #pragma unroll
       for (int ch = 0; ch < ROWS / 16; ++ch) {
@@ -310,17 +346,21 @@ __global__ __launch_bounds__(256) void k_exchange_b(
         for (int q = 0; q < NS; ++q) P = MF<T>::mma(zs[(r0 + lr) * LDZ + 4 * q + lq], xb[q], P);
         T S[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int nu = min(nu0 + r0 + MF<T>::orow(lq, r), M - 1);   // rows past M: P = 0
-          S[r] = P[r] * W[(long)nu * M + colc];
-        }
+        for (int r = 0; r < 4; ++r) S[r] = P[r] * wc[ch][r];
 #pragma unroll
         for (int r = 0; r < 4; ++r)
 #pragma unroll
           for (int it = 0; it < NI; ++it)
             acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r], acc[it]);
       }
-      __syncthreads();   // zs and xs[cur] are rewritten next stage
+      __syncthreads();   // every wave is done with zs and xs[cur]
+      if (more) {
+        zstore();
+#pragma unroll
+        for (int ch = 0; ch < ROWS / 16; ++ch)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) wc[ch][r] = wn[ch][r];
+      }
       cur ^= 1;
     }
   }
