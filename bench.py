@@ -390,6 +390,7 @@ def leg_full_action(inst, W, k=20, reps=5):
         if prec != 'fp64':
             ref64 = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V,
                                       precision='fp64', tda=False)[0]
+        run_davidson(act, d, inst.nroots, inst.tol, 200, tda=False, refine_apply=ref64)
         tm = {}
         sync()
         t0 = time.perf_counter()

@@ -131,14 +131,25 @@ class CasidaResult:
 
 
 def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=None,
-                    lindep=None, y0=None):
+                    lindep=None, y0=None, phases=None):
     """Paired Davidson for the `nroots` lowest Casida roots, on device (or
     CPU) tensors.
 
     apply_ab(Z (p, n)) -> (A Z, B Z), each (p, n), FP64.  diag (n,) the
     preconditioner diagonal.  x0 (n, p0) guess columns for X, y0 those for Y
     (default 0; a warm start passes converged (X, Y) pairs).  Every
-    returned root satisfies |[R_x; R_y]| <= tol where `conv` says so."""
+    returned root satisfies |[R_x; R_y]| <= tol where `conv` says so.
+    `phases`, a dict, receives the seconds of each part of the cycle (the
+    device is synchronised at every boundary, so only for profiling)."""
+    last = [time.perf_counter()]
+
+    def mark(name):
+        if phases is not None:
+            if diag.is_cuda:
+                torch.cuda.synchronize()
+            now = time.perf_counter()
+            phases[name] = phases.get(name, 0.0) + now - last[0]
+            last[0] = now
     dev = diag.device
     n = diag.numel()
     nroots = max(1, min(int(nroots), n))
@@ -214,11 +225,13 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     conv = np.zeros(nroots, bool)
     w = x = y = rn = None
     cycles = 0
+    mark('setup')
     for cycle in range(max_cycle):
         cycles = cycle + 1
         k = min(nroots, m)
         with _host_lapack():
             w, x, y = subspace_solve(a[:m, :m], b[:m, :m], sg[:m, :m], pi[:m, :m], k)
+        mark('subspace')
         xt = torch.as_tensor(x, device=dev)
         yt = torch.as_tensor(y, device=dev)
         wt = torch.as_tensor(w, device=dev)
@@ -230,6 +243,7 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         conv = rn <= tol
         trace.append({'cycle': cycle, 'space': int(m), 'max_residual': float(rn.max()),
                       'converged': int(conv.sum())})
+        mark('residuals')
         if conv.all() or len(w) < k:
             break
         idx = np.flatnonzero(~conv)
@@ -240,7 +254,9 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         den_y = torch.where(den_y.abs() < floor, torch.full_like(den_y, floor), den_y)
         tx, ty = Rx[:, idx] / den_x, Ry[:, idx] / den_y
         nrm = torch.sqrt((tx * tx).sum(0) + (ty * ty).sum(0)).clamp_min(1e-300)
+        mark('correction')
         tx, ty = orthonormal_new(tx / nrm, ty / nrm)
+        mark('orthonormal')
         if tx.shape[1] == 0:
             break
         if m + tx.shape[1] > max_space:
@@ -253,7 +269,11 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
             nU2 = U2h[:, :m] @ xk + U1h[:, :m] @ yk
             m = 0
             append(nV, nW, nU1, nU2)
-        append(tx, ty, *act(tx, ty))
+        mark('collapse')
+        prods = act(tx, ty)
+        mark('sigma')
+        append(tx, ty, *prods)
+        mark('append')
     if len(w) < nroots:
         warnings.warn(f'Casida Davidson: only {len(w)} roots in a space of {m}',
                       RuntimeWarning, stacklevel=2)

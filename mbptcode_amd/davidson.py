@@ -28,7 +28,6 @@ import time
 import warnings
 
 import numpy as np
-import scipy.linalg as sla
 import torch
 
 from .subspace import tn
@@ -252,18 +251,6 @@ def _orthonormal_new(T, V, drop):
     return T @ torch.as_tensor(C, device=T.device)
 
 
-def _lowest_eigh(H, top):
-    """The `top` lowest eigenpairs of the symmetric host matrix H: MRRR on
-    that subset (about 2/3 of a full solve's time at m ~ 200, where only the
-    tracked and kept Ritz vectors are used), a full solve when all are asked
-    for."""
-    m = H.shape[0]
-    with _host_lapack():
-        if top < m:
-            return sla.eigh(H, subset_by_index=[0, top - 1], driver='evr', check_finite=False)
-        return np.linalg.eigh(H)
-
-
 class DavidsonResult:
     __slots__ = ('e', 'X', 'conv', 'rnorm', 'cycles', 'vectors', 'seconds',
                  'trace', 'stalled')
@@ -357,8 +344,9 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         Hns = HG[:, :m]
         H = 0.5 * (Hns + Hns.T)
         G = 0.5 * (HG[:, m:] + HG[:, m:].T)
+        with _host_lapack():
+            evals, evecs = np.linalg.eigh(H)
         k = min(nroots, m)
-        evals, evecs = _lowest_eigh(H, max(k, keep or k))
         w, y = evals[:k], evecs[:, :k]
         y_rows = m
         sq = ((G @ y) * y).sum(0)
@@ -713,8 +701,8 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
         op64 = _operator_of(refine_apply)
         ab64 = ab_of(op64)
         Xr, Yr = X[:, :nroots].contiguous(), Y[:, :nroots].contiguous()
-        AX, BX = ab64(Xr.T.contiguous())
-        AY, BY = ab64(Yr.T.contiguous())
+        A2, B2 = ab64(torch.cat([Xr, Yr], dim=1).T.contiguous())   # one apply of both
+        AX, AY, BX, BY = A2[:nroots], A2[nroots:], B2[:nroots], B2[nroots:]
         w = torch.as_tensor(e[:nroots], device=dev)
         Rx = AX.T + BY.T - Xr * w
         Ry = BX.T + AY.T + Yr * w
