@@ -198,8 +198,10 @@ int grow(float *&buf, size_t &cap, size_t bytes) {
 }
 
 // time-point chunks per tile: the split whose item count best fills whole
-// rounds of one 512-thread workgroup per CU (131 VGPRs: 3 waves per SIMD fit,
-// not the 4 a second workgroup needs)
+// rounds of one workgroup per CU.  Two fit per CU (the kernel is held to 128
+// VGPRs), but counting 512 slots chose more chunks and measured 3-9% slower:
+// more partial sums to write and reduce, for a tail the second resident
+// workgroup already shortens.
 int chi0_chunks(int ntiles, int ntau) {
   const int slots = 256;
   int best = 1;
@@ -293,8 +295,12 @@ int sigma_f32(mbpt_chi0 *h, int ntau, const double *tau, const float *Zt, int ns
       M, Mp, ns, nsp, h->gw_cols, h->XT, h->gw_xs);
   if ((rc = launch_status("k_gw_pack_states"))) return rc;
   const int nblk = pad_grid((long)ntiles * ntau);
-  k_gw_sigma_f32<<<nblk, 512, 0, st>>>(M, Mp, ko, kv, nt, ntau, h->Yt, Zt, h->gw_xs, nsp,
-                                       h->gw_part, nblk);
+  if (nsp <= 64)
+    k_gw_sigma_f32<4><<<nblk, 512, 0, st>>>(M, Mp, ko, kv, nt, ntau, h->Yt, Zt, h->gw_xs, nsp,
+                                            h->gw_part, nblk);
+  else
+    k_gw_sigma_f32<1><<<nblk, 512, 0, st>>>(M, Mp, ko, kv, nt, ntau, h->Yt, Zt, h->gw_xs, nsp,
+                                            h->gw_part, nblk);
   if ((rc = launch_status("k_gw_sigma_f32"))) return rc;
   k_gw_sigma_reduce<<<2 * ntau * (nsp / 16), 256, 0, st>>>(ntau, ntiles, ns, nsp, h->gw_part,
                                                             out);

@@ -33,7 +33,14 @@
 namespace mbpt {
 
 // part[((t * ntiles + tile) * 2 + branch) * nsp + s]
-__global__ __launch_bounds__(512) void k_gw_sigma_f32(int M, int Mp, int ko, int kv, int nt,
+//
+// WAVES: the waves per SIMD the register allocation is held to.  At 4 (128
+// VGPRs, two workgroups per CU) the Green's function products run ~10%
+// faster, but ten values spill and are reloaded once per 16-state group, so
+// the host picks 4 for a few state groups and the unconstrained 143-VGPR
+// build (3 waves, one workgroup per CU) for many.
+template <int WAVES>
+__global__ __launch_bounds__(512, WAVES) void k_gw_sigma_f32(int M, int Mp, int ko, int kv, int nt,
                                                       int ntau, const float *__restrict__ Yt,
                                                       const float *__restrict__ Zt,
                                                       const float *__restrict__ Xs, int nsp,
