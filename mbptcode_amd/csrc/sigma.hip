@@ -246,13 +246,16 @@ __global__ __launch_bounds__(256) void k_exchange(
 // -- zXv[b][:, nu] staged transposed in LDS, X_o[nu, :] by the same
 // double-buffered direct loads as above.  The P^T result element r of lane l
 // (row orow(l/16, r)) is again the K slot of the second product.  Partials
-// U^T_s are written in the layout of T_s, so the tail is shared.
+// U^T_s are written in the layout of T_s, so the tail is shared.  Stages are
+// 16 rows (the A kernel's are 32): with zXv's stage beside X_o's in LDS and
+// the next stage's operands in registers, 32-row stages held the FP64 build
+// to one wave per SIMD; 16-row ones fit two (10% faster in FP64).
 template <typename T, int NOPAD>
 __global__ __launch_bounds__(256) void k_exchange_b(
     int M, int no, int k, const T *__restrict__ Xp, const T *__restrict__ W,
     const T *__restrict__ zXv, T *__restrict__ Upart, int nsplit,
     int rows_per_split, int nmb, int nblk) {
-  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, ROWS = 32;
+  constexpr int NS = NOPAD / 4, NI = NOPAD / 16, ROWS = 16;
   constexpr int LD = NOPAD + 16 / (int)sizeof(T), NL = NOPAD * (int)sizeof(T) / 16;
   constexpr int LDZ = NOPAD + 1;
   __shared__ __attribute__((aligned(16))) T xs[2][ROWS * LD];
