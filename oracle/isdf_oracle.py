@@ -40,8 +40,11 @@ Pinning status (what checks each function; see tests/test_oracle.py):
 * static_screening_matrix -- Dyson identity W (I - chi0) = I, closed form chi0(0)
 * rpa_correlation_energy_space_time -- Casida trace at quadrature error
 * davidson_tda            -- dense eigh of the same A
-* the cosine-transform weights' exact bits -- parity unpinned (only an import
-  of the reference could pin them; pinned here to the fit's model error)
+* the minimax transform weights (the static grid's split and the three
+  self-energy transforms) -- the reference's own outputs on the same axes,
+  recorded by tests/golden/make_transform_weights.py (an import of the
+  reference's time_frequency module): bitwise, and 1e-13 relative for the
+  omega -> tau cosine weights; also the fit's model error
 * gw_sigma_space_time and its parts -- the exact GW@RPA pole sum
   (gw_sigma_sum_over_poles, its W - I expansion against the matrix inverse),
   planted slips, a brute-force branch sum (tests/test_gw_oracle.py)

@@ -89,3 +89,35 @@ def test_refuses_bad_windows():
         grids.minimax_time_grid(7, 0.1, 1.0)
     with pytest.raises(ValueError):
         grids.gauss_legendre_grid(10, -1.0)
+
+
+def test_transform_weights_match_the_reference_fixture():
+    """The minimax split grid of the static W and the three self-energy
+    transforms against the reference's own outputs on the same axes
+    (tests/golden/transform_weights.npz, written by make_transform_weights.py
+    from time_frequency.py:282-380, :498-555).  Tolerance: 1e-11 relative to
+    the largest weight (the omega -> tau cosine weights reach 7e8 with
+    cancelling signs; the others agree to the bit)."""
+    import os
+    import warnings
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'transform_weights.npz')
+    g = np.load(path)
+    kinds = {'cos_tw': grids.COSINE_TW, 'cos_wt': grids.COSINE_WT, 'sin_tw': grids.SINE_TW}
+    for cfg in (0, 1):
+        p = f'cfg{cfg}_static_'
+        e_min, e_max = g[p + 'range']
+        with warnings.catch_warnings():
+            warnings.simplefilter('ignore', RuntimeWarning)
+            ours = grids.TimeFrequencyGrid.minimax_split(int(g[p + 'ntau']), e_min, e_max,
+                                                         [0.0], [1.0])
+        for key, got in (('tau', ours.tau_points), ('tau_weights', ours.tau_weights),
+                         ('cosft_wt', ours.cosft_wt)):
+            ref = g[p + key]
+            assert np.abs(np.asarray(got) - ref).max() <= 1e-11 * np.abs(ref).max(), (cfg, key)
+        for name, kind in kinds.items():
+            p = f'cfg{cfg}_gw_{name}_'
+            W, err = grids.transform_weights(kind, g[p + 'tau'], g[p + 'omega'], *g[p + 'range'],
+                                             warn=False)
+            ref = g[p + 'weights']
+            assert np.abs(W - ref).max() <= 1e-11 * np.abs(ref).max(), (cfg, name)
+            assert abs(err - float(g[p + 'fit_error'])) <= 1e-8 * float(g[p + 'fit_error'])
