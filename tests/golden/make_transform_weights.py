@@ -1,7 +1,9 @@
 """Writes tests/golden/transform_weights.npz: the reference's own minimax
 transform weights (src/Base/utils/grids/time_frequency.py:282-380, the
 `TimeFrequencyGrid.minimax_split` path at :498-555) on the axes the static W
-and the space-time self-energy use for the water- and benzene-size instances.
+and the space-time self-energy use for the water- and benzene-size instances,
+with those axes' fit ranges (:401-428), point count (:233-254) and the
+minimax and Gauss-Legendre frequency grids (grids.py:107-126, :161-170).
 
 The weights come from a regularized least-squares fit whose last bits depend
 on the path through the SVD, so only the reference's code gives them; this
@@ -27,6 +29,7 @@ def main():
     from mbptcode_amd import grids, synth
     sys.path.insert(0, REFERENCE)
     from src.Base.utils.grids import time_frequency as TF   # the reference
+    from src.Base.utils.grids import grids as RG             # the reference
     out = {}
     kinds = {'cos_tw': grids.COSINE_TW, 'cos_wt': grids.COSINE_WT, 'sin_tw': grids.SINE_TW}
     for cfg in (0, 1):
@@ -43,6 +46,21 @@ def main():
         out[p + 'cosft_wt'] = np.asarray(ref.cosft_wt, float)
         ax = grids.space_time_gw_grids(inst.eps, inst.nocc, ntau=16)
         rW, rS = ax['ranges']
+        # the self-energy's fit ranges, their point count and frequency grid
+        p = f'cfg{cfg}_axes_'
+        refW, refS = TF.sigma_fit_ranges(inst.eps, inst.nocc)
+        out[p + 'eps'] = np.asarray(inst.eps, float)
+        out[p + 'nocc'] = np.array(inst.nocc)
+        out[p + 'rW'] = np.array(refW, float)
+        out[p + 'rS'] = np.array(refS, float)
+        n, err = TF.minimax_points_for_ranges([refW[1] / refW[0], refS[1] / refS[0]])
+        out[p + 'npoints'] = np.array([n, err], float)
+        fp, fw = RG.minimax_frequency_grid(16, *refW)
+        out[p + 'freq16_points'] = np.asarray(fp, float)
+        out[p + 'freq16_weights'] = np.asarray(fw, float)
+        gp, gw = RG.gauss_legendre_grid(16, 1.0)
+        out[p + 'gl16_points'] = np.asarray(gp, float)
+        out[p + 'gl16_weights'] = np.asarray(gw, float)
         for name, kind in kinds.items():
             omega = ax['freq_points'] if name == 'cos_wt' else ax['pade_freq']
             rng = rW if name == 'cos_wt' else rS

@@ -121,3 +121,28 @@ def test_transform_weights_match_the_reference_fixture():
             ref = g[p + 'weights']
             assert np.abs(W - ref).max() <= 1e-11 * np.abs(ref).max(), (cfg, name)
             assert abs(err - float(g[p + 'fit_error'])) <= 1e-8 * float(g[p + 'fit_error'])
+
+
+def test_self_energy_axes_match_the_reference_fixture():
+    """The self-energy's fit ranges, their minimax point count, the minimax
+    frequency grid on rW and the Gauss-Legendre grid against the reference's
+    own outputs (tests/golden/transform_weights.npz; time_frequency.py:
+    233-254, :401-428, grids.py:107-126, :161-170), to 1e-14 relative."""
+    import os
+    path = os.path.join(os.path.dirname(__file__), 'golden', 'transform_weights.npz')
+    g = np.load(path)
+
+    def close(got, ref):
+        got, ref = np.asarray(got, float), np.asarray(ref, float)
+        return np.abs(got - ref).max() <= 1e-14 * max(np.abs(ref).max(), 1e-300)
+    for cfg in (0, 1):
+        p = f'cfg{cfg}_axes_'
+        eps, nocc = g[p + 'eps'], int(g[p + 'nocc'])
+        rW, rS = grids.sigma_fit_ranges(eps, nocc)
+        assert close(rW, g[p + 'rW']) and close(rS, g[p + 'rS']), cfg
+        n, err = grids.minimax_points_for_ranges([rW[1] / rW[0], rS[1] / rS[0]])
+        assert n == int(g[p + 'npoints'][0]) and np.isclose(err, g[p + 'npoints'][1], rtol=1e-12)
+        fp, fw = grids.minimax_frequency_grid(16, *rW)
+        assert close(fp, g[p + 'freq16_points']) and close(fw, g[p + 'freq16_weights']), cfg
+        gp, gw = grids.gauss_legendre_grid(16, 1.0)
+        assert close(gp, g[p + 'gl16_points']) and close(gw, g[p + 'gl16_weights']), cfg
