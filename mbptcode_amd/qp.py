@@ -89,14 +89,20 @@ def greedy_pade_order(z, f):
 
 class PadeFit:
     """The Thiele-Pade fraction through (z_i, f_i), greedily ordered unless
-    `greedy` is False (analytic_continuation.py:64-89)."""
+    `greedy` is False, cut to the first `max_order` nodes of that order when
+    given (analytic_continuation.py:64-89)."""
 
-    def __init__(self, z, f, greedy=True):
+    def __init__(self, z, f, greedy=True, max_order=None):
         z = np.asarray(z, complex)
         f = np.asarray(f, complex)
         if greedy:
             order = greedy_pade_order(z, f)
             z, f = z[order], f[order]
+        if max_order is not None and max_order < len(z):
+            if max_order < 2:
+                raise ValueError(f'max_order = {max_order}: a Pade fit needs at least two '
+                                 'points')
+            z, f = z[:max_order], f[:max_order]
         self.z, self.f = z, f
         self.coefficients = thiele_coefficients(z, f)
 

@@ -3,7 +3,9 @@ transform weights (src/Base/utils/grids/time_frequency.py:282-380, the
 `TimeFrequencyGrid.minimax_split` path at :498-555) on the axes the static W
 and the space-time self-energy use for the water- and benzene-size instances,
 with those axes' fit ranges (:401-428), point count (:233-254) and the
-minimax and Gauss-Legendre frequency grids (grids.py:107-126, :161-170).
+minimax and Gauss-Legendre frequency grids (grids.py:107-126, :161-170), and
+the Thiele-Pade continuation of an exact self-energy
+(analytic_continuation.py:15-89).
 
 The weights come from a regularized least-squares fit whose last bits depend
 on the path through the SVD, so only the reference's code gives them; this
@@ -71,6 +73,31 @@ def main():
             out[p + 'range'] = np.array(rng, float)
             out[p + 'weights'] = np.asarray(W, float)
             out[p + 'fit_error'] = np.array(err, float)
+    # Thiele-Pade continuation of the exact GW@RPA self-energy of the
+    # water-size instance at the Pade frequencies (the oracle's pole sum as
+    # input), and the fraction's values on the real axis
+    from src.Base.utils.grids import analytic_continuation as AC   # the reference
+    from oracle import isdf_oracle as O
+    inst = synth.config(0)
+    ax = grids.space_time_gw_grids(inst.eps, inst.nocc, ntau=16)
+    states = [inst.nocc - 1, inst.nocc]
+    z = ax['mu'] + 1j * np.asarray(ax['pade_freq'], float)
+    f = O.gw_sigma_sum_over_poles(inst.X, O.grid_factors(inst.V), inst.eps, inst.nocc, states,
+                                  ax['pade_freq'], ax['mu'])
+    wq = np.linspace(inst.eps[inst.nocc - 1] - 0.6, inst.eps[inst.nocc] + 0.6, 41) + 0j
+    for i, p_state in enumerate(states):
+        fit = AC.PadeFit(z, f[i])
+        cut = AC.PadeFit(z, f[i], max_order=9)
+        p = f'pade_state{i}_'
+        out[p + 'z'] = z
+        out[p + 'f'] = np.asarray(f[i], complex)
+        out[p + 'order_z'] = np.asarray(fit.z, complex)
+        out[p + 'coefficients'] = np.asarray(fit.coefficients, complex)
+        out[p + 'query'] = wq
+        out[p + 'values'] = np.asarray(fit(wq), complex)
+        out[p + 'values_cut9'] = np.asarray(cut(wq), complex)
+        out[p + 'plain_coefficients'] = np.asarray(AC.PadeFit(z, f[i], greedy=False).coefficients,
+                                                   complex)
     path = os.path.join(HERE, 'transform_weights.npz')
     np.savez(path, **out)
     print(f'wrote {path}: {len(out)} arrays')

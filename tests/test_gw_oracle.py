@@ -268,3 +268,26 @@ def test_quasiparticle_energies_against_the_exact_roots():
     sig, ax = oracle_sigma(inst, D, states)
     e2 = qp.quasiparticle_energies(sig, inst.eps, no, states, ax['mu'], ax['pade_freq'], xc)
     assert np.abs(e2 - exact[:, 0]).max() * HA_MEV < 1.0
+
+
+def test_pade_matches_the_reference_fixture():
+    """Greedy node order, Thiele coefficients (greedy and plain) and values on
+    the real axis, full and cut to 9 nodes, of the exact water-size
+    self-energy of two states against the reference's own PadeFit
+    (tests/golden/transform_weights.npz; analytic_continuation.py:15-89):
+    the order exactly, the rest to 1e-10 relative."""
+    import os
+    from mbptcode_amd.qp import PadeFit
+    g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'transform_weights.npz'))
+
+    def close(got, ref, tol=1e-10):
+        return np.abs(np.asarray(got) - ref).max() <= tol * np.abs(ref).max()
+    for i in range(2):
+        p = f'pade_state{i}_'
+        z, f, wq = g[p + 'z'], g[p + 'f'], g[p + 'query']
+        fit = PadeFit(z, f)
+        assert np.array_equal(fit.z, g[p + 'order_z'])
+        assert close(fit.coefficients, g[p + 'coefficients'])
+        assert close(fit(wq), g[p + 'values'])
+        assert close(PadeFit(z, f, max_order=9)(wq), g[p + 'values_cut9'])
+        assert close(PadeFit(z, f, greedy=False).coefficients, g[p + 'plain_coefficients'])
