@@ -12,7 +12,8 @@ potential (pyscf); here it is the caller's `xc_diagonal`, as the reference's
 
 The root search is a reduced form of the reference's `find_qp_root`: the
 roots of f(w) = w - a - Re Sigma(w) (a = eps_p + xc_p) are bracketed on a
-scan of half-width 0.5, then 1.0, then 2.0 Ha about a, refined by
+scan of half-width 0.5, then 1.0, then 2.0 Ha about the bare level eps_p
+(the reference centres its windows there too), refined by
 bisection, weighed by Z = 1/f'(w) (a central difference), and the root of
 largest valid weight (0 < Z <= 1) is kept; a weight above 0.5 ends the
 widening, and a scan with no valid weight keeps the root nearest a.  The
@@ -177,7 +178,7 @@ def solve_qp_from_imaginary_axis(eps_p, xc_p, z_fit, sigma_iw, greedy=True, with
         wa = np.asarray(w, float)
         out = wa - a - fit(wa.astype(complex)).real
         return out if out.ndim else float(out)
-    w, z = largest_weight_root(f, a, vectorized=True)
+    w, z = largest_weight_root(f, float(eps_p), vectorized=True)
     return (w, z) if with_z else w
 
 

@@ -5,7 +5,8 @@ and the space-time self-energy use for the water- and benzene-size instances,
 with those axes' fit ranges (:401-428), point count (:233-254) and the
 minimax and Gauss-Legendre frequency grids (grids.py:107-126, :161-170), and
 the Thiele-Pade continuation of an exact self-energy
-(analytic_continuation.py:15-89).
+(analytic_continuation.py:15-89) with the quasiparticle roots the reference's
+root search finds on it (Solvers/qp_equation.py:1353-1389).
 
 The weights come from a regularized least-squares fit whose last bits depend
 on the path through the SVD, so only the reference's code gives them; this
@@ -98,6 +99,36 @@ def main():
         out[p + 'values_cut9'] = np.asarray(cut(wq), complex)
         out[p + 'plain_coefficients'] = np.asarray(AC.PadeFit(z, f[i], greedy=False).coefficients,
                                                    complex)
+    # the quasiparticle equation on those continuations: the reference's
+    # root search (qp_equation.py:1353, `solve_qp_equation_pole_strength`) on
+    # the residual its qp_solve.py:529-558 forms, four states around the gap
+    from src.Solvers import qp_equation as QE                       # the reference
+    from mbptcode_amd.qp import imaginary_axis_sample_points
+    D = O.grid_factors(inst.V)
+    no = inst.nocc
+    qstates = [no - 2, no - 1, no, no + 1]
+    xc = np.array([-0.05, -0.05, 0.02, 0.02])
+    roots, zs, zfit, sfit = [], [], [], []
+    for i, p_state in enumerate(qstates):
+        zf, _ = imaginary_axis_sample_points(ax['pade_freq'], no, p_state, ax['mu'])
+        sf = O.gw_sigma_sum_over_poles(inst.X, D, inst.eps, no, [p_state], None, None,
+                                       z=zf)[0]
+        fit = AC.PadeFit(zf, sf, greedy=True)
+
+        def residual(w, fit=fit, p_state=p_state, x=xc[i]):
+            return w - inst.eps[p_state] - x - fit(np.array([w], dtype=complex))[0].real
+        root = QE.solve_qp_equation_pole_strength(residual, inst.eps[p_state])
+        roots.append(float(root))
+        zs.append(float(QE.pole_strength(residual, root)))
+        zfit.append(zf)
+        sfit.append(sf)
+    out['qp_states'] = np.array(qstates)
+    out['qp_eps'] = np.asarray(inst.eps, float)
+    out['qp_xc'] = xc
+    out['qp_zfit'] = np.array(zfit)
+    out['qp_sigma'] = np.array(sfit)
+    out['qp_roots'] = np.array(roots)
+    out['qp_z'] = np.array(zs)
     path = os.path.join(HERE, 'transform_weights.npz')
     np.savez(path, **out)
     print(f'wrote {path}: {len(out)} arrays')

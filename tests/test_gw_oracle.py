@@ -291,3 +291,19 @@ def test_pade_matches_the_reference_fixture():
         assert close(fit(wq), g[p + 'values'])
         assert close(PadeFit(z, f, max_order=9)(wq), g[p + 'values_cut9'])
         assert close(PadeFit(z, f, greedy=False).coefficients, g[p + 'plain_coefficients'])
+
+
+def test_quasiparticle_roots_match_the_reference_fixture():
+    """Roots and weights of the quasiparticle equation on the Pade fits of the
+    exact water-size self-energy, four states about the gap with a static
+    term, against the reference's own root search on the residual its
+    qp_solve.py:529-558 forms (tests/golden/transform_weights.npz;
+    qp_equation.py:1353-1389): to 2e-8 Ha, twice the root tolerance (they
+    agree to the bit), and Z to 1e-8."""
+    import os
+    from mbptcode_amd.qp import solve_qp_from_imaginary_axis
+    g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'transform_weights.npz'))
+    for i, p in enumerate(g['qp_states']):
+        w, z = solve_qp_from_imaginary_axis(g['qp_eps'][p], g['qp_xc'][i], g['qp_zfit'][i],
+                                            g['qp_sigma'][i], with_z=True)
+        assert abs(w - g['qp_roots'][i]) <= 2e-8 and abs(z - g['qp_z'][i]) <= 1e-8, p
