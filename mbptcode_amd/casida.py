@@ -171,6 +171,12 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     def act(V, W):
         nonlocal vectors
         p = V.shape[1]
+        if not bool(torch.any(W)):
+            # a cold start's pairs have no Y half: U1 = A V, U2 = B V from
+            # one apply of V alone
+            AZ, BZ = apply_ab(V.T.contiguous())
+            vectors += p
+            return AZ.T, BZ.T
         AB = apply_ab(torch.cat([V, W], dim=1).T.contiguous())
         AZ, BZ = AB[0].T, AB[1].T
         vectors += 2 * p
