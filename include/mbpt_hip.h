@@ -172,6 +172,21 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda,
                      const double *B, int ldb, double *C, int ldc,
                      double *work, long work_elems, void *stream);
 
+/* mbpt_gemm_tn_f64 without the work argument and without allocating: no
+ * counterpart in the reference.  It is for callers that cannot allocate
+ * device memory (code under hipGraph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  The chunk partials go to a fixed device array of
+ * mbpt_gemm_tn_f64_noalloc_max() doubles (32 MiB) that each GPU holds from
+ * the library's first use to the process's end; the kernels and the result
+ * are those of mbpt_gemm_tn_f64.  Products needing more partials than that
+ * (one 64-wide chunk with a * b above it) return MBPT_ERR_VALUE.  Every call
+ * on one GPU shares that GPU's array, so calls that overlap on one GPU (two
+ * streams, two threads) race: order them on one stream. */
+long mbpt_gemm_tn_f64_noalloc_max(void);
+int mbpt_gemm_tn_f64_noalloc(int n, int a, int b, const double *A, int lda,
+                             const double *B, int ldb, double *C, int ldc,
+                             void *stream);
+
 /* ------------------------------------------------------------------------
  * Batched small symmetric eigensolver (FP64): the Rayleigh-Ritz step of many
  * Davidson runs at once, the batched counterpart of the m x m eigh inside

@@ -8,6 +8,8 @@
 #include "common.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace mbpt {
 
@@ -92,6 +94,56 @@ int tn_chunk(int n, int a, int b) {
   const int c = (int)((n + nc - 1) / nc);
   return (c + 15) / 16 * 16;
 }
+
+// The two launches of C = A^T B with the chunk partials in `work`.
+int tn_launch(int n, int a, int b, const double *A, int lda, const double *B, int ldb,
+              double *C, int ldc, double *work, long work_elems, hipStream_t st) {
+  const int chunk = tn_chunk(n, a, b);
+  const int nc = (n + chunk - 1) / chunk;
+  MBPT_REQUIRE(work_elems >= (long)nc * a * b, "mbpt_gemm_tn_f64: work buffer too small");
+  const int nta = (a + kTile - 1) / kTile, ntb = (b + kTile - 1) / kTile;
+  const int nblk = pad_grid((long)nta * ntb * nc);
+  k_tn_partial<<<nblk, 256, 0, st>>>(n, a, b, chunk, A, lda, B, ldb, work, nta, ntb, nc,
+                                     nblk);
+  int rc = launch_status("k_tn_partial");
+  if (rc) return rc;
+  const long count = (long)a * b;
+  k_tn_sum<<<(int)std::min<long>((count + 15) / 16, 8192L), 256, 0, st>>>(a, b, nc, work,
+                                                                          C, ldc);
+  return launch_status("k_tn_sum");
+}
+
+// Chunk partials of the allocation-free entry point.  The chunking above
+// makes nc * a * b at most 1024 * 64 * 64 whenever the output has at most
+// 1024 tiles of 64 x 64 (a, b <= 2048 both), so this array covers every such
+// product; larger ones (one chunk, a * b above the size) are refused.  Each
+// GPU holds its own copy from the library's load to the process's end.
+constexpr long kTnScratchElems = 1024L * kTile * kTile;  // 4 Mi doubles, 32 MiB
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kTnScratchElems];
+
+namespace {
+// Device address of the calling thread's current GPU's copy of `scratch`,
+// looked up once per device.  The array's host-side name is not a device
+// address: only this pointer may reach a kernel.
+int tn_scratch(double **out) {
+  int dev = 0;
+  MBPT_HIP_TRY(hipGetDevice(&dev));
+  MBPT_REQUIRE(dev >= 0 && dev < kMaxDevices, "mbpt_gemm_tn_f64_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    MBPT_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    MBPT_REQUIRE(p != nullptr, "mbpt_gemm_tn_f64_noalloc: no device address for the scratch array");
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return MBPT_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -109,20 +161,28 @@ int mbpt_gemm_tn_f64(int n, int a, int b, const double *A, int lda, const double
   if (a == 0 || b == 0) return MBPT_OK;
   MBPT_REQUIRE(A && B && C && work, "mbpt_gemm_tn_f64: NULL argument");
   MBPT_REQUIRE(lda >= a && ldb >= b && ldc >= b, "mbpt_gemm_tn_f64: leading dimension too small");
-  hipStream_t st = (hipStream_t)stream;
-  const int chunk = tn_chunk(n, a, b);
-  const int nc = (n + chunk - 1) / chunk;
-  MBPT_REQUIRE(work_elems >= (long)nc * a * b, "mbpt_gemm_tn_f64: work buffer too small");
-  const int nta = (a + kTile - 1) / kTile, ntb = (b + kTile - 1) / kTile;
-  const int nblk = pad_grid((long)nta * ntb * nc);
-  k_tn_partial<<<nblk, 256, 0, st>>>(n, a, b, chunk, A, lda, B, ldb, work, nta, ntb, nc,
-                                     nblk);
-  int rc = launch_status("k_tn_partial");
+  return tn_launch(n, a, b, A, lda, B, ldb, C, ldc, work, work_elems, (hipStream_t)stream);
+}
+
+long mbpt_gemm_tn_f64_noalloc_max(void) { return kTnScratchElems; }
+
+int mbpt_gemm_tn_f64_noalloc(int n, int a, int b, const double *A, int lda,
+                             const double *B, int ldb, double *C, int ldc, void *stream) {
+  MBPT_REQUIRE(n > 0 && a >= 0 && b >= 0, "mbpt_gemm_tn_f64_noalloc: n must be positive, a and b non-negative");
+  if (a == 0 || b == 0) return MBPT_OK;
+  MBPT_REQUIRE(A && B && C, "mbpt_gemm_tn_f64_noalloc: NULL argument");
+  MBPT_REQUIRE(lda >= a && ldb >= b && ldc >= b, "mbpt_gemm_tn_f64_noalloc: leading dimension too small");
+  const long need = mbpt_gemm_tn_f64_work(n, a, b);
+  if (need > kTnScratchElems) {
+    set_error("mbpt_gemm_tn_f64_noalloc: the product needs " + std::to_string(need) +
+              " doubles of partials, above the fixed scratch of " +
+              std::to_string(kTnScratchElems) + " (a * b too large)");
+    return MBPT_ERR_VALUE;
+  }
+  double *work = nullptr;
+  const int rc = tn_scratch(&work);
   if (rc) return rc;
-  const long count = (long)a * b;
-  k_tn_sum<<<(int)std::min<long>((count + 15) / 16, 8192L), 256, 0, st>>>(a, b, nc, work,
-                                                                          C, ldc);
-  return launch_status("k_tn_sum");
+  return tn_launch(n, a, b, A, lda, B, ldb, C, ldc, work, kTnScratchElems, (hipStream_t)stream);
 }
 
 }  // extern "C"
