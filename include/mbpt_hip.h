@@ -84,6 +84,13 @@ int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream);
 /* Device bytes the handle holds (`apply_AB.held_bytes`, block_actions.py:705). */
 size_t mbpt_sigma_held_bytes(const mbpt_sigma *h);
 
+/* Per-apply workspace in bytes that an apply of k vectors needs (with_b != 0:
+ * mbpt_sigma_apply_ab); the workspace grows to it on first use and is kept.
+ * The counterpart of the reference's tile budget, the S/T/Tb/U row tiles
+ * sized by `tile_memory_gb` (block_actions.py:526-543): a caller bounds the
+ * workspace by applying at most the k this allows per call.  Host only. */
+size_t mbpt_sigma_workspace_bytes(const mbpt_sigma *h, int k, int with_b);
+
 /* Per-stage device timing, the counterpart of the action's `piece_clock`
  * (block_actions.py:56-70).  enable != 0 starts (and zeroes) accumulation:
  * every later apply records HIP events around its four launches on its

@@ -22,7 +22,7 @@ GW_MAX_STATES = 4096
 #: Every symbol include/mbpt_hip.h declares.
 EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_sigma_apply', 'mbpt_sigma_apply_ab', 'mbpt_sigma_screened_diag',
-           'mbpt_sigma_held_bytes', 'mbpt_sigma_profile',
+           'mbpt_sigma_held_bytes', 'mbpt_sigma_workspace_bytes', 'mbpt_sigma_profile',
            'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
            'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
            'mbpt_chi0_sigma', 'mbpt_chi0_sigma_profile_read', 'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
@@ -49,6 +49,7 @@ _SIGNATURES = {
     'mbpt_sigma_apply_ab': (_i, [_vp, _i, _vp, _vp, _vp, _vp]),
     'mbpt_sigma_screened_diag': (_i, [_vp, _vp, _vp]),
     'mbpt_sigma_held_bytes': (ctypes.c_size_t, [_vp]),
+    'mbpt_sigma_workspace_bytes': (ctypes.c_size_t, [_vp, _i, _i]),
     'mbpt_sigma_profile': (_i, [_vp, _i]),
     'mbpt_sigma_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
     'mbpt_sigma_destroy': (_i, [_vp]),

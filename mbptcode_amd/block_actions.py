@@ -21,7 +21,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .constants import ISDF_TILE_GB, KAPPA
+from .constants import BYTES_PER_GB, ISDF_TILE_GB, KAPPA
 from .orbital_spaces import bse_pair_diagonal
 
 COMM_KINDS = ('lockstep', 'gather', 'reduce_grid', 'reduce_pairs')
@@ -64,12 +64,15 @@ class SigmaOperator:
     """A z for batches of Tamm-Dancoff trial vectors, on one GPU.
 
     eps (norb,), X (M, norb) collocation, W (M, M) screened and V (M, M) bare
-    interaction on the grid, kappa the Hartree prefactor, precision 'fp64',
-    'fp32' or 'bf16x3' (the bulk GEMMs; d z and the output stay FP64).
+    interaction on the grid, kappa the Hartree prefactor, precision 'fp64'
+    (the default, as the reference computes), 'fp32' or 'bf16x3' (the bulk
+    GEMMs; d z and the output stay FP64).  workspace_bytes bounds the
+    per-apply device workspace: a batch of trial vectors whose workspace
+    would exceed it is applied in consecutive slices (None: no bound).
     """
 
-    def __init__(self, eps, nocc, X, W, V, kappa=2.0, precision='fp32',
-                 device=None):
+    def __init__(self, eps, nocc, X, W, V, kappa=2.0, precision='fp64',
+                 device=None, workspace_bytes=None):
         dev = _device() if device is None else torch.device(device)
         eps = np.ascontiguousarray(eps, dtype=np.float64)
         X = _as_device(X, dev)
@@ -106,6 +109,32 @@ class SigmaOperator:
         self._h = h
         self.applied_vectors = 0
         self.applied_calls = 0
+        self.workspace_bytes = workspace_bytes
+        self._kcap = {}
+
+    def vectors_per_apply(self, k, with_b=False):
+        """Largest slice of a k-vector batch whose workspace fits
+        `workspace_bytes` (at least 1)."""
+        if self.workspace_bytes is None or k <= 1:
+            return max(k, 1)
+        key = (k, bool(with_b))
+        if key not in self._kcap:
+            fits = lambda j: self._lib.mbpt_sigma_workspace_bytes(self._h, j, int(with_b)) \
+                <= self.workspace_bytes
+            if fits(k):
+                cap = k
+            else:
+                lo, hi = 1, k          # fits(hi) is False; find the largest j that fits
+                while hi - lo > 1:
+                    mid = (lo + hi) // 2
+                    lo, hi = (mid, hi) if fits(mid) else (lo, mid)
+                cap = lo
+            self._kcap[key] = cap
+        return self._kcap[key]
+
+    def workspace_for(self, k, with_b=False):
+        """Device bytes of the per-apply workspace an apply of k vectors uses."""
+        return int(self._lib.mbpt_sigma_workspace_bytes(self._h, k, int(with_b)))
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -133,9 +162,14 @@ class SigmaOperator:
                              f'the {self.npair} pairs')
         if out is None:
             out = torch.empty_like(z)
-        _lib.check(self._lib.mbpt_sigma_apply(self._h, k, _lib.dptr(z),
-                                               _lib.dptr(out), _lib.stream_ptr()),
-                   'mbpt_sigma_apply')
+        zf, of = z.reshape(k, self.npair), out.view(k, self.npair)
+        step = self.vectors_per_apply(k)
+        for j in range(0, k, step):
+            kj = min(step, k - j)
+            _lib.check(self._lib.mbpt_sigma_apply(self._h, kj, _lib.dptr(zf[j:j + kj]),
+                                                   _lib.dptr(of[j:j + kj]),
+                                                   _lib.stream_ptr()),
+                       'mbpt_sigma_apply')
         self.applied_vectors += k
         self.applied_calls += 1
         return out
@@ -153,9 +187,14 @@ class SigmaOperator:
                              f'the {self.npair} pairs')
         out_a = torch.empty_like(z) if out_a is None else out_a
         out_b = torch.empty_like(z) if out_b is None else out_b
-        _lib.check(self._lib.mbpt_sigma_apply_ab(self._h, k, _lib.dptr(z), _lib.dptr(out_a),
-                                                  _lib.dptr(out_b), _lib.stream_ptr()),
-                   'mbpt_sigma_apply_ab')
+        zf = z.reshape(k, self.npair)
+        af, bf = out_a.view(k, self.npair), out_b.view(k, self.npair)
+        step = self.vectors_per_apply(k, with_b=True)
+        for j in range(0, k, step):
+            kj = min(step, k - j)
+            _lib.check(self._lib.mbpt_sigma_apply_ab(
+                self._h, kj, _lib.dptr(zf[j:j + kj]), _lib.dptr(af[j:j + kj]),
+                _lib.dptr(bf[j:j + kj]), _lib.stream_ptr()), 'mbpt_sigma_apply_ab')
         self.applied_vectors += k
         self.applied_calls += 1
         return out_a, out_b
@@ -190,13 +229,15 @@ class SigmaOperator:
         return out
 
 
-def grid_block_action(eps, nocc, X, W, V, kappa=2.0, precision='fp32',
-                      device=None, tda=True):
+def grid_block_action(eps, nocc, X, W, V, kappa=2.0, precision='fp64',
+                      device=None, tda=True, tile_memory_gb=None):
     """(apply_AB, diag_d) from grid-gauge W and V: apply_AB(z) -> (A z, None)
     with z (k, nocc, nvir), NumPy or device FP64; with tda=False the full
-    Casida action, (A z, B z)."""
+    Casida action, (A z, B z).  precision 'fp64' (default), 'fp32' or
+    'bf16x3'; tile_memory_gb bounds the per-apply workspace (None: no bound)."""
+    ws = None if tile_memory_gb is None else int(tile_memory_gb * BYTES_PER_GB)
     op = SigmaOperator(eps, nocc, X, W, V, kappa=kappa, precision=precision,
-                       device=device)
+                       device=device, workspace_bytes=ws)
     return _wrap(op, tda=tda)
 
 
@@ -236,15 +277,20 @@ def _wrap(op, tda=True):
 
 def isdf_block_action(lr_solver, nocc, lBSE, W_aux, isdf_factors,
                       tile_memory_gb=ISDF_TILE_GB, spin='singlet', comm=None,
-                      tda=False, precision='fp32'):
+                      tda=False, precision='fp64'):
     """The reference's `isdf_block_action` (block_actions.py:365) on the GPU.
 
     lr_solver: anything with `.eps` and `.spin_mode`.  isdf_factors: (X_mo, D)
     or (X_mo, D, ...).  W_aux: screened interaction in D's gauge, None for
     TDHF (bare exchange), unused when lBSE is False (RPA).  tda=False gives
     the full Casida action, apply_AB(z) -> (A z, B z) (the B block,
-    block_actions.py:632-652, in the handle's precision).  comm: one GPU only, so None or a one-rank communicator.
-    precision: 'fp64', 'fp32' or 'bf16x3' for the bulk GEMMs.
+    block_actions.py:632-652, in the handle's precision).  comm: one GPU
+    only, so None or a one-rank communicator.  tile_memory_gb: the bound on
+    the action's per-apply scratch, as in the reference (whose row tiles it
+    sizes, block_actions.py:526-543); here trial vectors are applied in slices
+    whose device workspace fits it.  precision: 'fp64' (the default, the
+    reference's arithmetic), or 'fp32' / 'bf16x3' for the bulk GEMMs as an
+    explicit opt-in.
     """
     if getattr(lr_solver, 'spin_mode', 'restricted') == 'unrestricted':
         raise NotImplementedError('The ISDF Davidson route is restricted-spin only.')
@@ -283,11 +329,12 @@ def isdf_block_action(lr_solver, nocc, lBSE, W_aux, isdf_factors,
     else:
         Zt = torch.zeros_like(V)
     return grid_block_action(eps, nocc, X_mo, Zt, V, kappa=KAPPA[spin],
-                             precision=precision, device=dev, tda=tda)
+                             precision=precision, device=dev, tda=tda,
+                             tile_memory_gb=tile_memory_gb)
 
 
 def block_action(lr_solver, nocc, polarizability, W_aux, isdf_factors,
-                 spin='singlet', comm=None, tda=False, precision='fp32'):
+                 spin='singlet', comm=None, tda=False, precision='fp64'):
     """Mode dispatch (block_actions.py:73-96): 'RPA', 'TDHF' or 'BSE' on ISDF
     factors.  The DF route (isdf_factors None) needs pyscf's three-index
     factor and is not part of this package."""

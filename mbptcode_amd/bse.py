@@ -21,6 +21,7 @@ import time
 
 import numpy as np
 
+from .constants import BSE_DAVIDSON_CONV_TOL
 from .isdf_fit import unpack_factors
 
 
@@ -33,8 +34,9 @@ class _Spectrum:
 
 
 def solve_bse_isdf(eps, nocc, factors, V=None, qp=None, xc_diagonal=None, W_aux=None,
-                   tda=False, spin='singlet', nroots=5, conv_tol=1e-6, max_cycle=150,
-                   preconditioner='screened', precision='fp32', probe=False, guess=None,
+                   tda=False, spin='singlet', nroots=5, conv_tol=BSE_DAVIDSON_CONV_TOL,
+                   max_cycle=150, preconditioner='screened', precision='fp64', probe=False,
+                   guess=None,
                    refuse_unconverged=False, ntau=None):
     """(omega, X, Y, info) of the `nroots` lowest BSE excitations.
 

@@ -10,10 +10,16 @@ BYTES_PER_GB = 1e9
 #: Hartree-term prefactor per spin channel: singlet 2, triplet 0 (constants.py:287).
 KAPPA = {'singlet': 2.0, 'triplet': 0.0}
 
-#: Grid-row tile budget of the CPU kernels, in GB (constants.py:292). The GPU
-#: kernels tile by their own register/LDS shapes; the argument is accepted
-#: for interface parity and checked, nothing more.
+#: Scratch budget of the block action, in GB (constants.py:292).  The
+#: reference sizes its grid-row tiles by it; the GPU action applies trial
+#: vectors in slices whose device workspace fits it.
 ISDF_TILE_GB = 4.0
+
+#: The BSE Davidson's roots and residual tolerance (constants.py:649-650), and
+#: its cycle cap, three trial-space cycles of 50 (constants.py:1340).
+BSE_DAVIDSON_NROOTS = 5
+BSE_DAVIDSON_CONV_TOL = 1e-8
+BSE_FORCE_MAX_CYCLE = 150
 
 #: Gauss-Legendre frequencies of the dRPA energy quadrature (constants.py:469).
 RPA_ENERGY_NFREQ = 40

@@ -741,19 +741,68 @@ int ensure_buf(mbpt_sigma *h, int q, size_t bytes) {
   return MBPT_OK;
 }
 
-int ensure_ws(mbpt_sigma *h, int k, int nsplit, int nq, int nqu, int ldr, bool want_b) {
+// The work decomposition of one apply of k vectors: K splits of the
+// exchange, the tail's K chunks, the Hartree row chunks.
+struct ApplyPlan {
+  int nsplit, ldr, nq, kchunk, nqu, hchunk;
+};
+
+ApplyPlan plan_for(const mbpt_sigma *h, int k) {
+  ApplyPlan a;
+  const int M = h->M, no = h->no, nv = h->nv;
+  a.nsplit = nsplit_for(h, k);
+  const int rows = k * no;
+  const bool km = h->esz == 4 && h->prec != MBPT_BF16X3;  // FP32: K-major head and tail
+  a.ldr = (rows + 127) / 128 * 128;
+  // tail: K = M split in nq chunks of kchunk (multiples of 16), enough
+  // workgroups for the skinny output
+  if (h->prec == MBPT_BF16X3) {
+    const int tiles = (a.ldr / 128) * (h->nvq / 128);
+    a.nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp32 / 64));
+    a.kchunk = ((h->Mp32 + a.nq - 1) / a.nq + 31) / 32 * 32;
+    a.nq = (h->Mp32 + a.kchunk - 1) / a.kchunk;
+  } else if (km) {
+    const int tiles = (a.ldr / 128) * ((nv + 127) / 128);
+    a.nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp / 64));
+    a.kchunk = ((h->Mp + a.nq - 1) / a.nq + 15) / 16 * 16;
+    a.nq = (h->Mp + a.kchunk - 1) / a.kchunk;
+  } else {
+    a.nq = nq_for(rows, nv, M);
+    a.kchunk = ((M + a.nq - 1) / a.nq + 15) / 16 * 16;
+    a.nq = (M + a.kchunk - 1) / a.kchunk;
+  }
+  // Hartree u = p V streamed: 256-column blocks x nqu row chunks
+  a.nqu = std::max(1, std::min((512 + (M + 255) / 256 - 1) / ((M + 255) / 256), (M + 63) / 64));
+  a.hchunk = (M + a.nqu - 1) / a.nqu;
+  a.nqu = (M + a.hchunk - 1) / a.hchunk;
+  return a;
+}
+
+// Bytes of each workspace slot for an apply of k vectors.  The initializer
+// must name every slot: its length is checked against WS_N at compile time.
+void ws_sizes(const mbpt_sigma *h, int k, const ApplyPlan &a, bool want_b, size_t *out) {
   const size_t e = h->esz, M = h->M, no = h->no, nv = h->nv;
   const bool km = h->esz == 4;   // K-major head/tail operands (FP32 working precision)
-  const size_t sizes[WS_N] = {e * k * no * M, e * k * M, e * (size_t)nqu * k * M,
-                              e * (size_t)nsplit * k * h->nopad * M,
-                              km ? 0 : e * k * no * M,
-                              e * (size_t)nq * k * no * nv,
-                              km ? 4 * (size_t)std::max(h->nvp32, h->Mp32) * ldr : 0,
-                              want_b ? e * (size_t)nsplit * k * h->nopad * M : 0,
-                              // split BF16, B block: zXv in fragment order
-                              want_b && h->prec == MBPT_BF16X3
-                                  ? 16 * (size_t)k * ((M + 31) / 32) * bf16_shape(h->no).spp
-                                  : 0};
+  const size_t sizes[] = {e * k * no * M,                                   // WS_ZXV
+                          e * k * M,                                        // WS_P
+                          e * (size_t)a.nqu * k * M,                        // WS_U
+                          e * (size_t)a.nsplit * k * h->nopad * M,          // WS_TPART
+                          km ? 0 : e * k * no * M,                          // WS_TC
+                          e * (size_t)a.nq * k * no * nv,                   // WS_EP
+                          km ? 4 * (size_t)std::max(h->nvp32, h->Mp32) * a.ldr : 0,  // WS_ZT
+                          want_b ? e * (size_t)a.nsplit * k * h->nopad * M : 0,       // WS_UPART
+                          // split BF16, B block: zXv in fragment order        // WS_ZFRAG
+                          want_b && h->prec == MBPT_BF16X3
+                              ? 16 * (size_t)k * ((M + 31) / 32) * bf16_shape(h->no).spp
+                              : 0};
+  static_assert(sizeof(sizes) / sizeof(sizes[0]) == WS_N,
+                "every workspace slot needs its size here");
+  for (int q = 0; q < WS_N; ++q) out[q] = sizes[q];
+}
+
+int ensure_ws(mbpt_sigma *h, int k, const ApplyPlan &a, bool want_b) {
+  size_t sizes[WS_N];
+  ws_sizes(h, k, a, want_b, sizes);
   for (int q = 0; q < WS_N; ++q) {
     int rc = ensure_buf(h, q, sizes[q]);
     if (rc) return rc;
@@ -868,34 +917,12 @@ void launch_hartree_u(int M, int k, int chunk, int nqu, const T *p, const T *V, 
 template <typename T>
 int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hipStream_t st) {
   const int M = h->M, no = h->no, nv = h->nv;
-  const int nsplit = nsplit_for(h, k);
-  const int rows = k * no;
+  const ApplyPlan plan = plan_for(h, k);
+  const int nsplit = plan.nsplit, rows = k * no, ldr = plan.ldr;
+  const int nq = plan.nq, kchunk = plan.kchunk, nqu = plan.nqu, hchunk = plan.hchunk;
   const bool km = sizeof(T) == 4;   // FP32 working precision: K-major head and tail
-  const int ldr = (rows + 127) / 128 * 128;
-  // tail: K = M split in nq chunks of kchunk (multiples of 16), enough
-  // workgroups for the skinny output
-  int nq, kchunk;
   const bool b3 = h->prec == MBPT_BF16X3;   // split-BF16 head and tail GEMMs
-  if (b3) {
-    const int tiles = (ldr / 128) * (h->nvq / 128);
-    nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp32 / 64));
-    kchunk = ((h->Mp32 + nq - 1) / nq + 31) / 32 * 32;
-    nq = (h->Mp32 + kchunk - 1) / kchunk;
-  } else if (km) {
-    const int tiles = (ldr / 128) * ((nv + 127) / 128);
-    nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp / 64));
-    kchunk = ((h->Mp + nq - 1) / nq + 15) / 16 * 16;
-    nq = (h->Mp + kchunk - 1) / kchunk;
-  } else {
-    nq = nq_for(rows, nv, M);
-    kchunk = ((M + nq - 1) / nq + 15) / 16 * 16;
-    nq = (M + kchunk - 1) / kchunk;
-  }
-  // Hartree u = p V streamed: 256-column blocks x nqu row chunks
-  int nqu = std::max(1, std::min((512 + (M + 255) / 256 - 1) / ((M + 255) / 256), (M + 63) / 64));
-  const int hchunk = (M + nqu - 1) / nqu;
-  nqu = (M + hchunk - 1) / hchunk;
-  int rc = ensure_ws(h, k, nsplit, nq, nqu, ldr, Bz != nullptr);
+  int rc = ensure_ws(h, k, plan, Bz != nullptr);
   if (rc) return rc;
   T *zXv = (T *)h->ws[WS_ZXV], *p = (T *)h->ws[WS_P], *u = (T *)h->ws[WS_U];
   T *Tpart = (T *)h->ws[WS_TPART], *Tc = (T *)h->ws[WS_TC], *Ep = (T *)h->ws[WS_EP];
@@ -1242,6 +1269,15 @@ int mbpt_sigma_screened_diag(mbpt_sigma *h, double *out_dev, void *stream) {
   hipStream_t st = (hipStream_t)stream;
   return h->prec == MBPT_FP64 ? sdiag_impl<double>(h, out_dev, st)
                               : sdiag_impl<float>(h, out_dev, st);
+}
+
+size_t mbpt_sigma_workspace_bytes(const mbpt_sigma *h, int k, int with_b) {
+  if (!h || k <= 0) return 0;
+  size_t sizes[WS_N];
+  ws_sizes(h, k, plan_for(h, k), with_b != 0, sizes);
+  size_t total = 0;
+  for (int q = 0; q < WS_N; ++q) total += sizes[q];
+  return total;
 }
 
 size_t mbpt_sigma_held_bytes(const mbpt_sigma *h) {

@@ -761,7 +761,7 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
     return e[:nroots], Xn[:, :nroots], Yn[:, :nroots]
 
 
-def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
+def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='RPA',
                           W_aux=None, orbsym=None, isdf_factors=None, stats=None,
                           spin='singlet', comm=None, timings=None,
                           refuse_unconverged=False, max_memory=None,
@@ -769,16 +769,18 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='BSE',
                           one_body=None, tracked=None, probe=False, nroots=5,
                           conv_tol=1e-8, max_cycle=150, guess_factor=2,
                           preconditioner=DAVIDSON_PRECONDITIONER,
-                          precision='fp32', refine=True, learned=None):
+                          precision='fp64', refine=True, learned=None):
     """The matrix-free Davidson entry (LR/davidson.py:252) on the GPU: the
     Tamm-Dancoff solve (tda=True) or the full Casida one (tda=False, the
     paired loop of `casida.casida_davidson`).
 
     `numerics` may carry a `bse` section with nroots, conv_tol, max_cycle,
     guess_factor and preconditioner (numerics.py:241-286); otherwise the
-    keyword values (the reference defaults) are used.  precision: the block
-    action's; with refine (default) a lower-precision solve is re-checked and
-    polished on an FP64 action.  probe (full solve only): True or 'sign' runs
+    keyword values (the reference defaults) are used.  polarizability 'RPA'
+    (the reference's default), 'TDHF' or 'BSE'.  precision: the block
+    action's, 'fp64' by default as the reference computes; 'fp32' or 'bf16x3'
+    are an explicit opt-in, and with refine (default) such a solve is
+    re-checked and polished on an FP64 action.  probe (full solve only): True or 'sign' runs
     the (A - B) probe after the solve on the FP64 action (`probe_after_solve`),
     puts min eig(A - B) in stats['min_eig_amb'] and refuses the roots of an
     unstable reference (UnstableReference).  Returns (omega, X, Y).

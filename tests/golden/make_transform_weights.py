@@ -2,6 +2,8 @@
 transform weights (src/Base/utils/grids/time_frequency.py:282-380, the
 `TimeFrequencyGrid.minimax_split` path at :498-555) on the axes the static W
 and the space-time self-energy use for the water- and benzene-size instances,
+the regularized 20-point static and dRPA-energy grids of the large and
+learned-preconditioner instances,
 with those axes' fit ranges (:401-428), point count (:233-254) and the
 minimax and Gauss-Legendre frequency grids (grids.py:107-126, :161-170), and
 the Thiele-Pade continuation of an exact self-energy
@@ -74,6 +76,34 @@ def main():
             out[p + 'range'] = np.array(rng, float)
             out[p + 'weights'] = np.asarray(W, float)
             out[p + 'fit_error'] = np.array(err, float)
+    # the regularized branch (20 time points and more, time_frequency.py:
+    # 325-331) on the static W of the large and learned-preconditioner
+    # instances and on their dRPA energy grids (Gauss-Legendre frequencies,
+    # w0 = 0.5 gap; gradients/rpa/ground_state.py:92-106)
+    for cfg in (2, 4):
+        inst = synth.config(cfg)
+        sg = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+        e_min, e_max = sg.meta['e_min'], sg.meta['e_max']
+        ref = TF.TimeFrequencyGrid.minimax_split(inst.ntau, e_min, e_max, np.array([0.0]),
+                                                 np.array([1.0]))
+        p = f'cfg{cfg}_static_'
+        out[p + 'range'] = np.array([e_min, e_max])
+        out[p + 'ntau'] = np.array(inst.ntau)
+        out[p + 'tau'] = np.asarray(ref.tau_points, float)
+        out[p + 'tau_weights'] = np.asarray(ref.tau_weights, float)
+        out[p + 'cosft_wt'] = np.asarray(ref.cosft_wt, float)
+        nu, wt = RG.gauss_legendre_grid(40, RG.gap_scaled_w0(inst.eps, inst.nocc))
+        ref = TF.TimeFrequencyGrid.minimax_split(inst.ntau, e_min, e_max, np.asarray(nu),
+                                                 np.asarray(wt), with_sine=False,
+                                                 with_inverse=False)
+        p = f'cfg{cfg}_rpa_'
+        out[p + 'range'] = np.array([e_min, e_max])
+        out[p + 'ntau'] = np.array(inst.ntau)
+        out[p + 'freq_points'] = np.asarray(nu, float)
+        out[p + 'freq_weights'] = np.asarray(wt, float)
+        out[p + 'tau'] = np.asarray(ref.tau_points, float)
+        out[p + 'tau_weights'] = np.asarray(ref.tau_weights, float)
+        out[p + 'cosft_wt'] = np.asarray(ref.cosft_wt, float)
     # Thiele-Pade continuation of the exact GW@RPA self-energy of the
     # water-size instance at the Pade frequencies (the oracle's pole sum as
     # input), and the fraction's values on the real axis

@@ -1,0 +1,33 @@
+"""The drop-in entry points keep the reference's defaults: FP64 arithmetic
+(block_actions.py:365, which has no precision switch), the Davidson's 'RPA'
+polarizability (LR/davidson.py:252-253), the BSE residual tolerance
+BSE_DAVIDSON_CONV_TOL = 1e-8 (constants.py:650, read through
+numerics.py:241-266) and the ISDF tile budget of 4 GB (constants.py:292).
+CPU only: the signatures are read, nothing runs on a device."""
+import inspect
+
+
+def defaults(fn):
+    return {k: p.default for k, p in inspect.signature(fn).parameters.items()
+            if p.default is not inspect.Parameter.empty}
+
+
+def test_block_action_defaults_are_the_reference_arithmetic():
+    from mbptcode_amd import block_actions as B
+    d = defaults(B.isdf_block_action)
+    assert d['precision'] == 'fp64'
+    assert d['tile_memory_gb'] == 4.0 and d['spin'] == 'singlet' and d['tda'] is False
+    assert defaults(B.block_action)['precision'] == 'fp64'
+    assert defaults(B.grid_block_action)['precision'] == 'fp64'
+    assert defaults(B.SigmaOperator)['precision'] == 'fp64'
+
+
+def test_solver_defaults():
+    from mbptcode_amd.bse import solve_bse_isdf
+    from mbptcode_amd.davidson import solve_casida_davidson
+    d = defaults(solve_casida_davidson)
+    assert d['polarizability'] == 'RPA'
+    assert d['precision'] == 'fp64'
+    assert d['nroots'] == 5 and d['conv_tol'] == 1e-8 and d['tda'] is False
+    d = defaults(solve_bse_isdf)
+    assert d['conv_tol'] == 1e-8 and d['precision'] == 'fp64' and d['nroots'] == 5
