@@ -132,10 +132,36 @@ int mbpt_chi0_build(mbpt_chi0 *h, int ntau, const double *tau_host,
                     const double *weight_host, double *out_dev, int accumulate,
                     void *stream);
 
+/* The per-time-point terms of a frequency sweep, kept in the handle:
+ *
+ *   P(t) = -2 Go(t) o Gv(t),  t < ntau  (proj(tau_t) in the grid gauge)
+ *
+ * each built once, so that mbpt_chi0_transform can form chi0 at any number
+ * of frequencies from them: the reference's `chi0_tau_terms` and
+ * `polarizability_projected_sweep` (space_time.py:533-560, :634-651), whose
+ * terms its frequency blocks accumulate as they are built (:854-905).  Held
+ * until the next call: ntau M^2 floats (FP32 handles, the tiles padded to
+ * 128) or doubles (FP64).  ntau <= MBPT_CHI0_MAX_TAU. */
+int mbpt_chi0_terms(mbpt_chi0 *h, int ntau, const double *tau_host, void *stream);
+
+/* out_dev[f] = [accumulate ? out_dev[f] : 0] + sum_t weight_host[f][t] P(t)
+ * for f < nfreq: weight_host (nfreq, ntau) row-major (cosft_wt of a
+ * TimeFrequencyGrid, or a unit row for one term), out_dev (nfreq, M, M) FP64,
+ * symmetric, summed in FP64: chi0(i w_f) for all frequencies in one pass
+ * over the terms, the counterpart of `chi0_imaginary_frequency`
+ * (space_time.py:607-631).  Needs mbpt_chi0_terms first. */
+int mbpt_chi0_transform(mbpt_chi0 *h, int nfreq, const double *weight_host,
+                        double *out_dev, int accumulate, void *stream);
+
+/* Device milliseconds and calls of mbpt_chi0_transform since
+ * mbpt_chi0_profile (HIP events, waited for). */
+int mbpt_chi0_transform_profile_read(const mbpt_chi0 *h, double *ms, long *calls);
+
 /* Device timing of the fused chi0 launches (HIP events on the build's
  * stream, waited for): accumulated milliseconds, launches, and their
  * algorithmic flops, sum over time points of M(M+1)(nocc + nvir + 1)
- * for the upper triangle (with diagonal) each launch computes. */
+ * for the upper triangle (with diagonal) each launch computes.  Term builds
+ * (mbpt_chi0_terms) count here too, one launch per call. */
 int mbpt_chi0_profile(mbpt_chi0 *h, int enable);
 int mbpt_chi0_profile_read(const mbpt_chi0 *h, double *ms, long *launches,
                            double *flops);

@@ -24,7 +24,8 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_sigma_apply', 'mbpt_sigma_apply_ab', 'mbpt_sigma_screened_diag',
            'mbpt_sigma_held_bytes', 'mbpt_sigma_workspace_bytes', 'mbpt_sigma_profile',
            'mbpt_sigma_profile_read', 'mbpt_sigma_destroy', 'mbpt_chi0_create',
-           'mbpt_chi0_build', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
+           'mbpt_chi0_build', 'mbpt_chi0_terms', 'mbpt_chi0_transform',
+           'mbpt_chi0_transform_profile_read', 'mbpt_chi0_profile', 'mbpt_chi0_profile_read',
            'mbpt_chi0_sigma', 'mbpt_chi0_sigma_profile_read', 'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
            'mbpt_gemm_tn_f64_noalloc_max', 'mbpt_gemm_tn_f64_noalloc',
            'mbpt_mlp_nparams', 'mbpt_mlp_create', 'mbpt_mlp_get_params',
@@ -56,6 +57,9 @@ _SIGNATURES = {
     'mbpt_chi0_create': (_i, [ctypes.POINTER(_vp), _i, _i, _i, _i, _vp, _dp, _d,
                               _vp]),
     'mbpt_chi0_build': (_i, [_vp, _i, _dp, _dp, _vp, _i, _vp]),
+    'mbpt_chi0_terms': (_i, [_vp, _i, _dp, _vp]),
+    'mbpt_chi0_transform': (_i, [_vp, _i, _dp, _vp, _i, _vp]),
+    'mbpt_chi0_transform_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long)]),
     'mbpt_chi0_profile': (_i, [_vp, _i]),
     'mbpt_chi0_profile_read': (_i, [_vp, _dp, ctypes.POINTER(ctypes.c_long), _dp]),
     'mbpt_chi0_sigma': (_i, [_vp, _i, _dp, _vp, _i, _ip, _vp, _vp]),

@@ -11,6 +11,7 @@
 // is written once; off-diagonal tiles are mirrored into the lower triangle.
 #include "common.h"
 #include "chi0_f32.h"
+#include "chi0_transform.h"
 #include "gw_sigma_f32.h"
 
 #include <algorithm>
@@ -149,6 +150,14 @@ struct mbpt_chi0 {
   size_t cap_xs, cap_gwpart, cap_cols;
   double gw_ms, gw_flops;
   long gw_launches;
+  // kept per-time-point terms P(t) = -2 Go(t) o Gv(t) for the frequency
+  // transform: FP32 in the main kernel's tile layout (Mp^2 per point) or
+  // FP64 whole matrices (M^2 per point); terms_nt points held
+  void *terms;
+  size_t cap_terms;
+  int terms_nt;
+  double tr_ms;
+  long tr_launches;
 };
 
 namespace {
@@ -249,6 +258,103 @@ int build_f32(mbpt_chi0 *h, int ntau, const double *tau, const double *w, double
     h->prof_ms += ms;
     h->prof_flops += (double)ntau * (double)M * (M + 1.0) * (h->no + h->nv + 1.0);
     h->prof_launches += 1;
+  }
+  return MBPT_OK;
+}
+
+template <typename B>
+int grow_bytes(B *&buf, size_t &cap, size_t bytes);
+
+// P(t) for every time point, kept for mbpt_chi0_transform.  FP32: the main
+// kernel with one time point per chunk and unit weights writes each P(t)
+// into its own plane.  FP64: one fused launch per time point.
+int terms_f32(mbpt_chi0 *h, int ntau, const double *tau, hipStream_t st) {
+  const int M = h->M, Mp = (M + kCT - 1) / kCT * kCT;
+  const int ko = (h->no + 15) / 16 * 16, kv = (h->nv + 15) / 16 * 16;
+  const int nt = Mp / kCT, ntiles = nt * (nt + 1) / 2;
+  int rc = grow(h->Yt, h->cap_yt, sizeof(float) * (size_t)ntau * (ko + kv) * Mp);
+  if (!rc) rc = grow_bytes(h->terms, h->cap_terms, sizeof(float) * (size_t)ntau * Mp * Mp);
+  if (rc) return rc;
+  TauW tw;
+  tw.n = ntau;
+  for (int t = 0; t < ntau; ++t) {
+    tw.tau_half[t] = 0.5 * tau[t];
+    tw.w[t] = 1.f;
+  }
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->e0, st));
+  k_chi0_prep<<<ntau * (ko + kv), 256, 0, st>>>(M, Mp, h->no, h->nv, ko, kv, h->XT,
+                                                  h->eo, h->ev, tw, h->Yt);
+  if ((rc = launch_status("k_chi0_prep"))) return rc;
+  const int nblk = pad_grid((long)ntiles * ntau);
+  k_chi0_f32<<<nblk, 512, 0, st>>>(Mp, ko, kv, nt, ntau, 1, h->Yt, tw, (float *)h->terms,
+                                   nblk);
+  if ((rc = launch_status("k_chi0_f32"))) return rc;
+  if (h->prof) {
+    MBPT_HIP_TRY(hipEventRecord(h->e1, st));
+    MBPT_HIP_TRY(hipEventSynchronize(h->e1));
+    float ms = 0.f;
+    MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->e0, h->e1));
+    h->prof_ms += ms;
+    h->prof_flops += (double)ntau * (double)M * (M + 1.0) * (h->no + h->nv + 1.0);
+    h->prof_launches += 1;
+  }
+  return MBPT_OK;
+}
+
+int terms_f64(mbpt_chi0 *h, int ntau, const double *tau, hipStream_t st) {
+  const size_t plane = (size_t)h->M * h->M;
+  int rc = grow_bytes(h->terms, h->cap_terms, sizeof(double) * ntau * plane);
+  if (rc) return rc;
+  const double one = 1.0;
+  for (int t = 0; t < ntau; ++t) {
+    rc = build_impl<double>(h, 1, tau + t, &one, (double *)h->terms + t * plane, 0, st);
+    if (rc) return rc;
+  }
+  return MBPT_OK;
+}
+
+template <int NT>
+int transform_launch(mbpt_chi0 *h, const FreqW &fw, double *out, int accumulate,
+                     hipStream_t st) {
+  const int M = h->M;
+  if (h->prec == MBPT_FP32) {
+    const int Mp = (M + kCT - 1) / kCT * kCT, nt128 = Mp / kCT;
+    const int nblk = 16 * nt128 * (nt128 + 1) / 2;
+    k_chi0_transform_f32<NT><<<nblk, 512, 0, st>>>(M, Mp, nt128, (const float *)h->terms,
+                                                   fw, out, accumulate);
+    return launch_status("k_chi0_transform_f32");
+  }
+  const long n = (long)M * M;
+  const int nblk = (int)std::min<long>((n + 255) / 256, 8192L);
+  k_chi0_transform_f64<NT><<<nblk, 256, 0, st>>>(n, (const double *)h->terms, fw, out,
+                                                 accumulate);
+  return launch_status("k_chi0_transform_f64");
+}
+
+int transform_impl(mbpt_chi0 *h, int nfreq, const double *w, double *out, int accumulate,
+                   hipStream_t st) {
+  const int nt = h->terms_nt;
+  const int per = std::max(1, kFreqW / nt);
+  const long MM = (long)h->M * h->M;
+  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->e0, st));
+  for (int f0 = 0; f0 < nfreq; f0 += per) {
+    FreqW fw;
+    fw.nt = nt;
+    fw.nf = std::min(per, nfreq - f0);
+    for (int q = 0; q < fw.nf * nt; ++q) fw.w[q] = w[(long)f0 * nt + q];
+    double *o = out + f0 * MM;
+    const int rc = nt <= 16   ? transform_launch<16>(h, fw, o, accumulate, st)
+                   : nt <= 32 ? transform_launch<32>(h, fw, o, accumulate, st)
+                              : transform_launch<64>(h, fw, o, accumulate, st);
+    if (rc) return rc;
+  }
+  if (h->prof) {
+    MBPT_HIP_TRY(hipEventRecord(h->e1, st));
+    MBPT_HIP_TRY(hipEventSynchronize(h->e1));
+    float ms = 0.f;
+    MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->e0, h->e1));
+    h->tr_ms += ms;
+    h->tr_launches += 1;
   }
   return MBPT_OK;
 }
@@ -406,6 +512,37 @@ int mbpt_chi0_build(mbpt_chi0 *h, int ntau, const double *tau_host,
              : build_f32(h, ntau, tau_host, weight_host, out_dev, accumulate, st);
 }
 
+int mbpt_chi0_terms(mbpt_chi0 *h, int ntau, const double *tau_host, void *stream) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_chi0_terms: NULL handle");
+  MBPT_REQUIRE(ntau >= 1 && ntau <= MBPT_CHI0_MAX_TAU, "mbpt_chi0_terms: ntau must be in [1, 64]");
+  MBPT_REQUIRE(tau_host != nullptr, "mbpt_chi0_terms: NULL tau");
+  for (int t = 0; t < ntau; ++t)
+    MBPT_REQUIRE(tau_host[t] >= 0.0, "mbpt_chi0_terms: time points must be non-negative");
+  hipStream_t st = (hipStream_t)stream;
+  h->terms_nt = 0;
+  const int rc = h->prec == MBPT_FP64 ? terms_f64(h, ntau, tau_host, st)
+                                      : terms_f32(h, ntau, tau_host, st);
+  if (!rc) h->terms_nt = ntau;
+  return rc;
+}
+
+int mbpt_chi0_transform(mbpt_chi0 *h, int nfreq, const double *weight_host, double *out_dev,
+                        int accumulate, void *stream) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_chi0_transform: NULL handle");
+  MBPT_REQUIRE(h->terms_nt > 0, "mbpt_chi0_transform: no terms held (call mbpt_chi0_terms)");
+  MBPT_REQUIRE(nfreq >= 0, "mbpt_chi0_transform: nfreq < 0");
+  if (nfreq == 0) return MBPT_OK;
+  MBPT_REQUIRE(weight_host && out_dev, "mbpt_chi0_transform: NULL argument");
+  return transform_impl(h, nfreq, weight_host, out_dev, accumulate, (hipStream_t)stream);
+}
+
+int mbpt_chi0_transform_profile_read(const mbpt_chi0 *h, double *ms, long *calls) {
+  MBPT_REQUIRE(h != nullptr, "mbpt_chi0_transform_profile_read: NULL handle");
+  if (ms) *ms = h->tr_ms;
+  if (calls) *calls = h->tr_launches;
+  return MBPT_OK;
+}
+
 int mbpt_chi0_sigma(mbpt_chi0 *h, int ntau, const double *tau_host, const float *Zt_dev,
                     int nstates, const int *states_host, double *out_dev, void *stream) {
   MBPT_REQUIRE(h != nullptr, "mbpt_chi0_sigma: NULL handle");
@@ -450,6 +587,8 @@ int mbpt_chi0_profile(mbpt_chi0 *h, int enable) {
   h->prof_launches = 0;
   h->gw_ms = h->gw_flops = 0.0;
   h->gw_launches = 0;
+  h->tr_ms = 0.0;
+  h->tr_launches = 0;
   return MBPT_OK;
 }
 
@@ -472,6 +611,7 @@ int mbpt_chi0_destroy(mbpt_chi0 *h) {
   (void)hipFree(h->eo); (void)hipFree(h->ev);
   (void)hipFree(h->Yt); (void)hipFree(h->part); (void)hipFree(h->XT);
   (void)hipFree(h->gw_xs); (void)hipFree(h->gw_part); (void)hipFree(h->gw_cols);
+  (void)hipFree(h->terms);
   delete h;
   return MBPT_OK;
 }

@@ -146,6 +146,46 @@ class Chi0Builder:
             'mbpt_chi0_sigma_profile_read')
         return {'ms': float(ms.value), 'launches': int(n.value), 'flops': float(fl.value)}
 
+    def terms(self, tau):
+        """Build and keep P(t) = -2 Go(t) o Gv(t) for every time point
+        (`mbpt_chi0_terms`): each Green's-function product once, for any
+        number of later `transform` calls."""
+        tau = np.atleast_1d(np.asarray(tau, float))
+        if not 1 <= len(tau) <= _lib.CHI0_MAX_TAU:
+            raise ValueError(f'{len(tau)} time points: 1 to {_lib.CHI0_MAX_TAU} are held')
+        tt, tp = _lib.host_doubles(tau)
+        _lib.check(self._lib.mbpt_chi0_terms(self._h, len(tt), tp, _lib.stream_ptr()),
+                   'mbpt_chi0_terms')
+        self._terms_nt = len(tt)
+        return self
+
+    def transform(self, weights, out=None, accumulate=False):
+        """(nfreq, M, M) FP64: out[f] = [out[f]] + sum_t weights[f, t] P(t)
+        over the kept terms (`mbpt_chi0_transform`), weights (nfreq, ntau)."""
+        W = np.atleast_2d(np.asarray(weights, float))
+        nt = getattr(self, '_terms_nt', 0)
+        if W.shape[1] != nt:
+            raise ValueError(f'weights have {W.shape[1]} time points, the terms {nt}')
+        if out is None:
+            out = torch.empty((W.shape[0], self.M, self.M), dtype=torch.float64,
+                              device=self.device)
+            accumulate = False
+        if tuple(out.shape) != (W.shape[0], self.M, self.M):
+            raise ValueError(f'out must be ({W.shape[0]}, {self.M}, {self.M})')
+        ww, wp = _lib.host_doubles(W)
+        _lib.check(self._lib.mbpt_chi0_transform(self._h, W.shape[0], wp, _lib.dptr(out),
+                                                  int(accumulate), _lib.stream_ptr()),
+                   'mbpt_chi0_transform')
+        return out
+
+    def transform_profile_read(self):
+        """{'ms', 'calls'} of `transform` since `profile()`."""
+        ms, n = ctypes.c_double(), ctypes.c_long()
+        _lib.check(self._lib.mbpt_chi0_transform_profile_read(self._h, ctypes.byref(ms),
+                                                               ctypes.byref(n)),
+                   'mbpt_chi0_transform_profile_read')
+        return {'ms': float(ms.value), 'calls': int(n.value)}
+
     def build(self, tau, weights, out=None, accumulate=False):
         """out (M, M) FP64 = [out] + sum_t weights[t] (-2) Go(t) o Gv(t)."""
         tau = np.atleast_1d(np.asarray(tau, float))
@@ -166,10 +206,91 @@ class Chi0Builder:
         return out
 
 
+#: Largest sum over the time points of |weight| for which chi0 at a frequency
+#: is formed from FP32 terms.  Each FP32 term carries a rounding of about
+#: 1e-7 of its size and the weights multiply it: the regularized 20-point
+#: grids keep the sum below 600 (E_c within 0.03 uHa of FP64 on the 4000-point
+#: instance), while the unregularized 16-point fit of the 1200-point instance
+#: reaches 1e9 with cancelling signs, where FP32 terms leave nothing of chi0.
+#: Frequencies above the limit are formed from FP64 terms.
+FP32_WEIGHT_SUM_LIMIT = 1e4
+
+
+def fp32_weights_ok(weights):
+    """Per row of `weights` (nfreq, ntau): whether FP32 terms suffice."""
+    W = np.atleast_2d(np.asarray(weights, float))
+    return np.abs(W).sum(axis=1) <= FP32_WEIGHT_SUM_LIMIT
+
+
 def chi0_grid(X, eps, nocc, tau, weights, mu=None, precision='fp32', builder=None):
-    """sum_t weights[t] (-2) Go(t) o Gv(t) on the grid, (M, M) FP64 device."""
-    b = builder or Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
-    return b.build(tau, weights)
+    """sum_t weights[t] (-2) Go(t) o Gv(t) on the grid, (M, M) FP64 device.
+    precision='fp32' builds in FP64 when the weights would amplify FP32
+    rounding beyond `FP32_WEIGHT_SUM_LIMIT`."""
+    if builder is None:
+        if precision == 'fp32' and not fp32_weights_ok(weights)[0]:
+            precision = 'fp64'
+        builder = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    return builder.build(tau, weights)
+
+
+class FrequencySweep:
+    """chi0 at the frequencies of a weight matrix (nfreq, ntau) from terms
+    built once per time point (`Chi0Builder.terms`), the one-pass form of the
+    reference's tau-to-frequency accumulation (space_time.py:634-651,
+    :854-905).  precision='fp32': FP32 terms for the rows `fp32_weights_ok`
+    passes, FP64 terms for the others; 'fp64': FP64 terms throughout."""
+
+    def __init__(self, X, eps, nocc, tau, weights, mu=None, precision='fp32',
+                 device=None):
+        self.W = np.atleast_2d(np.asarray(weights, float))
+        tau = np.atleast_1d(np.asarray(tau, float))
+        if self.W.shape[1] != len(tau):
+            raise ValueError(f'weights have {self.W.shape[1]} time points, tau {len(tau)}')
+        if precision not in ('fp32', 'fp64'):
+            raise ValueError(f"precision {precision!r}: 'fp32' or 'fp64'")
+        self.use64 = (np.ones(len(self.W), bool) if precision == 'fp64'
+                      else ~fp32_weights_ok(self.W))
+        self.builders = {}
+        for key, need in (('fp32', ~self.use64), ('fp64', self.use64)):
+            if need.any():
+                self.builders[key] = Chi0Builder(X, eps, nocc, mu=mu, precision=key,
+                                                 device=device).terms(tau)
+        first = next(iter(self.builders.values()))
+        self.M, self.device = first.M, first.device
+
+    @property
+    def nfreq(self):
+        return len(self.W)
+
+    def build(self, rows, out=None):
+        """(len(rows), M, M) FP64 device: chi0 at the frequencies `rows`."""
+        rows = np.atleast_1d(np.asarray(rows, int))
+        if out is None:
+            out = torch.empty((len(rows), self.M, self.M), dtype=torch.float64,
+                              device=self.device)
+        hi = self.use64[rows]
+        for key, mask in (('fp32', ~hi), ('fp64', hi)):
+            idx = np.flatnonzero(mask)
+            if idx.size == len(rows):
+                self.builders[key].transform(self.W[rows], out=out[:len(rows)])
+            elif idx.size:
+                for j in idx:
+                    self.builders[key].transform(self.W[rows[j]][None], out=out[j:j + 1])
+        return out
+
+    def profile(self, enable=True):
+        for b in self.builders.values():
+            b.profile(enable)
+
+    def profile_read(self):
+        """{'terms_ms', 'terms_flops', 'transform_ms'} summed over the builders."""
+        out = {'terms_ms': 0.0, 'terms_flops': 0.0, 'transform_ms': 0.0}
+        for b in self.builders.values():
+            p, t = b.profile_read(), b.transform_profile_read()
+            out['terms_ms'] += p['ms']
+            out['terms_flops'] += p['flops']
+            out['transform_ms'] += t['ms']
+        return out
 
 
 def _project(P, D):
@@ -230,17 +351,21 @@ def polarizability_projected_sweep(X, D, eps, nocc, tau_points, mu=None,
 def chi0_imaginary_frequency(X, D, eps, nocc, grid, mu=None, tau_indices=None,
                              tile_memory_gb=ISDF_TILE_GB, precision='fp64'):
     """chi0(i w_k) = sum_tau cosft_wt[k, tau] proj(tau), (nfreq, naux, naux)
-    (space_time.py:607-631); each frequency is one fused kernel launch over
-    all time points."""
+    (space_time.py:607-631): every proj(tau) built once, then one transform
+    pass forms all frequencies (`FrequencySweep`)."""
     dev = _device()
-    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
     Dd = None if D is None else _dev(D, dev)
-    naux = b.M if Dd is None else Dd.shape[1]
     which = np.arange(grid.ntau) if tau_indices is None else np.atleast_1d(tau_indices)
-    tau = grid.tau_points[which]
-    chi = torch.empty((grid.nfreq, naux, naux), dtype=torch.float64, device=dev)
+    tau = np.asarray(grid.tau_points)[which]
+    W = np.asarray(grid.cosft_wt, float)[:, which]
+    sweep = FrequencySweep(X, eps, nocc, tau, W, mu=mu, precision=precision, device=dev)
+    if Dd is None:
+        return sweep.build(np.arange(grid.nfreq))
+    chi = torch.empty((grid.nfreq, Dd.shape[1], Dd.shape[1]), dtype=torch.float64,
+                      device=dev)
+    tmp = torch.empty((1, sweep.M, sweep.M), dtype=torch.float64, device=dev)
     for k in range(grid.nfreq):
-        chi[k] = _project(b.build(tau, grid.cosft_wt[k, which]), Dd)
+        chi[k] = _project(sweep.build([k], out=tmp)[0], Dd)
     return chi
 
 
@@ -309,12 +434,12 @@ def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
 FACTOR_BATCH_BYTES = 16 * 2 ** 30
 
 
-def factor_batch_size(n, count, budget=FACTOR_BATCH_BYTES):
-    """How many n x n FP64 matrices (with their factors) one batched Cholesky
-    takes, at most `count`.  rocSOLVER's factorization of one matrix of a few
-    thousand is latency-bound (22 ms at n = 6000); sixteen at once take
-    3.9 ms each."""
-    return int(max(1, min(count, budget // (2 * 8 * n * n))))
+def factor_batch_size(n, count, budget=FACTOR_BATCH_BYTES, per_matrix=2):
+    """How many n x n FP64 matrices (each with `per_matrix` buffers: the
+    matrix, its factor, ...) one batched Cholesky takes, at most `count`.
+    rocSOLVER's factorization of one matrix of a few thousand is
+    latency-bound (22 ms at n = 6000); sixteen at once take 3.9 ms each."""
+    return int(max(1, min(count, budget // (per_matrix * 8 * n * n))))
 
 
 def solve_lower_blocked(C, B, block=1024):
@@ -369,12 +494,14 @@ def rpa_correlation_energy_space_time(X, D, eps, nocc, grid, mu=None,
     if comm is not None and comm.Get_size() > 1:
         raise NotImplementedError('one GPU: no frequency split over ranks')
     dev = _device()
-    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    sweep = FrequencySweep(X, eps, nocc, grid.tau_points, grid.cosft_wt, mu=mu,
+                           precision=precision, device=dev)
     Dd = None if D is None else _dev(D, dev)
     mine = {}
     e_c = 0.0
+    tmp = torch.empty((1, sweep.M, sweep.M), dtype=torch.float64, device=dev)
     for k in range(grid.nfreq):
-        c = _project(b.build(grid.tau_points, grid.cosft_wt[k]), Dd)
+        c = _project(sweep.build([k], out=tmp)[0], Dd)
         n = c.shape[0]
         logdet = dielectric_log_determinant(torch.eye(n, dtype=c.dtype, device=dev) - c,
                                             k, mine)
@@ -387,8 +514,9 @@ def rpa_correlation_energy_space_time(X, D, eps, nocc, grid, mu=None,
 def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32',
                                 flags=None):
     """E_c with the interaction on the grid: log det(I - L^T chi0 L) with
-    L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); one fused chi0 launch per
-    frequency, FP64 GEMMs and Cholesky log-dets.
+    L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); every proj(tau) built once
+    and transformed to the frequencies a batch at a time (`FrequencySweep`),
+    FP64 GEMMs and Cholesky log-dets.
 
     The matrices of up to `factor_batch_size` frequencies are factorized in
     one batched Cholesky; the log-dets and traces stay on the device until
@@ -398,23 +526,24 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     concurrent factorizations through torch's rocBLAS / rocSOLVER (several
     streams, one or several host threads) corrupted results at random."""
     dev = _device()
-    b = Chi0Builder(X, eps, nocc, mu=mu, precision=precision)
+    sweep = FrequencySweep(X, eps, nocc, grid.tau_points, grid.cosft_wt, mu=mu,
+                           precision=precision, device=dev)
     Vd = _dev(V, dev)
     L = torch.linalg.cholesky(Vd)
     n = L.shape[0]
     nf = grid.nfreq
     eye = torch.eye(n, dtype=torch.float64, device=dev)
-    chi = torch.empty((n, n), dtype=torch.float64, device=dev)
     logdet = torch.zeros(nf, dtype=torch.float64, device=dev)
     trace = torch.zeros(nf, dtype=torch.float64, device=dev)
     info = torch.zeros(nf, dtype=torch.int32, device=dev)
-    g = factor_batch_size(n, nf)
+    g = factor_batch_size(n, nf, per_matrix=3)
+    chis = torch.empty((g, n, n), dtype=torch.float64, device=dev)
     A = torch.empty((g, n, n), dtype=torch.float64, device=dev)
     for k0 in range(0, nf, g):
-        ks = range(k0, min(nf, k0 + g))
+        ks = list(range(k0, min(nf, k0 + g)))
+        sweep.build(ks, out=chis[:len(ks)])
         for j, k in enumerate(ks):
-            b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
-            K = L.T @ (chi @ L)
+            K = L.T @ (chis[j] @ L)
             trace[k] = torch.trace(K)
             torch.sub(eye, 0.5 * (K + K.T), out=A[j])
         C, inf = torch.linalg.cholesky_ex(A[:len(ks)])
@@ -425,7 +554,7 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     logdet_h, trace_h = logdet.cpu().numpy(), trace.cpu().numpy()
     mine = {}
     for k in np.flatnonzero(info.cpu().numpy()):
-        b.build(grid.tau_points, grid.cosft_wt[k], out=chi)
+        chi = sweep.build([k], out=chis[:1])[0]
         K = L.T @ (chi @ L)
         logdet_h[k] = dielectric_log_determinant(eye - K, int(k), mine)
     e_c = float(np.dot(np.asarray(grid.omega_weights, float), logdet_h + trace_h))

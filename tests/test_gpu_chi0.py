@@ -134,3 +134,66 @@ def test_rpa_energy_reference_signature():
     ref = O.rpa_correlation_energy_space_time(inst.X, D, inst.eps, 4, g.tau_points,
                                               g.cosft_wt, g.omega_weights)
     assert abs(e - ref) < 1e-10
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32'])
+@pytest.mark.parametrize('shape,ntau', [((5, 19, 120), 8), ((7, 30, 300), 20),
+                                        ((4, 9, 130), 40)])
+def test_terms_and_transform_against_the_oracle(shape, ntau, precision):
+    """Every proj(tau) built once (`mbpt_chi0_terms`) and transformed to all
+    frequencies in one pass (`mbpt_chi0_transform`) against the oracle's
+    chi0(i w_k) = sum_tau cosft_wt[k, tau] proj(tau) (space_time.py:607-651),
+    on shapes with one, three (mirrored off-diagonal tiles) and two 128-point
+    tile rows and 8, 20 and 40 time points; then `accumulate` adds a second
+    transform on top."""
+    from mbptcode_amd.space_time import Chi0Builder
+    no, nv, M = shape
+    inst = synth.make_instance(no, nv, M, seed=[11, M])
+    print(f'chi0 terms+transform {shape} ntau={ntau} {precision}', flush=True)
+    g = grids.rpa_energy_grid(inst.eps, no, ntau=min(ntau, 20), nfreq=12)
+    tau = np.sort(np.concatenate([g.tau_points, np.linspace(0.05, 3.0, ntau - g.ntau)])) \
+        if ntau > g.ntau else g.tau_points
+    W = np.random.default_rng(ntau).normal(size=(12, ntau))
+    ref = O.chi0_imaginary_frequency(inst.X, np.eye(M), inst.eps, no, tau, W)
+    b = Chi0Builder(inst.X, inst.eps, no, precision=precision).terms(tau)
+    got = b.transform(W).cpu().numpy()
+    for k in range(len(W)):
+        assert rel(got[k], ref[k]) < TOL[precision], k
+        assert rel(got[k], got[k].T) < TOL[precision], k
+    out = b.transform(W[:3])
+    b.transform(W[3:6], out=out, accumulate=True)
+    assert rel(out.cpu().numpy(), ref[:3] + ref[3:6]) < TOL[precision]
+
+
+def test_frequency_sweep_guards_fp32_against_large_weights():
+    """The benzene-size instance's unregularized 16-point fit carries cosine
+    weights of 3e8 with cancelling signs: FP32 terms would lose chi0
+    entirely, so the sweep forms those frequencies from FP64 terms, and
+    chi0(i w) matches the oracle as an FP64 build does."""
+    from mbptcode_amd.space_time import FrequencySweep, fp32_weights_ok
+    inst = synth.config(1)
+    g = grids.rpa_energy_grid(inst.eps, inst.nocc, ntau=inst.ntau, nfreq=40)
+    ok = fp32_weights_ok(g.cosft_wt)
+    assert not ok[:30].any() and ok[-1]
+    big = synth.config(2)
+    g2 = grids.rpa_energy_grid(big.eps, big.nocc, ntau=big.ntau, nfreq=40)
+    assert fp32_weights_ok(g2.cosft_wt).all()
+    sweep = FrequencySweep(inst.X, inst.eps, inst.nocc, g.tau_points, g.cosft_wt,
+                           precision='fp32')
+    assert set(sweep.builders) == {'fp32', 'fp64'}
+    print('chi0 sweep guard cfg1', flush=True)
+    rows = [0, 3, 38, 39]
+    got = sweep.build(rows).cpu().numpy()
+    ref = O.chi0_imaginary_frequency(inst.X, np.eye(inst.M), inst.eps, inst.nocc,
+                                     g.tau_points, g.cosft_wt[rows])
+    # the bar is the rounding of the terms amplified by the weights, in the
+    # oracle as here: |error| <= 16 u sum_t |w_t| max|P(t)|, u the unit
+    # roundoff of the terms' precision (FP64 rows reach 1e-16 x 1e9)
+    proj = O.polarizability_projected_sweep(inst.X, np.eye(inst.M), inst.eps, inst.nocc,
+                                            g.tau_points)
+    pmax = np.abs(proj).reshape(len(proj), -1).max(axis=1)
+    for j, k in enumerate(rows):
+        u = 1.1e-16 if sweep.use64[k] else 6e-8
+        bound = 16 * u * float(np.abs(g.cosft_wt[k]) @ pmax)
+        err = float(np.abs(got[j] - ref[j]).max())
+        assert err <= bound, (k, err, bound)

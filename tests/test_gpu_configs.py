@@ -1,0 +1,216 @@
+"""The benchmark workloads at their full sizes on the GPU.
+
+* Learned-preconditioner instance (100 occupied, 700 virtual, 6000 points):
+  the sigma-vector in FP64, FP32 and split BF16 against the oracle's
+  `isdf_block_action` (the reference algorithm, LR/block_actions.py:563-661)
+  on three trial vectors, and the learned+mixed Davidson (split-BF16 sigma,
+  the learned correction, FP64 recheck) against the plain FP64 Davidson: all
+  20 roots within 1 meV, every FP64 residual at most the tolerance
+  (the gates of tests/test_davidson_tda.py:128-151 in the reference).
+* Large instance (60/400/4000): split-BF16 and FP32 roots against FP64.
+* Batched workload: the 256 instances of seed 1 through one BatchedSigma;
+  a sample (the largest and smallest grids among them) against the oracle,
+  and the batched FP32 solve with its FP64 recheck against per-instance FP64
+  solves of every instance.
+* Benzene-size instance: the space-time RPA correlation energy against the
+  oracle's (same grid, FP64) and the Casida trace (rpa_energy.py:50-60).
+"""
+import numpy as np
+import pytest
+import torch
+
+from mbptcode_amd import grids, synth
+from oracle import isdf_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+HA_MEV = 27211.386245988
+KTOL = {'fp64': 1e-12, 'fp32': 2e-6, 'bf16x3': 6e-5}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope='module')
+def headline():
+    """The learned-preconditioner instance with its static W (FP64 chi0 on
+    the 20-point minimax grid, FP64 Cholesky Dyson step)."""
+    from mbptcode_amd.space_time import screened_interaction_grid
+    inst = synth.config(4)
+    _say('cfg4: building W')
+    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    W, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g, precision='fp64')
+    torch.cuda.synchronize()
+    return inst, W
+
+
+@pytest.fixture(scope='module')
+def headline_oracle(headline):
+    """Three trial vectors and the oracle's A z on them, in the factor gauge
+    D = chol(V), W_aux = D^-1 W D^-T (so D W_aux D^T is the grid W)."""
+    inst, W = headline
+    _say('cfg4: oracle action on 3 vectors')
+    from mbptcode_amd.space_time import solve_lower_blocked
+    Vd = torch.as_tensor(inst.V, device='cuda')
+    L = torch.linalg.cholesky(Vd)
+    Wa = solve_lower_blocked(L, W)
+    Wa = solve_lower_blocked(L, Wa.T.contiguous())
+    Wa = 0.5 * (Wa + Wa.T)
+    D, W_aux = L.cpu().numpy(), Wa.cpu().numpy()
+    z = np.random.default_rng(41).normal(size=(3, inst.nocc, inst.nvir))
+    ref = O.isdf_block_action(inst.eps, inst.nocc, inst.X, D, W_aux)[0](z)
+    return z, ref
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'fp32', 'bf16x3'])
+def test_headline_sigma_against_the_oracle(headline, headline_oracle, precision):
+    from mbptcode_amd.block_actions import SigmaOperator
+    inst, W = headline
+    z, ref = headline_oracle
+    _say(f'cfg4 sigma {precision}')
+    op = SigmaOperator(inst.eps, inst.nocc, inst.X, W, inst.V, precision=precision)
+    got = op.apply(torch.as_tensor(z, device='cuda')).cpu().numpy()
+    dz = O.pair_diagonal(inst.eps, inst.nocc)[None] * z
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref - dz)
+    _say(f'cfg4 sigma {precision}: kernel-part error {err:.2e}')
+    assert err < KTOL[precision], err
+
+
+def _fp64_residuals(act64, w, X, nocc, nvir):
+    Xt = torch.as_tensor(np.ascontiguousarray(X.T), device='cuda').reshape(-1, nocc, nvir)
+    AX = act64.operator.apply(Xt).reshape(len(w), -1).cpu().numpy()
+    return np.linalg.norm(AX - X.T * w[:, None], axis=1)
+
+
+def test_headline_learned_mixed_davidson(headline):
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.davidson import run_davidson
+    from mbptcode_amd.mlp_precond import LearnedPreconditioner
+    inst, W = headline
+    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
+    actm, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='bf16x3')
+    _say('cfg4 davidson: plain FP64')
+    e64, X64, _ = run_davidson(act64, d, inst.nroots, inst.tol, 200, tda=True)
+    _say('cfg4 davidson: learned + split BF16 + FP64 recheck')
+    tm = {}
+    em, Xm, _ = run_davidson(actm, d, inst.nroots, inst.tol, 200, tda=True,
+                             preconditioner='learned', learned=LearnedPreconditioner.load(),
+                             refine_apply=act64, timings=tm)
+    err = np.abs(em - e64) * HA_MEV
+    _say(f'cfg4 davidson: max root error {err.max():.2e} meV, '
+         f'{tm["davidson_iterations"]} cycles, {tm["davidson_vectors"]} vectors')
+    assert len(em) == inst.nroots and err.max() <= 1.0
+    rn = _fp64_residuals(act64, em, Xm, inst.nocc, inst.nvir)
+    _say(f'cfg4 davidson: FP64 residuals max {rn.max():.2e}')
+    assert rn.max() <= inst.tol * 1.01
+    assert all(tm['davidson_converged'])
+    # the plain solve's own roots meet the same bar
+    assert _fp64_residuals(act64, e64, X64, inst.nocc, inst.nvir).max() <= inst.tol * 1.01
+
+
+def test_large_instance_mixed_roots():
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.davidson import run_davidson
+    from mbptcode_amd.space_time import screened_interaction_grid
+    inst = synth.config(2)
+    g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    W, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g, precision='fp64')
+    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
+    e64, _, _ = run_davidson(act64, d, inst.nroots, inst.tol, 200, tda=True)
+    for p in ('bf16x3', 'fp32'):
+        _say(f'cfg2 davidson {p}')
+        act, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision=p)
+        e, X, _ = run_davidson(act, d, inst.nroots, inst.tol, 200, tda=True,
+                               refine_apply=act64)
+        err = np.abs(e - e64) * HA_MEV
+        _say(f'cfg2 {p}: max root error {err.max():.2e} meV')
+        assert err.max() <= 1.0
+        assert _fp64_residuals(act64, e, X, inst.nocc, inst.nvir).max() <= inst.tol * 1.01
+
+
+@pytest.fixture(scope='module')
+def batch256():
+    from mbptcode_amd.space_time import screened_interaction_grid
+    sizes = synth.batched_sizes(256)
+    insts, Ws = [], []
+    _say('batched: 256 instances, building W')
+    for i in range(256):
+        inst = synth.batched_instance(i, sizes)
+        g = grids.static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+        W, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g,
+                                         precision='fp64')
+        insts.append(inst)
+        Ws.append(W)
+    torch.cuda.synchronize()
+    return insts, Ws
+
+
+def _batched(insts, Ws, precision, kmax=10):
+    from mbptcode_amd.batched import BatchedSigma
+    return BatchedSigma([i.eps for i in insts], [i.nocc for i in insts], [i.X for i in insts],
+                        Ws, [i.V for i in insts], precision=precision, kmax=kmax)
+
+
+def test_batched_256_sigma_against_the_oracle(batch256):
+    insts, Ws = batch256
+    op = _batched(insts, Ws, 'fp32', kmax=3)
+    rng = np.random.default_rng(2)
+    z = np.zeros((op.n, op.kmax, op.P))
+    for b in range(op.n):
+        z[b, :, :op.npair[b]] = rng.normal(size=(op.kmax, op.npair[b]))
+    zt = torch.as_tensor(z, device='cuda')
+    Az = op.apply(zt).cpu().numpy()
+    Ms = np.array([i.M for i in insts])
+    sample = sorted({int(np.argmax(Ms)), int(np.argmin(Ms)),
+                     int(np.argmax([i.nocc for i in insts])), 17, 128})
+    for b in sample:
+        inst = insts[b]
+        _say(f'batched oracle check: instance {b} ({inst.nocc}, {inst.nvir}, {inst.M})')
+        D = O.grid_factors(inst.V)
+        Wg = Ws[b].cpu().numpy()
+        W_aux = np.linalg.solve(D, np.linalg.solve(D, Wg).T).T
+        W_aux = 0.5 * (W_aux + W_aux.T)
+        zb = z[b, :, :inst.npair].reshape(-1, inst.nocc, inst.nvir)
+        ref = O.isdf_block_action(inst.eps, inst.nocc, inst.X, D, W_aux)[0](zb)
+        got = Az[b, :, :inst.npair].reshape(ref.shape)
+        err = np.abs(got - ref).max() / np.abs(ref).max()
+        assert err < 1e-5, (b, err)
+
+
+def test_batched_256_solve_against_per_instance_fp64(batch256):
+    from mbptcode_amd.batched import solve_batched
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.davidson import run_davidson
+    insts, Ws = batch256
+    op64 = _batched(insts, Ws, 'fp64')
+    op32 = _batched(insts, Ws, 'fp32')
+    _say('batched solve: FP32 sigma + FP64 recheck, 256 instances')
+    e, X, info = solve_batched(op32, 5, 1e-5, refine_op=op64)
+    assert info['converged'] and info['rnorm_fp64_after'] <= 1e-5
+    worst = 0.0
+    for b, inst in enumerate(insts):
+        act, d = grid_block_action(inst.eps, inst.nocc, inst.X, Ws[b], inst.V, precision='fp64')
+        w, _, _ = run_davidson(act, d, 5, 1e-8, 300, tda=True)
+        worst = max(worst, float(np.abs(e[b] - w).max() * HA_MEV))
+        if b % 64 == 63:
+            _say(f'batched: {b + 1} per-instance FP64 solves, worst {worst:.2e} meV')
+        del act
+    assert worst <= 1e-3
+
+
+def test_benzene_size_rpa_energy():
+    from mbptcode_amd.space_time import rpa_correlation_energy_grid
+    inst = synth.config(1)
+    g = grids.rpa_energy_grid(inst.eps, inst.nocc, ntau=inst.ntau, nfreq=40)
+    D = O.grid_factors(inst.V)
+    _say('cfg1 E_c: oracle')
+    ref = O.rpa_correlation_energy_space_time(inst.X, D, inst.eps, inst.nocc, g.tau_points,
+                                              g.cosft_wt, g.omega_weights)
+    casida = O.rpa_energy_casida(inst.X, D, inst.eps, inst.nocc)
+    for p, tol in (('fp64', 1e-9), ('fp32', 1e-6)):
+        e = rpa_correlation_energy_grid(inst.X, inst.V, inst.eps, inst.nocc, g, precision=p)
+        _say(f'cfg1 E_c {p}: {e!r}, oracle {ref!r} ({abs(e - ref) * 1e6:.2e} uHa), '
+             f'Casida trace {casida!r} ({abs(e - casida) * 1e6:.2e} uHa)')
+        assert abs(e - ref) <= tol
+        assert abs(e - casida) <= 1e-6
