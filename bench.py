@@ -283,7 +283,46 @@ def headline(args):
     out['chi0_roofline'] = chi0_roofline(inst)
     log(f'chi0 (headline instance): {out["chi0_roofline"]["achieved"]:.1f} TF '
         f'({100 * out["chi0_roofline"]["frac"]:.0f}% of FP32 MFMA peak)')
+    out['end_to_end'] = end_to_end(inst, e_plain, w_s, out, args, net)
     return inst, W, out, w_s
+
+
+def end_to_end(inst, e_plain, w64_s, out, args, net):
+    """W build plus solve, per solver.  The mixed solvers build W on the
+    precision plan -- chi0(0) from FP32 terms, FP64 Cholesky Dyson step --
+    and solve on it; the plain solver's W is FP64 throughout.  Roots are
+    judged against the plain FP64 solve on the FP64 W."""
+    from mbptcode_amd.block_actions import grid_block_action
+    build_problem(args.config, w_precision='fp32')          # warm-up
+    _, W32, w32_s = build_problem(args.config, w_precision='fp32')
+    act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W32, inst.V, precision='fp64')
+    actm, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W32, inst.V,
+                                precision=args.precision)
+    learned = Solver('learned', actm, d, inst, refine=act64, preconditioner='learned',
+                     learned=net, learned_mode=args.learned_mode,
+                     learned_until=args.learned_until)
+    mixed = Solver('mixed', actm, d, inst, refine=act64)
+    res = {'W_fp64_chi0_s': w64_s, 'W_fp32_chi0_s': w32_s,
+           'W_definition': 'chi0(0) on the 20-point static grid, FP64 Cholesky of V and '
+                           'of I - L^T chi0 L; the FP32 build forms chi0 from FP32 terms'}
+    for key, s, name in (('learned+mixed', learned, f'{args.precision}+learned'),
+                         ('screened+mixed', mixed, f'{args.precision}+screened')):
+        s()
+        sync()
+        t0 = time.perf_counter()
+        w, _, _ = s()
+        sync()
+        dt = time.perf_counter() - t0
+        err = float(np.abs(w - e_plain).max() * HA_MEV)
+        res[key] = {'solve_ms_fp32_W': dt * 1e3, 'max_err_meV_vs_plain_fp64_W': err,
+                    'roots_per_s_end_to_end': inst.nroots / (w32_s + dt)}
+    pl = out['plain_fp64']
+    res['plain_fp64'] = {'roots_per_s_end_to_end': inst.nroots / (w64_s + pl['ms_per_step'] * 1e-3)}
+    log(f'end to end: W fp32 {w32_s * 1e3:.0f} ms (fp64 {w64_s * 1e3:.0f} ms); learned+mixed '
+        f'{res["learned+mixed"]["roots_per_s_end_to_end"]:.1f} roots/s '
+        f'({res["learned+mixed"]["max_err_meV_vs_plain_fp64_W"]:.1e} meV), plain '
+        f'{res["plain_fp64"]["roots_per_s_end_to_end"]:.1f} roots/s')
+    return res
 
 
 def leg_benzene():
@@ -329,21 +368,40 @@ def leg_large():
     grid = grids.rpa_energy_grid(inst.eps, inst.nocc, ntau=inst.ntau)
     ec = {}
     for p in ('fp64', 'fp32'):
+        rpa_correlation_energy_grid(inst.X, inst.V, inst.eps, inst.nocc, grid, precision=p)
         sync()
         t0 = time.perf_counter()
         ec[p] = rpa_correlation_energy_grid(inst.X, inst.V, inst.eps, inst.nocc, grid,
                                             precision=p)
         sync()
         ec[p + '_s'] = time.perf_counter() - t0
+    split = {}
+    rpa_correlation_energy_grid(inst.X, inst.V, inst.eps, inst.nocc, grid, precision='fp32',
+                                timings=split)
+    terms_tf = split['terms_flops'] / (split['terms_ms'] * 1e-3) / 1e12
     res['rpa_ec'] = {'fp64_Ha': ec['fp64'], 'fp32_Ha': ec['fp32'],
                      'err_uHa': abs(ec['fp32'] - ec['fp64']) * 1e6,
                      'seconds': {'fp64': ec['fp64_s'], 'fp32': ec['fp32_s']},
-                     'nfreq': grid.nfreq}
+                     'nfreq': grid.nfreq, 'ntau': grid.ntau,
+                     'fp32_split': {'chi0_terms_ms': split['terms_ms'],
+                                    'chi0_transform_ms': split['transform_ms'],
+                                    'chi0_share_of_Ec': (split['terms_ms'] + split['transform_ms'])
+                                    * 1e-3 / split['seconds'],
+                                    'seconds_profiled': split['seconds'],
+                                    'fp64_frequencies': split['fp64_frequencies'],
+                                    'chi0_terms_tflops': terms_tf,
+                                    'chi0_terms_frac_fp32_peak': terms_tf / PEAK_TFLOPS['fp32'],
+                                    'note': 'each of the ntau terms -2 Go o Gv is built once '
+                                            '(one launch of the fused kernel over all time '
+                                            'points) and transformed to the nfreq '
+                                            'frequencies in FP64'}}
     # chi0 kernel MFMA fraction: the static chi0(0) build, 20 time points
     res['chi0_roofline'] = chi0_roofline(inst)
     log(f'large: fp32 {res["fp32"]["max_err_meV"]:.1e} meV, bf16x3 '
         f'{res["bf16x3"]["max_err_meV"]:.1e} meV, E_c err {res["rpa_ec"]["err_uHa"]:.2e} uHa, '
-        f'chi0 {res["chi0_roofline"]["achieved"]:.1f} TF')
+        f'E_c fp32 {ec["fp32_s"] * 1e3:.0f} ms (chi0 terms {split["terms_ms"]:.1f} ms at '
+        f'{terms_tf:.1f} TF, transforms {split["transform_ms"]:.1f} ms), static chi0 '
+        f'{res["chi0_roofline"]["achieved"]:.1f} TF')
     return res
 
 
@@ -607,8 +665,10 @@ def main():
                              f'FP64 subspace + FP64 recheck'},
         'max_err_meV': lm['max_err_meV'],
         'speedup_vs_plain_fp64': lm['roots_per_s'] / pl['roots_per_s'],
-        'solvers': {k: v for k, v in out.items() if k not in ('roofline', 'chi0_roofline')},
+        'solvers': {k: v for k, v in out.items()
+                    if k not in ('roofline', 'chi0_roofline', 'end_to_end')},
         'W_build_s': w_s,
+        'end_to_end': out['end_to_end'],
         'roofline': out['roofline'],
         'chi0_roofline': out['chi0_roofline'],
     }

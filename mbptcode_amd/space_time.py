@@ -19,6 +19,7 @@ Cholesky factorisation of the positive-definite I - D^T chi0 D, never an
 explicit inverse.
 """
 import ctypes
+import time
 import warnings
 
 import numpy as np
@@ -241,7 +242,7 @@ class FrequencySweep:
     passes, FP64 terms for the others; 'fp64': FP64 terms throughout."""
 
     def __init__(self, X, eps, nocc, tau, weights, mu=None, precision='fp32',
-                 device=None):
+                 device=None, profile=False):
         self.W = np.atleast_2d(np.asarray(weights, float))
         tau = np.atleast_1d(np.asarray(tau, float))
         if self.W.shape[1] != len(tau):
@@ -253,8 +254,10 @@ class FrequencySweep:
         self.builders = {}
         for key, need in (('fp32', ~self.use64), ('fp64', self.use64)):
             if need.any():
-                self.builders[key] = Chi0Builder(X, eps, nocc, mu=mu, precision=key,
-                                                 device=device).terms(tau)
+                b = Chi0Builder(X, eps, nocc, mu=mu, precision=key, device=device)
+                if profile:
+                    b.profile(True)
+                self.builders[key] = b.terms(tau)
         first = next(iter(self.builders.values()))
         self.M, self.device = first.M, first.device
 
@@ -512,7 +515,7 @@ def rpa_correlation_energy_space_time(X, D, eps, nocc, grid, mu=None,
 
 
 def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32',
-                                flags=None):
+                                flags=None, timings=None):
     """E_c with the interaction on the grid: log det(I - L^T chi0 L) with
     L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); every proj(tau) built once
     and transformed to the frequencies a batch at a time (`FrequencySweep`),
@@ -524,10 +527,16 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     I - L^T chi0 L is not positive definite is redone through
     `dielectric_log_determinant` (slogdet, warning, flags).  One stream:
     concurrent factorizations through torch's rocBLAS / rocSOLVER (several
-    streams, one or several host threads) corrupted results at random."""
+    streams, one or several host threads) corrupted results at random.
+    `timings`, a dict, receives the device time of the term builds and of
+    the transforms (HIP events, waited for), their flops and the count of
+    frequencies formed from FP64 terms."""
     dev = _device()
+    if timings is not None:
+        torch.cuda.synchronize()
+        t_all = time.perf_counter()
     sweep = FrequencySweep(X, eps, nocc, grid.tau_points, grid.cosft_wt, mu=mu,
-                           precision=precision, device=dev)
+                           precision=precision, device=dev, profile=timings is not None)
     Vd = _dev(V, dev)
     L = torch.linalg.cholesky(Vd)
     n = L.shape[0]
@@ -560,4 +569,10 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     e_c = float(np.dot(np.asarray(grid.omega_weights, float), logdet_h + trace_h))
     if flags is not None and mine.get('nonpositive_det'):
         flags.setdefault('nonpositive_det', []).extend(mine['nonpositive_det'])
+    if timings is not None:
+        torch.cuda.synchronize()
+        timings.update(sweep.profile_read())
+        timings.update({'seconds': time.perf_counter() - t_all, 'nfreq': nf,
+                        'ntau': len(grid.tau_points),
+                        'fp64_frequencies': int(sweep.use64.sum())})
     return e_c / (2.0 * np.pi)
