@@ -33,11 +33,76 @@ class _Spectrum:
         self.spin_mode = 'restricted'
 
 
+def isdf_bse_factors(eps, nocc, factors, screening='imaginary-time', numerics=None,
+                     precision='fp64'):
+    """(X_mo, D, W_aux) for the ISDF Davidson BSE from one set of factors: the
+    pyscf-free part of `isdf_bse_factors` (BSE/solve.py:1038-1142), with the
+    factors given (the reference's `factors=` route) and the mean field's
+    orbital energies as `eps`.
+
+    W_aux = [1 - chi0(i0)]^-1 is built in D's auxiliary gauge from the
+    factors themselves, so the block action and W cannot mix gauges:
+
+    * screening='imaginary-time' (default): chi0(0) in imaginary time on
+      `static_screening_grid(eps, nocc, ntau=quadrature.ntau_w,
+      tau_target=quadrature.tau_target)` (`static_screening_matrix`), cubic
+      and on the device kernels at `precision`;
+    * screening='df': chi0(0) = -4 sum_ia B_ia B_ia^T / d_ia from the
+      three-index factor the factors imply, B_A,ia = sum_k X_ki X_ka D_kA
+      (`isdf_df_coefficients`, BSE/solve.py:396-407), exact for the factors
+      but (naux, n_occ n_vir)-sized: small systems only.
+    """
+    from .numerics import numerics_or_default
+    from .space_time import static_screening_matrix
+    from .grids import static_screening_grid
+    eps = np.asarray(eps, float)
+    X, D = unpack_factors(factors)[:2]
+    if D is None:
+        raise ValueError('isdf_bse_factors needs the factor D of (X_mo, D)')
+    X, D = np.asarray(X, float), np.asarray(D, float)
+    if X.shape[1] != len(eps):
+        raise ValueError(f'X has {X.shape[1]} orbitals, eps has {len(eps)}')
+    if D.shape[0] != X.shape[0]:
+        raise ValueError(f'X has {X.shape[0]} points, D has {D.shape[0]}')
+    if screening == 'df':
+        return X, D, _static_w_from_df(X, D, eps, nocc)
+    if screening != 'imaginary-time':
+        raise ValueError(f"screening={screening!r}; choose 'imaginary-time' or 'df'.")
+    q = numerics_or_default(numerics).quadrature
+    grid = static_screening_grid(eps, nocc, ntau=q.ntau_w, tau_target=q.tau_target)
+    return X, D, static_screening_matrix(X, D, eps, nocc, grid, precision=precision)
+
+
+#: Device bytes the 'df' static screening may take for its pair factors.
+DF_SCREENING_BYTES = 8 * 2 ** 30
+
+
+def _static_w_from_df(X, D, eps, nocc):
+    """[1 - chi0(0)]^-1 with chi0(0) = 2 C f C^T, f = -2 / d (the static limit
+    of imaginary_frequency.py:91-122 on the three-index factor of X and D),
+    FP64 on the device."""
+    import torch
+    from .orbital_spaces import bse_pair_diagonal
+    from .space_time import dyson_screening
+    M, naux = D.shape
+    npair = nocc * (X.shape[1] - nocc)
+    if 8 * npair * (M + naux) > DF_SCREENING_BYTES:
+        raise ValueError(f"screening='df' would hold {npair} x ({M} + {naux}) doubles; "
+                         "use screening='imaginary-time' at this size")
+    dev = torch.device('cuda', torch.cuda.current_device())
+    Xd = torch.as_tensor(X, device=dev)
+    Dd = torch.as_tensor(D, device=dev)
+    C = (Xd[:, :nocc, None] * Xd[:, None, nocc:]).reshape(M, npair)
+    B = Dd.T @ C
+    d = torch.as_tensor(bse_pair_diagonal(eps, nocc).ravel(), device=dev)
+    chi = -4.0 * (B / d[None, :]) @ B.T
+    return dyson_screening(0.5 * (chi + chi.T)).cpu().numpy()
+
+
 def solve_bse_isdf(eps, nocc, factors, V=None, qp=None, xc_diagonal=None, W_aux=None,
-                   tda=False, spin='singlet', nroots=5, conv_tol=BSE_DAVIDSON_CONV_TOL,
-                   max_cycle=150, preconditioner='screened', precision='fp64', probe=False,
-                   guess=None,
-                   refuse_unconverged=False, ntau=None):
+                   tda=False, spin='singlet', nroots=None, conv_tol=None,
+                   max_cycle=None, preconditioner=None, precision='fp64', probe=False,
+                   guess=None, refuse_unconverged=False, ntau=None, numerics=None):
     """(omega, X, Y, info) of the `nroots` lowest BSE excitations.
 
     eps (n_mo,), nocc: the mean-field spectrum.  factors: (X_mo, D[, X_ao,
@@ -46,13 +111,20 @@ def solve_bse_isdf(eps, nocc, factors, V=None, qp=None, xc_diagonal=None, W_aux=
     qp='G0W0' (the space-time self-energy runs in the grid gauge) or D None.
     qp: None (BSE on eps), an array of n_mo energies, or 'G0W0' (then
     xc_diagonal, n_mo values of <Sigma_x - v_xc>_pp, is required).
-    W_aux: a static W in D's gauge, built at eps when None.
+    W_aux: a static W in D's gauge, built at eps when None (`isdf_bse_factors`,
+    its time points `ntau`, else numerics.quadrature.ntau_w, else sized by the
+    Laplace test).  numerics: a `Numerics`; the Davidson settings nroots,
+    conv_tol, max_cycle and preconditioner left None come from its bse
+    section (numerics.py:241-286; None: the defaults, 5 roots to 1e-8).
     The remaining keywords are `solve_casida_davidson`'s (tda, spin, probe,
-    guess, refuse_unconverged, the Davidson settings and the block action's
-    precision)."""
+    guess, refuse_unconverged and the block action's precision)."""
+    from .numerics import numerics_or_default
+    num = numerics_or_default(numerics)
+    nroots = num.bse.nroots if nroots is None else nroots
+    conv_tol = num.bse.conv_tol if conv_tol is None else conv_tol
+    max_cycle = num.bse.max_cycle if max_cycle is None else max_cycle
+    preconditioner = num.bse.preconditioner if preconditioner is None else preconditioner
     from .davidson import solve_casida_davidson
-    from .grids import static_screening_grid
-    from .space_time import static_screening_matrix
     eps = np.asarray(eps, float)
     X, D = unpack_factors(factors)[:2]
     X = np.asarray(X, float)
@@ -92,7 +164,11 @@ def solve_bse_isdf(eps, nocc, factors, V=None, qp=None, xc_diagonal=None, W_aux=
         eps_qp, z = eps, None
     t0 = time.perf_counter()
     if W_aux is None:
-        W_aux = static_screening_matrix(X, D, eps, nocc, static_screening_grid(eps, nocc))
+        if ntau is not None:
+            num = num.derive('static W at the requested time points',
+                             quadrature={'ntau_w': int(ntau)}) \
+                if num.quadrature.ntau_w != int(ntau) else num
+        W_aux = isdf_bse_factors(eps, nocc, (X, D), numerics=num)[2]
         t['static_w'] = time.perf_counter() - t0
     W_aux = np.asarray(W_aux, float)
     stats, tracked, timings = {}, {}, {}
@@ -102,7 +178,7 @@ def solve_bse_isdf(eps, nocc, factors, V=None, qp=None, xc_diagonal=None, W_aux=
         stats=stats, spin=spin, timings=timings, refuse_unconverged=refuse_unconverged,
         guess=guess, tda=tda, tracked=tracked, probe=probe, nroots=nroots,
         conv_tol=conv_tol, max_cycle=max_cycle, preconditioner=preconditioner,
-        precision=precision)
+        guess_factor=num.bse.guess_factor, precision=precision)
     t['davidson'] = time.perf_counter() - t0
     flags = {}
     conv = timings.get('davidson_converged')

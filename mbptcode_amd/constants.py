@@ -21,6 +21,19 @@ BSE_DAVIDSON_NROOTS = 5
 BSE_DAVIDSON_CONV_TOL = 1e-8
 BSE_FORCE_MAX_CYCLE = 150
 
+#: The Casida eigensolver choices (constants.py:1865), the root adjoint forms
+#: (:1243) and the root-following modes (:1654) a `BSENumerics` accepts.
+BSE_SOLVERS = ('auto', 'dense', 'davidson')
+BSE_ADJOINTS = ('explicit', 'grid')
+ROOT_FOLLOWING = ('overlap',)
+
+#: Time points of the self-energy's contour grid (constants.py:493), Pade
+#: points of the space-time self-energy (:1849) and the scale of their
+#: Gauss-Legendre grid (:1851).
+CD_NTAU = 24
+DEFAULT_NPADE = 16
+GL_FREQUENCY_SCALE = 1.0
+
 #: Gauss-Legendre frequencies of the dRPA energy quadrature (constants.py:469).
 RPA_ENERGY_NFREQ = 40
 
@@ -44,6 +57,8 @@ DAVIDSON_EXTRA_ROOTS_DIVISOR = 2        # :1650
 DAVIDSON_GUESS_FACTOR = 2               # :1886
 DAVIDSON_GUESS_DEG_TOL = 1e-3           # :2190
 DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
+DAVIDSON_COMPLETION_SEED = 0            # :2124 random directions of the completion solve
+DAVIDSON_FOLLOW_OVERLAP_TIE = 1e-6      # :2129 overlaps this close are a tie
 
 # the ISDF fit (constants.py:36, :1755): the metric root's relative eigenvalue
 # floor and the Tikhonov floor of the balanced Gram matrix

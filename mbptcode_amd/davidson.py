@@ -36,7 +36,9 @@ from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
                         DAVIDSON_LINDEP_CEILING, DAVIDSON_LINDEP_FLOOR,
                         DAVIDSON_MAX_CYCLE, DAVIDSON_PRECONDITIONER,
                         DAVIDSON_PRECONDITIONERS, DAVIDSON_TOL_RESIDUAL,
-                        DAVIDSON_EXTRA_ROOTS_DIVISOR, DAVIDSON_EXTRA_ROOTS_FLOOR)
+                        DAVIDSON_EXTRA_ROOTS_DIVISOR, DAVIDSON_EXTRA_ROOTS_FLOOR,
+                        DAVIDSON_COMPLETION_SEED, DAVIDSON_FOLLOW_OVERLAP_TIE,
+                        DAVIDSON_NULL_DIRECTION)
 
 
 _BLAS = {}
@@ -263,7 +265,7 @@ class DavidsonResult:
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
              lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None,
-             keep=None):
+             keep=None, pick=None):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -283,6 +285,10 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     `phases`, a dict, receives the seconds of each part of the cycle (the
     device is synchronised at every boundary, so only for profiling).
     `keep`: Ritz vectors kept at a collapse (thick restart; default nroots).
+    `pick(w, v, nroots, envs) -> (w, v, order)` selects the Ritz pairs to
+    converge instead of the lowest (root following, `overlap_pick`): envs
+    holds 'ref_coeff', V^T span (m, r), when the pick carries a `span`
+    (n, r), else 'xs', the trial vectors as host arrays.
     """
     last = [time.perf_counter()]
 
@@ -329,6 +335,19 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
 
     append(T0, matmat(T0))
     vectors = m
+    span = getattr(pick, 'span', None)
+    if span is not None:
+        span = torch.as_tensor(np.asarray(span, float).reshape(n, -1), device=dev)
+
+    def ritz_select(evals, evecs, k):
+        """(w, y, order): the k Ritz pairs the iteration converges."""
+        if pick is None:
+            return evals[:k], evecs[:, :k], np.arange(k)
+        if span is not None:
+            envs = {'ref_coeff': _tn(Vb[:, :m], span).cpu().numpy()}
+        else:
+            envs = {'xs': list(Vb[:, :m].T.cpu().numpy())}
+        return pick(evals, evecs, k, envs)
     trace = []
     conv = np.zeros(nroots, bool)
     w = rn = None
@@ -347,7 +366,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         with _host_lapack():
             evals, evecs = np.linalg.eigh(H)
         k = min(nroots, m)
-        w, y = evals[:k], evecs[:, :k]
+        w, y, order = ritz_select(evals, evecs, k)
         y_rows = m
         sq = ((G @ y) * y).sum(0)
         xax = ((Hns @ y) * y).sum(0)
@@ -404,7 +423,8 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             # collapse onto the lowest `keep` Ritz vectors (T is already
             # orthogonal to the whole space)
             kk = min(m, max(k, keep or k))
-            yk = evecs[:, :kk]
+            cols = list(order) + [j for j in range(m) if j not in set(order)]
+            yk = evecs[:, cols[:kk]]
             yt = torch.as_tensor(yk, device=dev)
             X = Vb[:, :m] @ yt
             AX = AVb[:, :m] @ yt
@@ -426,7 +446,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         H = Hb[:m, :m].cpu().numpy()
         evals, evecs = np.linalg.eigh(0.5 * (H + H.T))
         k = min(nroots, m)
-        w, y = evals[:k], evecs[:, :k]
+        w, y, _ = ritz_select(evals, evecs, k)
     yt = torch.as_tensor(y, device=dev)
     X = Vb[:, :m] @ yt
     AX = AVb[:, :m] @ yt
@@ -438,20 +458,113 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
                           stalled=stalled)
 
 
+class overlap_pick:
+    """A root-following `pick` (Solvers/davidson.py:80-101): at every step the
+    `nroots` Ritz vectors with the largest overlap onto the fixed reference
+    `ref` -- the norm of their projection onto the span of ref's columns --
+    rather than the lowest, reported ascending in energy.  Overlaps within
+    DAVIDSON_FOLLOW_OVERLAP_TIE of each other tie, and ties go to the lower
+    energy, so roots the reference does not reach are taken in a fixed order.
+    Called with envs['ref_coeff'] (V^T span, as the device iteration passes
+    it) or envs['xs'] (the trial vectors, pyscf's form)."""
+
+    def __init__(self, ref):
+        ref = np.asarray(ref, float)
+        self.span = np.linalg.qr(ref)[0] if ref.ndim == 2 else (ref / np.linalg.norm(ref))[:, None]
+
+    def __call__(self, w, v, nroots, envs):
+        w = np.asarray(w, float)
+        if 'ref_coeff' in envs:
+            coeff = np.asarray(envs['ref_coeff'], float)              # (m, r)
+        else:
+            coeff = np.array([self.span.T @ np.asarray(x, float).ravel() for x in envs['xs']])
+        overlap = np.linalg.norm(coeff.T @ v, axis=0)
+        tier = np.round(overlap / DAVIDSON_FOLLOW_OVERLAP_TIE)
+        chosen = np.lexsort((w, -tier))[:nroots]
+        order = chosen[np.argsort(w[chosen], kind='stable')]
+        return w[order], v[:, order], order
+
+
+def _off(V, X):
+    """Columns of V projected off the orthonormal X twice and normalised;
+    those left below DAVIDSON_NULL_DIRECTION are dropped."""
+    for _ in range(2):
+        V = V - X @ _tn(X, V)
+    norms = V.norm(dim=0)
+    keep = norms > DAVIDSON_NULL_DIRECTION
+    return V[:, keep] / norms[keep]
+
+
+def _deflated(block, X, lift):
+    """The operator P A P + lift X X^T, P = 1 - X X^T, on (n, m) device
+    blocks: A restricted to the complement of X, with X moved to `lift`."""
+    def mm(V):
+        C = _tn(X, V)
+        AV = block(V - X @ C)
+        return AV - X @ _tn(X, AV) + lift * (X @ C)
+    return mm
+
+
+def _complete_lowest(block, diag_t, e, X, warn, opts, label):
+    """Certify that the converged (e, X) are the lowest len(e) eigenpairs
+    (Solvers/davidson.py:362-404): the lowest root of the operator on the
+    complement of X, with X lifted above the top root, replaces the top root
+    when it lies below it by more than the tolerance; repeated until a
+    complement solve finds nothing lower, at most k + 1 rounds, one root per
+    round.  A complement solve that does not converge, or no certificate
+    after k + 1 rounds, reports every root unconverged."""
+    n, k = X.shape
+    tol = opts['tol']
+    rng = np.random.default_rng(DAVIDSON_COMPLETION_SEED)
+    nseed = min(n, 2 * k + 4)
+    seeds = torch.zeros((n, nseed), dtype=torch.float64, device=X.device)
+    low = torch.argsort(diag_t)[:nseed]
+    seeds[low, torch.arange(nseed, device=X.device)] = 1.0
+    reason = None
+    for _ in range(k + 1):
+        if n - k < 1:
+            return e, X, np.ones(k, bool)
+        lift = e[-1] + abs(e[-1]) + (e[-1] - e[0]) + 1.0
+        rand = torch.as_tensor(rng.standard_normal((n, k)), device=X.device)
+        start = _off(torch.cat([seeds, rand], dim=1), X)
+        res = davidson(_deflated(block, X, lift), diag_t, 1, start, **opts)
+        if not res.conv.all():
+            reason = 'the solve on their complement did not converge'
+            break
+        mu = float(res.e[0])
+        if mu >= e[-1] - tol:
+            return e, X, np.ones(k, bool)
+        y = res.X[:, 0]
+        y = y - X @ (X.T @ y)
+        y = y / y.norm()
+        w_all = np.append(e, mu)
+        X_all = torch.cat([X, y[:, None]], dim=1)
+        order = np.argsort(w_all, kind='stable')[:k]
+        e, X = w_all[order], X_all[:, torch.as_tensor(order, device=X.device)]
+    else:
+        reason = f'{k + 1} rounds of merging did not end in a certificate'
+    if warn:
+        warnings.warn(f'{label}: the {k} converged roots are not certified to be the '
+                      f'lowest ({reason}); all reported unconverged.', RuntimeWarning,
+                      stacklevel=3)
+    return e, X, np.zeros(k, bool)
+
+
 def solve_symmetric(matvec, diag, nroots=1, x0=None, pick=None,
                     tol_residual=DAVIDSON_TOL_RESIDUAL, tol_eig=None,
                     max_cycle=DAVIDSON_MAX_CYCLE, max_subspace=None, precond=None,
                     lindep=None, warn_unconverged=True, label='Davidson',
-                    **unused):
-    """(e, X, conv) of the `nroots` lowest eigenpairs (Solvers/davidson.py:179).
+                    complete_lowest=True, **unused):
+    """(e, X, conv) of the `nroots` lowest eigenpairs, or with `pick` the ones
+    it selects (Solvers/davidson.py:179-326).
 
     matvec: an object with `.matmat` (n, m) -> (n, m) on device FP64 tensors,
-    or a callable on columns.  diag (n,).  x0 None seeds the 2 nroots + 4
-    lowest-diagonal unit vectors.  `pick` (root following) is not built.
-    Returns NumPy e, conv and a device X.
+    or a callable on such blocks.  diag (n,).  x0 None seeds the 2 nroots + 4
+    lowest-diagonal unit vectors.  pick: e.g. `overlap_pick(ref)`.
+    complete_lowest (without pick): the converged set is certified to be the
+    lowest by solves on its complement (`_complete_lowest`).  Returns NumPy
+    e and conv and a device X, ascending.
     """
-    if pick is not None:
-        raise NotImplementedError('root following (pick) is not built')
     block = getattr(matvec, 'matmat', matvec)
     dev = torch.device('cuda', torch.cuda.current_device())
     diag_t = torch.as_tensor(np.asarray(diag, float).ravel(), device=dev)
@@ -467,13 +580,22 @@ def solve_symmetric(matvec, diag, nroots=1, x0=None, pick=None,
     if precond is not None:
         def pre(R, U, theta, cycle):
             return precond(R, U, theta)
-    res = davidson(block, diag_t, nroots, x0, tol=tol_residual, max_cycle=max_cycle,
-                   max_space=max_subspace, precond=pre, lindep=lindep)
-    if warn_unconverged and not res.conv.all():
-        warnings.warn(f'{label} left {int((~res.conv).sum())} of {len(res.conv)} '
+    opts = dict(tol=tol_residual, max_cycle=max_cycle, max_space=max_subspace,
+                precond=pre, lindep=lindep)
+    res = davidson(block, diag_t, nroots, x0, pick=pick, **opts)
+    e, X, conv = np.asarray(res.e), res.X, np.asarray(res.conv)
+    order = np.argsort(e, kind='stable')
+    e, X, conv = e[order], X[:, torch.as_tensor(order, device=dev)], conv[order]
+    if complete_lowest and pick is None and conv.all() and e.size < n:
+        e, X, conv = _complete_lowest(block, diag_t, e, X, warn_unconverged, opts,
+                                      label + ' (complement)')
+        if not conv.all():
+            return e, X, conv
+    if warn_unconverged and not conv.all():
+        warnings.warn(f'{label} left {int((~conv).sum())} of {len(conv)} '
                       f'roots unconverged after {max_cycle} cycles',
                       RuntimeWarning, stacklevel=2)
-    return res.e, res.X, res.conv
+    return e, X, conv
 
 
 # ---------------------------------------------------------------------------

@@ -24,7 +24,8 @@ import warnings
 
 import numpy as np
 
-from .constants import DEFAULT_TAU_TARGET, TRANSFORM_FIT_RCOND
+from .constants import (DEFAULT_NPADE, DEFAULT_TAU_TARGET, GL_FREQUENCY_SCALE,  # noqa: F401
+                        TRANSFORM_FIT_RCOND)
 from .orbital_spaces import particle_hole_window
 
 _DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data')
@@ -36,10 +37,6 @@ _tables = {}
 COSINE_TW, COSINE_WT, SINE_TW, SINE_WT = 1, 2, 3, 4
 #: the self-energy fit ranges' padding (time_frequency.py:230)
 SELF_ENERGY_PAD = (0.3, 3.0)
-#: Pade points of the space-time self-energy and the scale of their
-#: Gauss-Legendre axis, Ha (constants.py:1849-1851)
-DEFAULT_NPADE = 16
-GL_FREQUENCY_SCALE = 1.0
 
 #: Fits at or above this many input points get a Tikhonov term
 #: (time_frequency.py:109-110).

@@ -94,3 +94,30 @@ def test_lindep_follows_the_tolerance():
     assert lindep_for(1e-5) == pytest.approx(1e-12)
     assert lindep_for(1e-12) == 1e-22
     assert lindep_for(1.0) == 1e-14
+
+
+def test_overlap_pick_ranks_by_overlap_and_breaks_ties_by_energy():
+    """overlap_pick (Solvers/davidson.py:80-101) on pyscf's envs['xs'] form:
+    the Ritz vector with the largest projection on the reference span wins;
+    overlaps within 1e-6 tie and go to the lower energy; the chosen roots come
+    back ascending."""
+    from mbptcode_amd.davidson import overlap_pick
+    n = 6
+    xs = list(np.eye(n))                       # trial vectors = unit vectors
+    w = np.array([-1.0, 0.5, 0.2, 0.9, 2.0, 3.0])
+    v = np.eye(n)                              # Ritz vectors = trial vectors
+    ref = np.zeros(n)
+    ref[3] = 1.0
+    we, ve, order = overlap_pick(ref)(w, v, 1, {'xs': xs})
+    assert order.tolist() == [3] and we.tolist() == [0.9]
+    # two roots: root 3 first by overlap, then the lowest of the zero-overlap tie
+    we, ve, order = overlap_pick(ref)(w, v, 2, {'xs': xs})
+    assert order.tolist() == [0, 3] and np.all(np.diff(we) > 0)
+    # the device form (V^T span) gives the same choice
+    we2, _, order2 = overlap_pick(ref)(w, v, 2, {'ref_coeff': np.eye(n)[:, [3]]})
+    assert order2.tolist() == order.tolist()
+    # a two-column reference follows the span of both
+    ref2 = np.zeros((n, 2))
+    ref2[4, 0] = ref2[5, 1] = 1.0
+    _, _, order = overlap_pick(ref2)(w, v, 2, {'xs': xs})
+    assert order.tolist() == [4, 5]
