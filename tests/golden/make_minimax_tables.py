@@ -3,7 +3,8 @@ minimax tables.
 
 The numbers are the GreenX coefficients (GX-TimeFrequency, minimax_tau.F90 for
 the imaginary-time "Laplace" grids and minimax_omega.F90 for the
-imaginary-frequency grids; Apache-2.0, see ``NOTICE`` beside this file) as the
+imaginary-frequency grids; Apache-2.0, see ``NOTICE`` beside the tables in
+mbptcode_amd/data/) as the
 reference ships them in ``src/Base/utils/grids/minimax_tau_data.json`` and
 ``minimax_omega_data.json``.  They are data, not code: a minimax grid is a
 table lookup, so the product needs the same numbers.  This script only
@@ -19,8 +20,10 @@ import json
 import os
 import sys
 
-SRC_DIR = '/root/reference/src/Base/utils/grids'
-HERE = os.path.dirname(os.path.abspath(__file__))
+SRC_DIR = os.path.join(os.environ.get('MBPT_REFERENCE', '/root/reference'),
+                       'src', 'Base', 'utils', 'grids')
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), '..', '..', 'mbptcode_amd',
+                    'data')
 TABLES = (('minimax_tau_data.json', 'minimax_tau.json',
            'GreenX GX-TimeFrequency minimax_tau (Apache-2.0)'),
           ('minimax_omega_data.json', 'minimax_omega.json',
@@ -41,7 +44,7 @@ def pack(src, out, origin):
     return out
 
 
-def main(src_dir=SRC_DIR, out_dir=HERE):
+def main(src_dir=SRC_DIR, out_dir=DATA_DIR):
     return [pack(os.path.join(src_dir, s), os.path.join(out_dir, o), origin)
             for s, o, origin in TABLES]
 
