@@ -221,6 +221,44 @@ int mbpt_gemm_tn_f64_noalloc(int n, int a, int b, const double *A, int lda,
                              void *stream);
 
 /* ------------------------------------------------------------------------
+ * Products of the single-instance Davidson iteration (FP64), the work inside
+ * pyscf's lib.davidson1 (called at src/Solvers/davidson.py:277) and its
+ * Jacobi-Davidson preconditioner (Solvers/davidson.py:43-77), on a trial
+ * space kept as column blocks of row-major (n, ld) buffers, n the pair count.
+ *
+ * An operand is up to two column ranges of one buffer: logical column i is
+ * column off1 + i for i < n1 and off2 + (i - n1) after, so [V | AV] or
+ * [V | T] pass as one operand.
+ *
+ * TN: C (a, b), row stride ldc, = A^T B for A (n, a) and B (n, b) so given;
+ *     work: mbpt_gemm_tn_seg_f64_work(n, a, b) doubles.
+ * NN: C[:, c_off : c_off + c] = beta C + alpha A Y, A (n, k) so given, Y
+ *     (k, c) row-major with ldy, c <= 64 (beta 0: C is not read).
+ * Ritz: X = V Y and R = AV Y - X diag(theta), V and AV the m columns at
+ *     voff / avoff, Y (m, k), k <= 64: the Ritz vectors and their residuals.
+ * JD: T = K^-1 R - alpha K^-1 U per column, K = diag - theta floored at
+ *     `floor` in magnitude, alpha = U.K^-1 R / U.K^-1 U; work:
+ *     mbpt_jd_correction_f64_work(n, k) doubles.
+ * ------------------------------------------------------------------------ */
+long mbpt_gemm_tn_seg_f64_work(int n, int a, int b);
+int mbpt_gemm_tn_seg_f64(int n, int a, int b, const double *A, int lda, int a1,
+                         int aoff1, int aoff2, const double *B, int ldb, int b1,
+                         int boff1, int boff2, double *C, int ldc, double *work,
+                         long work_elems, void *stream);
+int mbpt_gemm_nn_seg_f64(int n, int k, int c, const double *A, int lda, int k1,
+                         int aoff1, int aoff2, const double *Y, int ldy, double *C,
+                         int ldc, int c_off, double alpha, double beta, void *stream);
+int mbpt_ritz_residual_f64(int n, int m, int k, const double *V, int ldv, int voff,
+                           const double *AV, int ldav, int avoff, const double *Y,
+                           int ldy, const double *theta, double *X, int ldx,
+                           double *R, int ldr, void *stream);
+long mbpt_jd_correction_f64_work(int n, int k);
+int mbpt_jd_correction_f64(int n, int k, const double *U, int ldu, const double *R,
+                           int ldr, const double *diag, const double *theta,
+                           double floor, double *T, int ldt, double *work,
+                           long work_elems, void *stream);
+
+/* ------------------------------------------------------------------------
  * Batched small symmetric eigensolver (FP64): the Rayleigh-Ritz step of many
  * Davidson runs at once, the batched counterpart of the m x m eigh inside
  * pyscf's lib.davidson1 (called at src/Solvers/davidson.py:277).

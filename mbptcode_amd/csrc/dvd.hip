@@ -1,0 +1,373 @@
+// FP64 kernels of the single-instance Davidson iteration on gfx950: the
+// products over the long pair index that its Rayleigh-Ritz step, residuals,
+// corrections and trial-space orthogonalisation are made of.  The trial
+// space V and its image AV are column blocks of row-major buffers (n rows,
+// the pair index); every kernel takes an operand as up to two column ranges
+// of one such buffer, so [V | AV] or [V | T] pass without a copy.
+//
+// All products use v_mfma_f64_16x16x4f64 with each operand fragment loaded
+// straight from HBM.  The k order inside an MFMA is free as long as both
+// operands agree on it, so a lane's four k values are chosen to be adjacent
+// in memory: every row access is a 128-byte segment per 16 lanes.
+#include "common.h"
+
+#include <algorithm>
+
+namespace mbpt {
+
+// Logical column i of a two-range operand: p[r * ld + (i < n1 ? off1 + i : off2 + i - n1)]
+struct Cols2 {
+  const double *p;
+  int ld, n1, off1, off2;
+  __device__ __forceinline__ int col(int i) const { return i < n1 ? off1 + i : off2 + (i - n1); }
+};
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f64x4 mma64(double a, double b, f64x4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// ---- C (a x b) = A^T B over rows [r0, r1), one wave per (64 x 32 tile, chunk)
+//
+// Lane l holds columns i = 16 f + l % 16 of A (f < 4) and j = 16 g + l % 16
+// of B (g < 2).  A step covers 4 rows: lane l reads row base + l / 16, which
+// is the k index of its operand in all eight MFMAs (f, g) of the step.
+// Loads for the next step are issued before the current step's MFMAs.
+__global__ __launch_bounds__(64) void k_tns_partial(int n, int a, int b, int chunk, Cols2 A,
+                                                   Cols2 B, double *__restrict__ P, int nta,
+                                                   int ntb, int nblk) {
+  const int lid = xcd_logical(blockIdx.x, nblk);
+  const int ntiles = nta * ntb;
+  const int c = lid / ntiles, t = lid % ntiles;
+  const int ta = t / ntb, tb = t % ntb;
+  const int r0 = c * chunk;
+  if (r0 >= n) return;
+  const int r1 = min(n, r0 + chunk);
+  const int lane = threadIdx.x, lr = lane & 15, lq = lane >> 4;
+  int ca[4], cb[2];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) ca[f] = A.col(min(a - 1, 64 * ta + 16 * f + lr));
+#pragma unroll
+  for (int g = 0; g < 2; ++g) cb[g] = B.col(min(b - 1, 32 * tb + 16 * g + lr));
+  f64x4 acc[4][2];
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int g = 0; g < 2; ++g) acc[f][g] = f64x4{0.0, 0.0, 0.0, 0.0};
+  double va[4], vb[2];
+  int r = r0 + lq;
+  auto load = [&](int row) {
+    const bool in = row < r1;
+    const long ra = (long)(in ? row : r0) * A.ld, rb = (long)(in ? row : r0) * B.ld;
+#pragma unroll
+    for (int f = 0; f < 4; ++f) va[f] = in ? A.p[ra + ca[f]] : 0.0;
+#pragma unroll
+    for (int g = 0; g < 2; ++g) vb[g] = in ? B.p[rb + cb[g]] : 0.0;
+  };
+  load(r);
+  for (int base = r0; base < r1; base += 4) {
+    double xa[4], xb[2];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) xa[f] = va[f];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) xb[g] = vb[g];
+    r += 4;
+    if (base + 4 < r1) load(r);
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int g = 0; g < 2; ++g) acc[f][g] = mma64(xa[f], xb[g], acc[f][g]);
+  }
+  // D[l / 16 + 4 q][l % 16] of each 16 x 16 tile (the f64 MFMA output layout)
+  double *out = P + (long)c * a * b;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int g = 0; g < 2; ++g)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int i = 64 * ta + 16 * f + lq + 4 * q, j = 32 * tb + 16 * g + lr;
+        if (i < a && j < b) out[(long)i * b + j] = acc[f][g][q];
+      }
+}
+
+// Fixed-order sum of the chunk partials: C[i][j] = sum_c P[c][i][j].
+__global__ __launch_bounds__(256) void k_tns_sum(int a, int b, int nc,
+                                                 const double *__restrict__ P,
+                                                 double *__restrict__ C, int ldc) {
+  __shared__ double red[16][17];
+  const long count = (long)a * b;
+  const int el = threadIdx.x & 15, g = threadIdx.x >> 4;
+  for (long base = blockIdx.x * 16L; base < count; base += (long)gridDim.x * 16) {
+    const long e = base + el;
+    double s = 0.0;
+    if (e < count)
+      for (int c = g; c < nc; c += 16) s += P[(long)c * count + e];
+    red[g][el] = s;
+    __syncthreads();
+    if (g == 0 && e < count) {
+      double t = 0.0;
+      for (int q = 0; q < 16; ++q) t += red[q][el];
+      C[(e / b) * ldc + (e % b)] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- C (n x c) = beta C + alpha A (n x k) Y (k x c); with R != null also
+// R = A2 Y - X diag(theta) where X = A Y is the first product (the Ritz
+// vectors and their residuals from one pass over V and AV).
+//
+// One wave per 16 rows and all of c (<= 64, four 16-wide tiles).  Lane l
+// reads row i0 + l % 16 at the four adjacent columns k0 + 4 (l / 16) + s and
+// uses column s in MFMA s; Y's row for that k comes from L1.
+template <bool RITZ>
+__global__ __launch_bounds__(64) void k_nns(int n, int k, int c, Cols2 A, Cols2 A2,
+                                            const double *__restrict__ Y, int ldy,
+                                            const double *__restrict__ theta,
+                                            double *__restrict__ C, int ldc, int c_off,
+                                            double *__restrict__ R, int ldr, double alpha,
+                                            double beta) {
+  const int i0 = blockIdx.x * 16;
+  const int lane = threadIdx.x, lr = lane & 15, lq = lane >> 4;
+  const int row = min(n - 1, i0 + lr);
+  const long ra = (long)row * A.ld, ra2 = (long)row * A2.ld;
+  f64x4 acc[4], acc2[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    acc[g] = f64x4{0.0, 0.0, 0.0, 0.0};
+    acc2[g] = f64x4{0.0, 0.0, 0.0, 0.0};
+  }
+  const int ntile = (c + 15) / 16;
+  for (int k0 = 0; k0 < k; k0 += 16) {
+    double va[4], va2[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kk = k0 + 4 * lq + s;
+      const bool in = kk < k;
+      va[s] = in ? A.p[ra + A.col(kk)] : 0.0;
+      if (RITZ) va2[s] = in ? A2.p[ra2 + A2.col(kk)] : 0.0;
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      if (g >= ntile) break;
+      const int j = 16 * g + lr;
+#pragma unroll
+      for (int s = 0; s < 4; ++s) {
+        const int kk = k0 + 4 * lq + s;
+        const double y = (kk < k && j < c) ? Y[(long)kk * ldy + j] : 0.0;
+        acc[g] = mma64(va[s], y, acc[g]);
+        if (RITZ) acc2[g] = mma64(va2[s], y, acc2[g]);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    if (g >= ntile) break;
+    const int j = 16 * g + lr;
+    if (j >= c) continue;
+    const double th = RITZ ? theta[j] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int i = i0 + lq + 4 * q;
+      if (i >= n) continue;
+      const long e = (long)i * ldc + c_off + j;
+      const double x = alpha * acc[g][q];
+      C[e] = beta == 0.0 ? x : beta * C[e] + x;
+      if (RITZ) R[(long)i * ldr + j] = acc2[g][q] - acc[g][q] * th;
+    }
+  }
+}
+
+// ---- Jacobi-Davidson correction (Solvers/davidson.py:43-77) for k columns:
+// t = K^-1 r - alpha K^-1 u, K = diag - theta floored at `floor` in magnitude,
+// alpha = u.K^-1 r / u.K^-1 u.  Pass 1: per-block partial sums of u.K^-1 u
+// and u.K^-1 r (fixed order); pass 2: every block sums the partials itself
+// and writes t.
+constexpr int kJdRows = 256;
+
+__device__ __forceinline__ double jd_den(double d, double th, double floor) {
+  const double x = d - th;
+  return fabs(x) < floor ? floor : x;
+}
+
+__global__ __launch_bounds__(256) void k_jd_partial(int n, int k, const double *__restrict__ U,
+                                                    int ldu, const double *__restrict__ R,
+                                                    int ldr, const double *__restrict__ dg,
+                                                    const double *__restrict__ theta,
+                                                    double floor, double *__restrict__ part) {
+  __shared__ double su[256], sr[256];
+  const int j = blockIdx.y, tid = threadIdx.x;
+  const int i = blockIdx.x * kJdRows + tid;
+  double uku = 0.0, ukr = 0.0;
+  if (i < n) {
+    const double den = jd_den(dg[i], theta[j], floor);
+    const double u = U[(long)i * ldu + j], r = R[(long)i * ldr + j];
+    uku = u * u / den;
+    ukr = u * r / den;
+  }
+  su[tid] = uku;
+  sr[tid] = ukr;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      su[tid] += su[tid + h];
+      sr[tid] += sr[tid + h];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    part[((long)j * gridDim.x + blockIdx.x) * 2] = su[0];
+    part[((long)j * gridDim.x + blockIdx.x) * 2 + 1] = sr[0];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_jd_apply(int n, int k, int nblk,
+                                                  const double *__restrict__ U, int ldu,
+                                                  const double *__restrict__ R, int ldr,
+                                                  const double *__restrict__ dg,
+                                                  const double *__restrict__ theta,
+                                                  double floor,
+                                                  const double *__restrict__ part,
+                                                  double *__restrict__ T, int ldt) {
+  __shared__ double su[256], sr[256];
+  const int j = blockIdx.y, tid = threadIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int q = tid; q < nblk; q += 256) {
+    a += part[((long)j * nblk + q) * 2];
+    b += part[((long)j * nblk + q) * 2 + 1];
+  }
+  su[tid] = a;
+  sr[tid] = b;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (tid < h) {
+      su[tid] += su[tid + h];
+      sr[tid] += sr[tid + h];
+    }
+    __syncthreads();
+  }
+  const double alpha = fabs(su[0]) > 1e-300 ? sr[0] / su[0] : 0.0;
+  const int i = blockIdx.x * kJdRows + tid;
+  if (i < n) {
+    const double den = jd_den(dg[i], theta[j], floor);
+    T[(long)i * ldt + j] = (R[(long)i * ldr + j] - alpha * U[(long)i * ldu + j]) / den;
+  }
+}
+
+}  // namespace mbpt
+
+using namespace mbpt;
+
+namespace {
+
+int tns_chunk(int n, int a, int b, int *nc_out) {
+  const long tiles = (long)((a + 63) / 64) * ((b + 31) / 32);
+  long nc = std::max<long>(1, 4096 / tiles);
+  nc = std::min<long>(nc, (n + 63) / 64);
+  int chunk = (int)((n + nc - 1) / nc);
+  chunk = (chunk + 3) / 4 * 4;
+  *nc_out = (n + chunk - 1) / chunk;
+  return chunk;
+}
+
+bool cols_ok(int ncol, int ld, int n1, int off1, int off2) {
+  if (ncol < 0 || n1 < 0 || n1 > ncol || off1 < 0 || off2 < 0) return false;
+  if (n1 > 0 && off1 + n1 > ld) return false;
+  if (ncol > n1 && off2 + (ncol - n1) > ld) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+long mbpt_gemm_tn_seg_f64_work(int n, int a, int b) {
+  if (n <= 0 || a <= 0 || b <= 0) return 0;
+  int nc = 0;
+  tns_chunk(n, a, b, &nc);
+  return (long)nc * a * b;
+}
+
+int mbpt_gemm_tn_seg_f64(int n, int a, int b, const double *A, int lda, int a1, int aoff1,
+                         int aoff2, const double *B, int ldb, int b1, int boff1, int boff2,
+                         double *C, int ldc, double *work, long work_elems, void *stream) {
+  MBPT_REQUIRE(n > 0 && a >= 0 && b >= 0, "mbpt_gemm_tn_seg_f64: n must be positive, a and b non-negative");
+  if (a == 0 || b == 0) return MBPT_OK;
+  MBPT_REQUIRE(A && B && C && work, "mbpt_gemm_tn_seg_f64: NULL argument");
+  MBPT_REQUIRE(cols_ok(a, lda, a1, aoff1, aoff2) && cols_ok(b, ldb, b1, boff1, boff2),
+               "mbpt_gemm_tn_seg_f64: a column range runs past its row");
+  MBPT_REQUIRE(ldc >= b, "mbpt_gemm_tn_seg_f64: ldc < b");
+  int nc = 0;
+  const int chunk = tns_chunk(n, a, b, &nc);
+  MBPT_REQUIRE(work_elems >= (long)nc * a * b, "mbpt_gemm_tn_seg_f64: work buffer too small");
+  hipStream_t st = (hipStream_t)stream;
+  const int nta = (a + 63) / 64, ntb = (b + 31) / 32;
+  const int nblk = pad_grid((long)nta * ntb * nc);
+  k_tns_partial<<<nblk, 64, 0, st>>>(n, a, b, chunk, Cols2{A, lda, a1, aoff1, aoff2},
+                                     Cols2{B, ldb, b1, boff1, boff2}, work, nta, ntb, nblk);
+  int rc = launch_status("k_tns_partial");
+  if (rc) return rc;
+  const long count = (long)a * b;
+  k_tns_sum<<<(int)std::min<long>((count + 15) / 16, 8192L), 256, 0, st>>>(a, b, nc, work, C,
+                                                                          ldc);
+  return launch_status("k_tns_sum");
+}
+
+int mbpt_gemm_nn_seg_f64(int n, int k, int c, const double *A, int lda, int k1, int aoff1,
+                         int aoff2, const double *Y, int ldy, double *C, int ldc, int c_off,
+                         double alpha, double beta, void *stream) {
+  MBPT_REQUIRE(n > 0 && k >= 0 && c >= 0 && c <= 64, "mbpt_gemm_nn_seg_f64: need n > 0, k >= 0, 0 <= c <= 64");
+  if (c == 0) return MBPT_OK;
+  MBPT_REQUIRE(A && Y && C, "mbpt_gemm_nn_seg_f64: NULL argument");
+  MBPT_REQUIRE(cols_ok(k, lda, k1, aoff1, aoff2), "mbpt_gemm_nn_seg_f64: a column range runs past its row");
+  MBPT_REQUIRE(ldy >= c && c_off >= 0 && c_off + c <= ldc, "mbpt_gemm_nn_seg_f64: bad ldy, ldc or c_off");
+  hipStream_t st = (hipStream_t)stream;
+  const Cols2 a{A, lda, k1, aoff1, aoff2};
+  k_nns<false><<<(n + 15) / 16, 64, 0, st>>>(n, k, c, a, a, Y, ldy, nullptr, C, ldc, c_off,
+                                             nullptr, 0, alpha, beta);
+  return launch_status("k_nns");
+}
+
+int mbpt_ritz_residual_f64(int n, int m, int k, const double *V, int ldv, int voff,
+                           const double *AV, int ldav, int avoff, const double *Y, int ldy,
+                           const double *theta, double *X, int ldx, double *R, int ldr,
+                           void *stream) {
+  MBPT_REQUIRE(n > 0 && m >= 0 && k >= 0 && k <= 64, "mbpt_ritz_residual_f64: need n > 0, m >= 0, 0 <= k <= 64");
+  if (k == 0) return MBPT_OK;
+  MBPT_REQUIRE(V && AV && Y && theta && X && R, "mbpt_ritz_residual_f64: NULL argument");
+  MBPT_REQUIRE(voff >= 0 && voff + m <= ldv && avoff >= 0 && avoff + m <= ldav,
+               "mbpt_ritz_residual_f64: a column range runs past its row");
+  MBPT_REQUIRE(ldy >= k && ldx >= k && ldr >= k, "mbpt_ritz_residual_f64: leading dimension too small");
+  hipStream_t st = (hipStream_t)stream;
+  k_nns<true><<<(n + 15) / 16, 64, 0, st>>>(n, m, k, Cols2{V, ldv, m, voff, 0},
+                                            Cols2{AV, ldav, m, avoff, 0}, Y, ldy, theta, X,
+                                            ldx, 0, R, ldr, 1.0, 0.0);
+  return launch_status("k_nns<ritz>");
+}
+
+long mbpt_jd_correction_f64_work(int n, int k) {
+  return 2L * k * ((n + kJdRows - 1) / kJdRows);
+}
+
+int mbpt_jd_correction_f64(int n, int k, const double *U, int ldu, const double *R, int ldr,
+                           const double *diag, const double *theta, double floor, double *T,
+                           int ldt, double *work, long work_elems, void *stream) {
+  MBPT_REQUIRE(n > 0 && k >= 0 && k <= 65535, "mbpt_jd_correction_f64: need n > 0, 0 <= k <= 65535");
+  if (k == 0) return MBPT_OK;
+  MBPT_REQUIRE(U && R && diag && theta && T && work, "mbpt_jd_correction_f64: NULL argument");
+  MBPT_REQUIRE(ldu >= k && ldr >= k && ldt >= k, "mbpt_jd_correction_f64: leading dimension too small");
+  MBPT_REQUIRE(floor > 0.0, "mbpt_jd_correction_f64: floor must be positive");
+  const int nblk = (n + kJdRows - 1) / kJdRows;
+  MBPT_REQUIRE(work_elems >= 2L * k * nblk, "mbpt_jd_correction_f64: work buffer too small");
+  hipStream_t st = (hipStream_t)stream;
+  k_jd_partial<<<dim3(nblk, k), 256, 0, st>>>(n, k, U, ldu, R, ldr, diag, theta, floor, work);
+  int rc = launch_status("k_jd_partial");
+  if (rc) return rc;
+  k_jd_apply<<<dim3(nblk, k), 256, 0, st>>>(n, k, nblk, U, ldu, R, ldr, diag, theta, floor,
+                                            work, T, ldt);
+  return launch_status("k_jd_apply");
+}
+
+}  // extern "C"

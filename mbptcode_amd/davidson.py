@@ -30,7 +30,7 @@ import warnings
 import numpy as np
 import torch
 
-from .subspace import tn
+from .subspace import Ranges, jd_correction, nn_ranges, ritz_residual, tn, tn_ranges
 from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
                         DAVIDSON_GUESS_DEG_TOL, DAVIDSON_GUESS_FACTOR,
                         DAVIDSON_LINDEP_CEILING, DAVIDSON_LINDEP_FLOOR,
@@ -262,6 +262,67 @@ class DavidsonResult:
             setattr(self, k, kw.get(k))
 
 
+# The iteration runs on device tensors through dvd.hip; the same steps on
+# CPU tensors (plain torch) let its host logic be tested without a GPU.
+
+def _cols(k):
+    """Column blocks of at most 64 (the width of the device update kernels)."""
+    return [(a, min(64, k - a)) for a in range(0, k, 64)]
+
+
+def _tnr(A, B, out=None):
+    """A^T B for two `Ranges`."""
+    if A.buf.is_cuda:
+        return tn_ranges(A, B, out=out)
+    r = A.columns().T @ B.columns()
+    if out is None:
+        return r
+    out.copy_(r)
+    return out
+
+
+def _nn(A, Y, C, c_off, alpha=1.0, beta=0.0):
+    """C[:, c_off : c_off + c] = beta C + alpha A Y, in blocks of 64 columns."""
+    if not A.buf.is_cuda:
+        tgt = C[:, c_off:c_off + Y.shape[1]]
+        r = alpha * (A.columns() @ Y)
+        tgt.copy_(r if beta == 0.0 else beta * tgt + r)
+        return
+    for a, c in _cols(Y.shape[1]):
+        nn_ranges(A, Y[:, a:a + c].contiguous() if Y.shape[1] > 64 else Y, C, c_off + a,
+                  alpha, beta)
+
+
+def _ritz(VA, m, cap, Y, theta):
+    """(X, R) = (V Y, AV Y - X diag(theta)) for the m columns of V and AV in
+    the trial-space buffer VA (n, 2 cap): (n, k) each."""
+    n, k = VA.shape[0], Y.shape[1]
+    if not VA.is_cuda:
+        X = VA[:, :m] @ Y
+        return X, VA[:, cap:cap + m] @ Y - X * theta[None, :]
+    X = torch.empty((n, k), dtype=torch.float64, device=VA.device)
+    R = torch.empty_like(X)
+    for a, c in _cols(k):
+        if k <= 64:
+            ritz_residual(VA, m, 0, cap, Y, theta, X, R)
+        else:
+            Xc = torch.empty((n, c), dtype=torch.float64, device=VA.device)
+            Rc = torch.empty_like(Xc)
+            ritz_residual(VA, m, 0, cap, Y[:, a:a + c].contiguous(),
+                          theta[a:a + c].contiguous(), Xc, Rc)
+            X[:, a:a + c], R[:, a:a + c] = Xc, Rc
+    return X, R
+
+
+def _jd(U, R, diag, theta):
+    """The Jacobi-Davidson corrections of the columns of R (n, k)."""
+    if not R.is_cuda:
+        return jacobi_davidson_correction(R, U, diag, theta)
+    out = torch.empty_like(R)
+    jd_correction(U, R, diag, theta, out, floor=DAVIDSON_DENOMINATOR_FLOOR)
+    return out
+
+
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
              lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None,
@@ -271,17 +332,25 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
 
     matmat(V (n, m)) -> A V (n, m), FP64 device tensors.  diag (n,) device.
     x0 (n, m0) guess columns.  precond(R, U, theta, cycle) -> corrections; the
-    default is the Jacobi-Davidson one on `diag`.  When the space would exceed
-    `max_space` it collapses onto the current Ritz vectors.  `on_stall` is
-    called when the largest unconverged residual has not fallen by 10x over
-    `stall_cycles` cycles (the learned preconditioner's fallback hook).
+    default is the Jacobi-Davidson one on `diag` (one fused device pass).
+    When the space would exceed `max_space` it collapses onto the current
+    Ritz vectors.  `on_stall` is called when the largest unconverged residual
+    has not fallen by 10x over `stall_cycles` cycles (the learned
+    preconditioner's fallback hook).
 
-    The trial space V and its image AV live in preallocated device buffers;
-    the Rayleigh matrix H = V^T A V and G = (AV)^T AV grow block by block on
-    the device, and one (m, 2m) transfer per cycle gives the host both the
-    Ritz pairs (eigh of H) and the residual norms,
-    |A x - theta x|^2 = y^T G y - theta^2 for x = V y, |x| = 1.
-    The returned residual norms are recomputed explicitly from A X - X theta.
+    The trial space V and its image AV are the two column halves of one
+    device buffer, so a product with [V | AV] or [V | T] is one pass over
+    the pair index (dvd.hip).  The Rayleigh matrix H = V^T A V (kept whole:
+    a lower-precision operator is not exactly symmetric) and G = (AV)^T AV
+    grow block by block on the device.  Per cycle the host receives H and G
+    once (Ritz pairs by LAPACK, and the residual norms
+    |A x - theta x|^2 = y^T G y - 2 theta y^T H y + theta^2 for x = V y) and
+    the new block's projections once (its Gram eigensolve); everything else
+    stays on the device.  The corrections are projected off V twice
+    (classical Gram-Schmidt) and orthonormalised among themselves by the
+    eigendecomposition of their Gram matrix; columns left below `drop` of
+    their norm are discarded.  The returned residual norms are recomputed
+    explicitly from A X - X theta.
     `phases`, a dict, receives the seconds of each part of the cycle (the
     device is synchronised at every boundary, so only for profiling).
     `keep`: Ritz vectors kept at a collapse (thick restart; default nroots).
@@ -306,34 +375,18 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     max_space = int(min(max(max_space, 2 * nroots), n))
     lindep = lindep_for(tol) if lindep is None else lindep
     drop = float(np.sqrt(lindep))
-    if precond is None:
-        def precond(R, U, theta, cycle):
-            return jacobi_davidson_correction(R, U, diag, theta)
+    use_jd = precond is None
     t0 = time.time()
     T0 = _orthonormal_new(x0, None, 1e-12)
-    cap = max_space + max(T0.shape[1], nroots)
-    Vb = torch.empty((n, cap), dtype=torch.float64, device=dev)
-    AVb = torch.empty_like(Vb)
-    Hb = torch.zeros((cap, cap), dtype=torch.float64, device=dev)
-    Gb = torch.zeros_like(Hb)
-    m = 0
-
-    def append(T, AT):
-        nonlocal m
-        p = T.shape[1]
-        Vb[:, m:m + p] = T
-        AVb[:, m:m + p] = AT
-        # Hb holds V^T (A V) whole: a lower-precision operator is not exactly
-        # symmetric, and x^T A x must be the one the residual sees.
-        Hb[:m + p, m:m + p] = _tn(Vb[:, :m + p], AT)
-        if m:
-            Hb[m:m + p, :m] = _tn(T, AVb[:, :m])
-        blkG = _tn(AVb[:, :m + p], AT)
-        Gb[:m + p, m:m + p] = blkG
-        Gb[m:m + p, :m] = blkG[:m].T
-        m += p
-
-    append(T0, matmat(T0))
+    cap = max_space + max(T0.shape[1], 2 * nroots)
+    VA = torch.empty((n, 2 * cap), dtype=torch.float64, device=dev)    # [V | AV]
+    HG = torch.zeros((cap, 2 * cap), dtype=torch.float64, device=dev)  # [H | G]
+    m = T0.shape[1]
+    VA[:, :m] = T0
+    VA[:, cap:cap + m] = matmat(T0)
+    P = _tnr(Ranges(VA, (0, m), (cap, m)), Ranges(VA, (cap, m)))
+    HG[:m, :m] = P[:m]
+    HG[:m, cap:cap + m] = P[m:]
     vectors = m
     span = getattr(pick, 'span', None)
     if span is not None:
@@ -344,10 +397,11 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         if pick is None:
             return evals[:k], evecs[:, :k], np.arange(k)
         if span is not None:
-            envs = {'ref_coeff': _tn(Vb[:, :m], span).cpu().numpy()}
+            envs = {'ref_coeff': _tn(VA[:, :m], span).cpu().numpy()}
         else:
-            envs = {'xs': list(Vb[:, :m].T.cpu().numpy())}
+            envs = {'xs': list(VA[:, :m].T.cpu().numpy())}
         return pick(evals, evecs, k, envs)
+
     trace = []
     conv = np.zeros(nroots, bool)
     w = rn = None
@@ -355,14 +409,15 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     stalled = False
     cycles = 0
     y = None
+    y_rows = m
     mark('setup')
     for cycle in range(max_cycle):
         cycles = cycle + 1
-        HG = torch.cat([Hb[:m, :m], Gb[:m, :m]], dim=1).cpu().numpy()
+        hg = HG[:m].cpu().numpy()
+        Hns, G = hg[:, :m], hg[:, cap:cap + m]
         mark('rr_transfer')
-        Hns = HG[:, :m]
         H = 0.5 * (Hns + Hns.T)
-        G = 0.5 * (HG[:, m:] + HG[:, m:].T)
+        G = 0.5 * (G + G.T)
         with _host_lapack():
             evals, evecs = np.linalg.eigh(H)
         k = min(nroots, m)
@@ -379,9 +434,9 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         floor = np.sqrt(1e3 * np.finfo(float).eps * np.maximum(sq, w ** 2))
         fuzzy = np.flatnonzero(rn < 10 * floor)
         if fuzzy.size and tol < 10 * floor.max():
-            yf = torch.as_tensor(y[:, fuzzy], device=dev)
-            wf = torch.as_tensor(w[fuzzy], device=dev)
-            rn[fuzzy] = (AVb[:, :m] @ yf - (Vb[:, :m] @ yf) * wf[None, :]).norm(dim=0).cpu().numpy()
+            yf = torch.as_tensor(np.ascontiguousarray(y[:, fuzzy]), device=dev)
+            wf = torch.as_tensor(np.ascontiguousarray(w[fuzzy]), device=dev)
+            rn[fuzzy] = _ritz(VA, m, cap, yf, wf)[1].norm(dim=0).cpu().numpy()
         conv = rn <= tol
         mark('rr_eigh')
         trace.append({'cycle': cycle, 'space': int(m), 'max_residual': float(rn.max()),
@@ -395,67 +450,133 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
                 stalled = True
                 new_pre = on_stall(cycle)
                 if new_pre is not None:
-                    precond = new_pre
+                    precond, use_jd = new_pre, False
                 hist = []
         idx = np.flatnonzero(~conv)
-        yu = torch.as_tensor(y[:, idx], device=dev)
-        Xu = Vb[:, :m] @ yu
-        AXu = AVb[:, :m] @ yu
-        wu = torch.as_tensor(w[idx], device=dev)
-        Ru = AXu - Xu * wu[None, :]
+        ku = len(idx)
+        # the unconverged Ritz vectors and their residuals, one pass over V, AV
+        Yt = np.empty((m + 1, ku))
+        Yt[:m], Yt[m] = y[:, idx], w[idx]
+        Yd = torch.as_tensor(Yt, device=dev)
+        wu = Yd[m]
+        Xu, Ru = _ritz(VA, m, cap, Yd[:m], wu)
         mark('residuals')
         if observe is not None:
-            observe(cycle, Vb[:, :m], Ru, Xu, wu)
-        Tc = precond(Ru, Xu, wu, cycle)
+            observe(cycle, VA[:, :m], Ru, Xu, wu)
+
+        def correction(pre, jd):
+            if jd:
+                return _jd(Xu, Ru, diag, wu)
+            return pre(Ru, Xu, wu, cycle).contiguous()
+        Tc = correction(precond, use_jd)
         mark('precond')
-        T = _orthonormal_new(Tc, Vb[:, :m], drop)
-        mark('orthonormal')
-        if T.shape[1] == 0 and on_stall is not None and not stalled:
+        p = Tc.shape[1]
+        if m + p > max_space:
+            # collapse onto the selected and then the lowest Ritz vectors
+            kk = min(m, max(k, keep or k))
+            cols = list(order) + [j for j in range(m) if j not in set(order)]
+            yk = np.ascontiguousarray(evecs[:, cols[:kk]])
+            ykd = torch.as_tensor(yk, device=dev)
+            Xc = torch.empty((n, kk), dtype=torch.float64, device=dev)
+            AXc = torch.empty_like(Xc)
+            _nn(Ranges(VA, (0, m)), ykd, Xc, 0)
+            _nn(Ranges(VA, (cap, m)), ykd, AXc, 0)
+            VA[:, :kk] = Xc
+            VA[:, cap:cap + kk] = AXc
+            # the host copies first: on a CPU tensor `hg` views HG itself
+            Hns, Gc = yk.T @ Hns @ yk, yk.T @ G @ yk
+            HG[:kk, :kk] = torch.as_tensor(Hns, device=dev)
+            HG[:kk, cap:cap + kk] = torch.as_tensor(Gc, device=dev)
+            m = kk
+        mark('collapse')
+        q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop)
+        if q == 0 and on_stall is not None and not stalled:
             # the correction added nothing new: the same fallback as a stall
             stalled = True
             new_pre = on_stall(cycle)
             if new_pre is not None:
-                precond = new_pre
-            T = _orthonormal_new(precond(Ru, Xu, wu, cycle), Vb[:, :m], drop)
-        if T.shape[1] == 0:
+                precond, use_jd = new_pre, False
+            Tc = correction(precond, use_jd)
+            q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop)
+        mark('orthonormal')
+        if q == 0:
             break
-        if m + T.shape[1] > max_space:
-            # collapse onto the lowest `keep` Ritz vectors (T is already
-            # orthogonal to the whole space)
-            kk = min(m, max(k, keep or k))
-            cols = list(order) + [j for j in range(m) if j not in set(order)]
-            yk = evecs[:, cols[:kk]]
-            yt = torch.as_tensor(yk, device=dev)
-            X = Vb[:, :m] @ yt
-            AX = AVb[:, :m] @ yt
-            Hc = torch.as_tensor(yk.T @ Hns @ yk, device=dev)
-            Gc = torch.as_tensor(yk.T @ G @ yk, device=dev)
-            Vb[:, :kk] = X
-            AVb[:, :kk] = AX
-            Hb[:kk, :kk] = Hc
-            Gb[:kk, :kk] = Gc
-            m = kk
-        mark('collapse')
-        AT = matmat(T)
+        AT = matmat(VA[:, m:m + q])
         mark('sigma')
-        vectors += T.shape[1]
-        append(T, AT)
+        vectors += q
+        VA[:, cap + m:cap + m + q] = AT
+        P = _tnr(Ranges(VA, (0, m + q), (cap, m + q)), Ranges(VA, (cap + m, q)))
+        HG[:m + q, m:m + q] = P[:m + q]
+        HG[:m + q, cap + m:cap + m + q] = P[m + q:]
+        HG[m:m + q, :m] = Hrow
+        HG[m:m + q, cap:cap + m] = P[m + q:m + q + m].T
+        m += q
         mark('append')
     if y_rows != m:
         # the space grew after the last Rayleigh-Ritz step (cycle budget ran out)
-        H = Hb[:m, :m].cpu().numpy()
+        H = HG[:m, :m].cpu().numpy()
         evals, evecs = np.linalg.eigh(0.5 * (H + H.T))
         k = min(nroots, m)
         w, y, _ = ritz_select(evals, evecs, k)
-    yt = torch.as_tensor(y, device=dev)
-    X = Vb[:, :m] @ yt
-    AX = AVb[:, :m] @ yt
-    wt = torch.as_tensor(w, device=dev)
-    rn = (AX - X * wt[None, :]).norm(dim=0).cpu().numpy()
+    yd = torch.as_tensor(np.ascontiguousarray(y), device=dev)
+    wt = torch.as_tensor(np.ascontiguousarray(w), device=dev)
+    X, R = _ritz(VA, m, cap, yd, wt)
+    rn = R.norm(dim=0).cpu().numpy()
     conv = rn <= tol
     return DavidsonResult(e=np.asarray(w), X=X, conv=conv, rnorm=rn, cycles=cycles,
                           vectors=vectors, seconds=time.time() - t0, trace=trace,
                           stalled=stalled)
+
+
+def _orthonormalize_block(VA, m, cap, Tc, Hns, drop):
+    """Append the corrections Tc (n, p) orthonormalised against the m columns
+    of V in VA (n, 2 cap) as V's next columns.  Returns (q, C, Hrow): the
+    number of columns kept, the device (p, q) map from Tc's projection to
+    them, and the device block (q, m) = T^T AV of the rows they add to H.
+
+    Two classical Gram-Schmidt passes, each one product [V | T]^T T over the
+    pair index; the first also gives AV^T T.  The projected block's Gram
+    matrix G2 = T1^T T1 - S2^T S2 (V orthonormal) goes to the host with the
+    projections in one transfer: a column whose norm falls below `drop` of
+    its norm before the projection is dropped, the rest are orthonormalised
+    by G2's eigendecomposition (T2 C).  T^T AV follows from the algebra
+    AV^T T2 = AV^T T - H^T (S1 + S2), H = V^T AV the whole Rayleigh matrix
+    (the host copy `Hns`)."""
+    dev = VA.device
+    n, p = Tc.shape
+    VA[:, m:m + p] = Tc
+    # pass 1: [V | T | AV]^T T -> S1 = V^T T, G0 = T^T T, Pav = AV^T T
+    stage = torch.empty((3 * m + 2 * p, p), dtype=torch.float64, device=dev)
+    P1 = stage[:2 * m + p]
+    _tnr(Ranges(VA, (0, m + p), (cap, m)), Ranges(VA, (m, p)), out=P1)
+    _nn(Ranges(VA, (0, m)), P1[:m], VA, m, alpha=-1.0, beta=1.0)
+    # pass 2: [V | T1]^T T1 -> S2 = V^T T1, G1 = T1^T T1
+    P2 = stage[2 * m + p:]
+    _tnr(Ranges(VA, (0, m + p)), Ranges(VA, (m, p)), out=P2)
+    _nn(Ranges(VA, (0, m)), P2[:m], VA, m, alpha=-1.0, beta=1.0)
+    h = stage.cpu().numpy()
+    S1, G0, Pav = h[:m], h[m:m + p], h[m + p:2 * m + p]
+    S2, G1 = h[2 * m + p:3 * m + p], h[3 * m + p:]
+    G2 = G1 - S2.T @ S2
+    G2 = 0.5 * (G2 + G2.T)
+    before = np.sqrt(np.clip(np.diag(G0), 0.0, None))
+    after = np.sqrt(np.clip(np.diag(G2), 0.0, None))
+    keep = np.flatnonzero(after > drop * np.maximum(before, 1e-300))
+    if keep.size == 0:
+        return 0, None, None
+    with _host_lapack():
+        s, U = np.linalg.eigh(G2[np.ix_(keep, keep)])
+    good = s > 1e-10 * s.max()
+    q = int(good.sum())
+    C = np.zeros((p, q))
+    C[keep] = U[:, good] / np.sqrt(s[good])
+    Hrow = ((Pav - Hns.T @ (S1 + S2)) @ C).T
+    up = torch.as_tensor(np.concatenate([C.ravel(), Hrow.ravel()]), device=dev)
+    Cd, Hd = up[:p * q].reshape(p, q), up[p * q:].reshape(q, m)
+    Tn = torch.empty((n, q), dtype=torch.float64, device=dev)
+    _nn(Ranges(VA, (m, p)), Cd, Tn, 0)
+    VA[:, m:m + q] = Tn
+    return q, Cd, Hd
 
 
 class overlap_pick:
