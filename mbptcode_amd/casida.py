@@ -123,7 +123,7 @@ def symmetric_coefficients(s11, s21, lindep):
 
 class CasidaResult:
     __slots__ = ('e', 'X', 'Y', 'conv', 'rnorm', 'cycles', 'vectors', 'seconds',
-                 'trace')
+                 'trace', 'exhausted')
 
     def __init__(self, **kw):
         for k in self.__slots__:
@@ -167,6 +167,7 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     b, sg, pi = np.zeros_like(a), np.zeros_like(a), np.zeros_like(a)
     m = 0
     vectors = 0
+    exhausted = False
 
     def act(V, W):
         nonlocal vectors
@@ -235,8 +236,15 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     for cycle in range(max_cycle):
         cycles = cycle + 1
         k = min(nroots, m)
-        with _host_lapack():
-            w, x, y = subspace_solve(a[:m, :m], b[:m, :m], sg[:m, :m], pi[:m, :m], k)
+        try:
+            with _host_lapack():
+                w, x, y = subspace_solve(a[:m, :m], b[:m, :m], sg[:m, :m], pi[:m, :m], k)
+        except np.linalg.LinAlgError as exc:
+            # whether the trial space had spanned the pair space when the
+            # projected (A - B) block broke down (the caller may then solve
+            # densely)
+            exc.space_exhausted = m >= n or exhausted
+            raise
         mark('subspace')
         xt = torch.as_tensor(x, device=dev)
         yt = torch.as_tensor(y, device=dev)
@@ -264,6 +272,7 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         tx, ty = orthonormal_new(tx / nrm, ty / nrm)
         mark('orthonormal')
         if tx.shape[1] == 0:
+            exhausted = True
             break
         if m + tx.shape[1] > max_space:
             # restart on the lowest Ritz pairs, their products combined from
@@ -284,4 +293,5 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         warnings.warn(f'Casida Davidson: only {len(w)} roots in a space of {m}',
                       RuntimeWarning, stacklevel=2)
     return CasidaResult(e=np.asarray(w), X=X, Y=Y, conv=conv, rnorm=rn, cycles=cycles,
-                        vectors=vectors, seconds=time.time() - t0, trace=trace)
+                        vectors=vectors, seconds=time.time() - t0, trace=trace,
+                        exhausted=exhausted or m >= n)

@@ -60,6 +60,10 @@ DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
 DAVIDSON_COMPLETION_SEED = 0            # :2124 random directions of the completion solve
 DAVIDSON_FOLLOW_OVERLAP_TIE = 1e-6      # :2129 overlaps this close are a tie
 
+#: Largest pair space a Davidson whose trial space ran out is completed in
+#: by a dense solve (constants.py:590).
+BSE_DENSE_MAX_NOV = 12000
+
 # the ISDF fit (constants.py:36, :1755): the metric root's relative eigenvalue
 # floor and the Tikhonov floor of the balanced Gram matrix
 AUX_METRIC_ROOT_FLOOR = 1e-12

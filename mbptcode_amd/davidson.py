@@ -38,7 +38,7 @@ from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
                         DAVIDSON_PRECONDITIONERS, DAVIDSON_TOL_RESIDUAL,
                         DAVIDSON_EXTRA_ROOTS_DIVISOR, DAVIDSON_EXTRA_ROOTS_FLOOR,
                         DAVIDSON_COMPLETION_SEED, DAVIDSON_FOLLOW_OVERLAP_TIE,
-                        DAVIDSON_NULL_DIRECTION)
+                        DAVIDSON_NULL_DIRECTION, BSE_DENSE_MAX_NOV)
 
 
 _BLAS = {}
@@ -223,6 +223,16 @@ def jacobi_davidson_correction(R, U, diag, theta, floor=DAVIDSON_DENOMINATOR_FLO
 # the iteration
 # ---------------------------------------------------------------------------
 
+def _to_host(t):
+    """A device tensor as a NumPy array, through page-locked memory (the
+    copy of the returned eigenvectors runs at the link's rate)."""
+    if not t.is_cuda:
+        return t.numpy()
+    out = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    out.copy_(t)
+    return out.numpy()
+
+
 def _tn(A, B):
     """A^T B: on the device through the split-contraction FP64 kernel
     (the subspace reductions over the pair index), else plain torch."""
@@ -255,7 +265,7 @@ def _orthonormal_new(T, V, drop):
 
 class DavidsonResult:
     __slots__ = ('e', 'X', 'conv', 'rnorm', 'cycles', 'vectors', 'seconds',
-                 'trace', 'stalled')
+                 'trace', 'stalled', 'exhausted')
 
     def __init__(self, **kw):
         for k in self.__slots__:
@@ -406,7 +416,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     conv = np.zeros(nroots, bool)
     w = rn = None
     hist = []
-    stalled = False
+    stalled = exhausted = False
     cycles = 0
     y = None
     y_rows = m
@@ -500,6 +510,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop)
         mark('orthonormal')
         if q == 0:
+            exhausted = True
             break
         AT = matmat(VA[:, m:m + q])
         mark('sigma')
@@ -525,7 +536,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     conv = rn <= tol
     return DavidsonResult(e=np.asarray(w), X=X, conv=conv, rnorm=rn, cycles=cycles,
                           vectors=vectors, seconds=time.time() - t0, trace=trace,
-                          stalled=stalled)
+                          stalled=stalled, exhausted=exhausted or m >= n)
 
 
 def _orthonormalize_block(VA, m, cap, Tc, Hns, drop):
@@ -719,6 +730,63 @@ def solve_symmetric(matvec, diag, nroots=1, x0=None, pick=None,
     return e, X, conv
 
 
+def dense_casida(apply_AB, diag_d, nroots, conv_tol, batch=64, tda=False):
+    """(omega, X, Y, converged) of the `nroots` lowest roots from A (and B)
+    built whole out of a device block action, one unit vector per pair
+    (LR/davidson.py:1296-1330, `_dense_casida`): the completion of a
+    Davidson whose trial space ran out.  Tamm-Dancoff: eigh(A), Y = 0.  Full:
+    with A - B = L L^T (a Cholesky failure raises CasidaInstability),
+    L^T (A + B) L z = omega^2 z, X + Y = L z / omega^1/2 and
+    X - Y = L^-T z omega^1/2, so that X^T X - Y^T Y = 1.  FP64 on the
+    device; `converged` is each root's residual against conv_tol.  X, Y
+    are device (n_pair, nroots)."""
+    op = _operator_of(apply_AB)
+    no, nv = np.shape(diag_d)
+    n = no * nv
+    if n > BSE_DENSE_MAX_NOV:
+        raise ValueError(f'a dense solve of {n} pairs exceeds BSE_DENSE_MAX_NOV '
+                         f'({BSE_DENSE_MAX_NOV})')
+    dev = op.device
+    A = torch.empty((n, n), dtype=torch.float64, device=dev)
+    B = None if tda else torch.empty_like(A)
+    for c0 in range(0, n, batch):
+        c1 = min(c0 + batch, n)
+        unit = torch.zeros((c1 - c0, n), dtype=torch.float64, device=dev)
+        unit[torch.arange(c1 - c0, device=dev), torch.arange(c0, c1, device=dev)] = 1.0
+        z = unit.reshape(c1 - c0, no, nv)
+        if tda:
+            A[c0:c1] = op.apply(z).reshape(c1 - c0, n)
+        else:
+            Az, Bz = op.apply_ab(z)
+            A[c0:c1], B[c0:c1] = Az.reshape(c1 - c0, n), Bz.reshape(c1 - c0, n)
+    A = 0.5 * (A + A.T)
+    k = min(nroots, n)
+    if tda:
+        w, V = torch.linalg.eigh(A)
+        X = V[:, :k].contiguous()
+        w = w[:k]
+        r = (A @ X - X * w[None, :]).norm(dim=0)
+        Y = torch.zeros_like(X)
+    else:
+        B = 0.5 * (B + B.T)
+        L, info = torch.linalg.cholesky_ex(A - B)
+        if int(info) != 0:
+            raise CasidaInstability('(A - B) is not positive definite on the pair space')
+        H = L.T @ (A + B) @ L
+        w2, Z = torch.linalg.eigh(0.5 * (H + H.T))
+        if not float(w2[0]) > 0:
+            raise CasidaInstability('omega^2 is not positive on the pair space')
+        w = torch.sqrt(w2[:k])
+        Z = Z[:, :k]
+        plus = (L @ Z) / torch.sqrt(w)[None, :]
+        minus = torch.linalg.solve_triangular(L.T, Z, upper=True) * torch.sqrt(w)[None, :]
+        X, Y = 0.5 * (plus + minus), 0.5 * (plus - minus)
+        r = torch.sqrt(((A @ X + B @ Y - X * w[None, :]) ** 2).sum(0) +
+                       ((B @ X + A @ Y + Y * w[None, :]) ** 2).sum(0))
+    rn = r.cpu().numpy()
+    return w.cpu().numpy(), X, Y, rn <= conv_tol, rn
+
+
 # ---------------------------------------------------------------------------
 # the TDA driver (LR/davidson.py)
 # ---------------------------------------------------------------------------
@@ -823,6 +891,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                    on_stall=on_stall, phases=phases, keep=keep)
     cycles, vectors = res.cycles, res.vectors
     e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
+    exhausted = res.exhausted
     refine = {}
     if refine_apply is not None:
         # the FP64 check covers the requested roots; the extra tracked ones
@@ -839,6 +908,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
             res2 = davidson(mm64, pdiag, track, X, tol=conv_tol, max_cycle=max_cycle,
                             max_space=max_space, phases=phases)
             e, X, conv, rn = res2.e, res2.X, res2.conv, res2.rnorm
+            exhausted = res2.exhausted
             refine['cycles'] = res2.cycles
             refine['vectors'] = res2.vectors
             cycles += res2.cycles
@@ -851,6 +921,15 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
             rn[:nroots] = rn64
             conv = rn <= conv_tol
         refine['rnorm_after'] = np.asarray(rn64).copy()
+    dense = False
+    if not np.all(conv[:nroots]) and exhausted and npair <= BSE_DENSE_MAX_NOV:
+        # the trial space ran out short of the tolerance: the pair space is
+        # small enough to finish densely (LR/davidson.py:1147-1193)
+        act = refine_apply if refine_apply is not None else apply_AB
+        e, X, _, conv, rn = dense_casida(act, diag_d, track, conv_tol, tda=True)
+        dense = True
+        cycles += 1
+        vectors += npair
     unconfirmed = np.zeros(nroots, bool)
     extra = np.flatnonzero(~conv[nroots:]) + nroots
     if extra.size:
@@ -859,7 +938,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     seconds = time.time() - t_all
     if stats is not None:
         stats.update({'davidson_vind_calls': cycles, 'davidson_block_actions': vectors,
-                      'davidson_dense_completion': False})
+                      'davidson_dense_completion': dense})
     if timings is not None:
         timings.update({'davidson_s': seconds, 'davidson_iterations': cycles,
                         'davidson_vectors': vectors, 'davidson_precond_s': t_precond,
@@ -880,7 +959,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
         if refuse_unconverged:
             raise DavidsonNotConverged(msg)
         warnings.warn(msg, RuntimeWarning, stacklevel=2)
-    Xn = X[:, :nroots].cpu().numpy()
+    Xn = _to_host(X[:, :nroots])
     return np.asarray(e[:nroots]), Xn, np.zeros_like(Xn)
 
 
@@ -932,15 +1011,33 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
         return ab
 
     probe_on = apply_AB if refine_apply is None else refine_apply
+
+    def complete_densely(why):
+        """The dense finish of a pair space the trial space ran out on
+        (LR/davidson.py:1147-1193); an unstable reference refuses there."""
+        try:
+            return dense_casida(probe_on, diag_d, track, conv_tol, tda=False)
+        except CasidaInstability as exc:
+            raise UnstableReference(f'completed densely ({why}): {exc}', None) from exc
+
+    dense = None
     try:
         res = casida_davidson(ab_of(op), pdiag, track, x0, tol=conv_tol,
                               max_cycle=max_cycle, max_space=max_space, y0=y0)
     except np.linalg.LinAlgError as exc:
-        raise casida_breakdown(probe_on, diag_d, conv_tol) from exc
-    cycles, vectors = res.cycles, res.vectors
-    e, X, Y, conv, rn = np.asarray(res.e, float), res.X, res.Y, res.conv, res.rnorm
+        if not (getattr(exc, 'space_exhausted', False) and npair <= BSE_DENSE_MAX_NOV):
+            raise casida_breakdown(probe_on, diag_d, conv_tol) from exc
+        dense = complete_densely('the projected (A - B) broke down')
+        res = None
+    if res is None:
+        cycles, vectors = 1, npair
+        e, X, Y, conv, rn = dense
+        trace = []
+    else:
+        cycles, vectors, trace = res.cycles, res.vectors, res.trace
+        e, X, Y, conv, rn = np.asarray(res.e, float), res.X, res.Y, res.conv, res.rnorm
     refine = {}
-    if refine_apply is not None:
+    if refine_apply is not None and dense is None:
         op64 = _operator_of(refine_apply)
         ab64 = ab_of(op64)
         Xr, Yr = X[:, :nroots].contiguous(), Y[:, :nroots].contiguous()
@@ -970,6 +1067,12 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
             conv = rn <= conv_tol
         refine['rnorm_after'] = np.asarray(rn64).copy()
     conv = np.asarray(conv, bool)
+    if (dense is None and res is not None and not conv[:nroots].all() and res.exhausted
+            and npair <= BSE_DENSE_MAX_NOV):
+        dense = complete_densely('the trial space ran out')
+        e, X, Y, conv, rn = dense
+        cycles += 1
+        vectors += npair
     if len(e) < nroots:
         raise RuntimeError(f'Casida Davidson found {len(e)} of {nroots} roots')
     unconfirmed = np.zeros(nroots, bool)
@@ -979,14 +1082,14 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
         unconfirmed = e[:nroots] > lower
     if stats is not None:
         stats.update({'davidson_vind_calls': cycles, 'davidson_block_actions': vectors,
-                      'davidson_dense_completion': False})
+                      'davidson_dense_completion': dense is not None})
     if timings is not None:
         timings.update({'davidson_s': time.time() - t_all, 'davidson_iterations': cycles,
                         'davidson_vectors': vectors,
                         'davidson_converged': conv[:nroots].tolist(),
                         'davidson_residuals': np.asarray(rn[:nroots]).tolist(),
                         'davidson_unconfirmed': unconfirmed.tolist(),
-                        'davidson_trace': res.trace,
+                        'davidson_trace': trace,
                         'davidson_refine': {k: (v.tolist() if hasattr(v, 'tolist') else v)
                                             for k, v in refine.items()},
                         'davidson_flags': counters})

@@ -176,3 +176,29 @@ def test_solve_casida_davidson_entry(tda, probe):
         assert stats['min_eig_amb'] > 0
         if probe is True:
             assert abs(stats['min_eig_amb'] - amb) < 1e-9
+
+
+@pytest.mark.parametrize('tda', [True, False])
+def test_dense_completion_against_the_oracle(tda):
+    """`dense_casida`, the finish of a pair space the trial space ran out on
+    (LR/davidson.py:1147-1193, :1296-1330): A (and B) built whole from the
+    device action on unit vectors, then the dense eigensolve; against the
+    oracle's dense roots, with the Casida equations' residuals and
+    X^T X - Y^T Y = 1 checked on the returned vectors."""
+    from mbptcode_amd.block_actions import grid_block_action
+    from mbptcode_amd.davidson import dense_casida
+    inst = synth.config(0)
+    D, W_aux, W_grid = screened(inst)
+    A, B = dense(inst, D, W_aux)
+    act, d = grid_block_action(inst.eps, inst.nocc, inst.X, W_grid, inst.V, precision='fp64',
+                               tda=tda)
+    print(f'dense completion tda={tda}', flush=True)
+    w, X, Y, conv, rn = dense_casida(act, d, 6, 1e-8, batch=17, tda=tda)
+    ref = np.linalg.eigvalsh(A)[:6] if tda else O.dense_casida_roots(A, B)[0][:6]
+    assert np.abs(w - ref).max() < 1e-10 and conv.all()
+    Xh, Yh = X.cpu().numpy(), Y.cpu().numpy()
+    if tda:
+        assert not Yh.any()
+        assert np.abs(A @ Xh - Xh * w).max() < 1e-10
+    else:
+        check_pairs(A, B, w, Xh, Yh, 1e-10)
