@@ -60,47 +60,88 @@ def parse():
     p.add_argument('--no-legs', action='store_true', help='headline only')
     p.add_argument('--leg', choices=['benzene', 'large', 'casida', 'gw', 'fit', 'batched'],
                    help='run this side leg alone and print its JSON (no headline line)')
-    p.add_argument('--kernel-probe', action='store_true',
-                   help='only a few sigma applies of the headline instance (k = 20, the '
-                        'mixed precision) and one 256 MiB device copy, for counter runs')
+    p.add_argument('--kernel-probe', nargs='?', const='k20', choices=['k20', 'mix'],
+                   help='counter-run workload, then one 256 MiB device copy: k20, three '
+                        'sigma applies of the headline instance at k = 20 in the mixed '
+                        'precision; mix, one learned+mixed and one screened+mixed solve '
+                        '(the launch sequence the timed steps repeat), printing the k of '
+                        'every mixed-precision apply in order')
     return p.parse_args()
 
 
-#: HBM traffic of the headline exchange kernel, from the counter runs recorded
-#: in profiles/exchange_bf16_headline_pmc.md (FETCH_SIZE + WRITE_SIZE per
-#: trial vector of a k = 20 launch, scaled by the same counters on a copy of
-#: known size)
-TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'exchange_bf16_headline_pmc.md')
+#: HBM traffic of the headline exchange kernel per launch, from the counter
+#: runs over the bench's own launch sequence recorded in
+#: profiles/exchange_bf16_bench_mix_pmc.md (FETCH_SIZE + WRITE_SIZE of every
+#: dispatch, scaled by the same counters on a copy of known size, averaged
+#: over the sequence)
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'exchange_bf16_bench_mix_pmc.md')
 
 
 def recorded_traffic():
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
-                if line.startswith('traffic_bytes_per_vector:'):
+                if line.startswith('traffic_bytes_per_launch:'):
                     return float(line.split(':', 1)[1])
     except OSError:
         pass
     return None
 
 
+class _ApplyLog:
+    """Stands in for the library object of one operator and records the k of
+    every mbpt_sigma_apply call (one exchange launch each) in order."""
+
+    def __init__(self, lib):
+        self._lib, self.ks = lib, []
+
+    def __getattr__(self, name):
+        fn = getattr(self._lib, name)
+        if name != 'mbpt_sigma_apply':
+            return fn
+
+        def apply(h, k, *rest):
+            self.ks.append(int(k))
+            return fn(h, k, *rest)
+        return apply
+
+
 def kernel_probe(args):
-    """Counter-run workload: the headline instance's sigma-vector at k = 20 in
-    the mixed precision, three applies, then a 256 MiB device-to-device copy
-    (the calibration kernel for FETCH_SIZE / WRITE_SIZE)."""
+    """Counter-run workload on the headline instance, then a 256 MiB
+    device-to-device copy (the calibration kernel for FETCH_SIZE /
+    WRITE_SIZE).  'k20': three mixed-precision sigma applies at k = 20.
+    'mix': one learned+mixed and one screened+mixed solve, the sequence of
+    exchange launches the bench's timed steps repeat; the k of each
+    mixed-precision apply is printed in launch order."""
     import torch
-    from mbptcode_amd.block_actions import SigmaOperator
+    from mbptcode_amd.block_actions import SigmaOperator, grid_block_action
     inst, W, _ = build_problem(args.config)
-    op = SigmaOperator(inst.eps, inst.nocc, inst.X, W, inst.V, precision=args.precision)
-    z = torch.randn((20, inst.nocc, inst.nvir), dtype=torch.float64, device='cuda')
-    for _ in range(3):
-        op.apply(z)
+    if args.kernel_probe == 'k20':
+        op = SigmaOperator(inst.eps, inst.nocc, inst.X, W, inst.V, precision=args.precision)
+        z = torch.randn((20, inst.nocc, inst.nvir), dtype=torch.float64, device='cuda')
+        for _ in range(3):
+            op.apply(z)
+    else:
+        from mbptcode_amd.mlp_precond import LearnedPreconditioner
+        act64, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision='fp64')
+        actm, _ = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V,
+                                    precision=args.precision)
+        net = LearnedPreconditioner.load()
+        rec = _ApplyLog(actm.operator._lib)
+        actm.operator._lib = rec
+        for s in (Solver('learned', actm, d, inst, refine=act64, preconditioner='learned',
+                         learned=net, learned_mode=args.learned_mode,
+                         learned_until=args.learned_until),
+                  Solver('screened', actm, d, inst, refine=act64)):
+            s()
+        sync()
+        log(f'mixed-precision applies in launch order, k = {json.dumps(rec.ks)}')
     sync()
     a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device='cuda')
     b = torch.empty_like(a)
     b.copy_(a)
     sync()
-    log(f'kernel probe done: 3 applies (k=20, {args.precision}), one 256 MiB copy')
+    log(f'kernel probe done ({args.kernel_probe}, {args.precision}), one 256 MiB copy')
 
 
 def log(msg):
@@ -272,14 +313,16 @@ def headline(args):
             f'{tims.get("davidson_iterations")} cycles, {tims.get("davidson_vectors")} vectors')
     out['roofline'] = exchange_roofline(prof_all, args.precision, inst.nocc, inst.M)
     tr = recorded_traffic()
-    if tr is not None and args.precision == 'bf16x3' and args.config == 4:
-        # bytes per trial vector at k = 20, times the average k of the launches
-        k_avg = out['roofline']['flops_per_launch'] / (4.0 * inst.nocc * inst.M ** 2 + inst.M ** 2)
-        out['roofline']['traffic'] = tr * k_avg
-        out['roofline']['traffic_note'] = ('HBM bytes per launch: FETCH_SIZE + WRITE_SIZE per trial '
-                                           'vector from the counter runs at k = 20 in ' +
-                                           os.path.relpath(TRAFFIC_FILE, ROOT) +
-                                           f', times the average k per launch ({k_avg:.1f})')
+    default_mix = (args.precision == 'bf16x3' and args.config == 4
+                   and args.learned_mode == 'augment' and args.learned_until == 1000.0)
+    if tr is not None and default_mix:
+        # the counter runs replayed this launch sequence (k = 40, 60, 30, ...)
+        out['roofline']['traffic'] = tr
+        out['roofline']['traffic_note'] = ('HBM bytes per launch: FETCH_SIZE + WRITE_SIZE of '
+                                           'each dispatch of one learned+mixed and one '
+                                           'screened+mixed solve (the timed steps\' launch '
+                                           'sequence), averaged, from ' +
+                                           os.path.relpath(TRAFFIC_FILE, ROOT))
     out['chi0_roofline'] = chi0_roofline(inst)
     log(f'chi0 (headline instance): {out["chi0_roofline"]["achieved"]:.1f} TF '
         f'({100 * out["chi0_roofline"]["frac"]:.0f}% of FP32 MFMA peak)')
@@ -556,7 +599,7 @@ def leg_batched():
     """The batched workload: 256 instances, 5 roots each, tol 1e-5."""
     import torch
     from mbptcode_amd import synth, grids
-    from mbptcode_amd.batched import BatchedSigma, autotune, solve_batched
+    from mbptcode_amd.batched import BatchedSigma, autotune, solve_batched, time_pass
     from mbptcode_amd.space_time import screened_interaction_grid
     sizes = synth.batched_sizes(256)
     insts = [synth.batched_instance(i, sizes) for i in range(256)]
@@ -573,6 +616,20 @@ def leg_batched():
     for op in (op64, op32):
         plan, info = autotune(op)
         tune[op.precision] = {'plan': plan, 'pass_ms': info.get('ms'), 'source': info['source']}
+    # one captured sigma pass at kmax vectors per instance against the MFMA
+    # peak of its precision (algorithmic flops of every stage)
+    no = np.asarray(op32.nocc, float)
+    nv = np.asarray(op32.nvir, float)
+    Mg = np.asarray(op32.M, float)
+    per_vec = 4 * no * Mg ** 2 + 3 * Mg ** 2 + 4 * no * nv * Mg + 2 * no * Mg
+    pass_flops = float(op32.kmax * per_vec.sum())
+    passes = {}
+    for op in (op32, op64):
+        ms = time_pass(op, reps=7)
+        tf = pass_flops / (ms * 1e-3) / 1e12
+        peak = PEAK_TFLOPS['fp32' if op.precision == 'fp32' else 'fp64']
+        passes[op.precision] = {'pass_ms': ms, 'achieved': tf, 'peak': peak,
+                                'unit': 'TFLOP/s', 'frac': tf / peak, 'bound': 'mfma'}
     runs = {}
     e_plain = None
     for name, op, ref in (('plain_fp64', op64, None), ('fp32+fp64_recheck', op32, op64)):
@@ -590,9 +647,16 @@ def leg_batched():
         runs[name] = {'roots_per_s': 256 * 5 / dt, 'ms_per_solve': dt * 1e3,
                       'cycles': info['cycles'], 'refine_cycles': info.get('refine_cycles'),
                       'converged': info['converged'], 'max_err_meV': err}
-    log('batched: ' + ', '.join(f'{k} {v["roots_per_s"]:.0f} roots/s' for k, v in runs.items()))
+    log('batched: ' + ', '.join(f'{k} {v["roots_per_s"]:.0f} roots/s' for k, v in runs.items())
+        + '; sigma pass ' + ', '.join(f'{k} {v["pass_ms"]:.2f} ms ({100 * v["frac"]:.0f}% of '
+                                      f'peak)' for k, v in passes.items()))
     return {'workload': '256 instances, nocc~U[5,40], nvir~U[20,200], M=8(nocc+nvir), '
-                        '5 roots, tol 1e-5', 'solvers': runs, 'plan': tune}
+                        '5 roots, tol 1e-5', 'solvers': runs, 'plan': tune,
+            'sigma_pass': dict(passes, kmax=op32.kmax, flops_per_pass=pass_flops,
+                               flop_definition='kmax vectors per instance, each '
+                                               '4 nocc M^2 + 3 M^2 + 4 nocc nvir M + '
+                                               '2 nocc M (exchange, Hartree, head and '
+                                               'tail products)')}
 
 
 def cpu_baseline(inst, W_grid, e_ref, vectors):

@@ -235,20 +235,37 @@ class BatchedSigma:
         return idx[None, :] < torch.as_tensor(self.npair, device=self.device)[:, None]
 
     def pair_energies(self):
-        """(n, P) d_ia = eps_a - eps_i, padding set to a large value."""
-        d = np.full((self.n, self.P), _PAD_DIAG)
-        for b in range(self.n):
-            e, no = self._eps[b], int(self.nocc[b])
-            d[b, :self.npair[b]] = (e[no:][None, :] - e[:no][:, None]).ravel()
-        return torch.as_tensor(d, device=self.device)
+        """(n, P) d_ia = eps_a - eps_i, padding set to a large value (a fresh
+        tensor; the operator keeps its own copy)."""
+        if getattr(self, '_dpair', None) is None:
+            d = np.full((self.n, self.P), _PAD_DIAG)
+            for b in range(self.n):
+                e, no = self._eps[b], int(self.nocc[b])
+                d[b, :self.npair[b]] = (e[no:][None, :] - e[:no][:, None]).ravel()
+            self._dpair = torch.as_tensor(d, device=self.device)
+        return self._dpair.clone()
 
     def screened_diagonal(self):
-        """(n, P) d - (ii|W|aa) in the handle's precision, padding large."""
-        out = torch.full((self.n, self.P), _PAD_DIAG, dtype=torch.float64, device=self.device)
-        _lib.check(_lib.lib().mbpt_batch_screened_diag(self._h, _lib.dptr(out),
-                                                       _lib.stream_ptr()),
-                   'mbpt_batch_screened_diag')
-        return out
+        """(n, P) d - (ii|W|aa) in the handle's precision, padding large (a
+        fresh tensor).  Computed on the first call and kept: it depends on
+        the operator only, not on the plan."""
+        if getattr(self, '_sdiag', None) is None:
+            out = torch.full((self.n, self.P), _PAD_DIAG, dtype=torch.float64,
+                             device=self.device)
+            _lib.check(_lib.lib().mbpt_batch_screened_diag(self._h, _lib.dptr(out),
+                                                           _lib.stream_ptr()),
+                       'mbpt_batch_screened_diag')
+            self._sdiag = out
+        return self._sdiag.clone()
+
+    def sigma_graph(self, graph=True):
+        """The operator's SigmaGraph (one per `graph` setting), kept across
+        solves so that a pass is captured once per plan, not once per solve.
+        Solves that share it must not overlap."""
+        graphs = self.__dict__.setdefault('_graphs', {})
+        if graph not in graphs:
+            graphs[graph] = SigmaGraph(self, graph=graph)
+        return graphs[graph]
 
     # -- work -----------------------------------------------------------------
     def plan(self, rows_per_split, tail_kchunk, hartree_kchunk, grid_cap=0):
@@ -606,11 +623,16 @@ def _apply_via(sg):
 def batched_guess(diag_host, npair, nroots, P, device):
     """Unit-vector guesses per instance (guess_indices on each instance's
     pair energies): x0 (B, P, c0) and live counts (B,)."""
-    idx = [guess_indices(diag_host[b][:npair[b]], nroots) for b in range(len(npair))]
+    idx = [np.asarray(guess_indices(diag_host[b][:npair[b]], nroots), np.int64)
+           for b in range(len(npair))]
     c0 = max(len(i) for i in idx)
     x0 = torch.zeros((len(npair), P, c0), dtype=torch.float64, device=device)
-    for b, i in enumerate(idx):
-        x0[b, torch.as_tensor(i, device=device), torch.arange(len(i), device=device)] = 1.0
+    # one scatter for the whole batch: (instance, pair, column) of every unit entry
+    where = np.stack([np.concatenate([np.full(len(i), b) for b, i in enumerate(idx)]),
+                      np.concatenate(idx),
+                      np.concatenate([np.arange(len(i)) for i in idx])])
+    b_, p_, c_ = torch.as_tensor(where, device=device)
+    x0[b_, p_, c_] = 1.0
     return x0, torch.as_tensor([len(i) for i in idx], dtype=torch.int32, device=device)
 
 
@@ -631,8 +653,7 @@ def solve_batched(op, nroots, tol, max_cycle=100, preconditioner='screened', ref
     if pdiag is None:
         pdiag = dpair if preconditioner == 'bare' else op.screened_diagonal()
     x0, live0 = batched_guess(d_host, op.npair, nroots, op.P, dev)
-    sg = SigmaGraph(op, graph=graph)
-    apply = _apply_via(sg)
+    apply = _apply_via(op.sigma_graph(graph))
     precond = None
     if preconditioner == 'learned':
         if learned is None:
@@ -654,8 +675,7 @@ def solve_batched(op, nroots, tol, max_cycle=100, preconditioner='screened', ref
             'sync_s': res.sync_s, 'track': track}
     e, X, rn = res.e, res.X, res.rnorm
     if refine_op is not None:
-        sg64 = SigmaGraph(refine_op, graph=graph)
-        apply64 = _apply_via(sg64)
+        apply64 = _apply_via(refine_op.sigma_graph(graph))
         live = torch.full((op.n,), track, dtype=torch.int32, device=dev)
         AX = apply64(X, live)
         theta = (X * AX).sum(1)

@@ -19,6 +19,7 @@ screened FP64 solve.
 """
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -78,6 +79,49 @@ def evaluate(net_params, instances, precision='fp64', residual_of_jd=True, learn
     return out
 
 
+def evaluate_single(net_params, inst, precisions=('fp64', 'bf16x3'), residual_of_jd=True,
+                    learned_until=1000.0, repeats=2):
+    """The learned and screened corrections on one single instance (W from
+    FP32 chi0 terms), per sigma precision: cycles, sigma-vectors, the best of
+    `repeats` wall times (after one untimed solve) and the worst root error
+    (meV) against the screened FP64 solve.  Mixed precisions recheck their
+    roots in FP64, as the headline solver does."""
+    g = static_screening_grid(inst.eps, inst.nocc, ntau=inst.ntau)
+    W, _ = screened_interaction_grid(inst.X, inst.V, inst.eps, inst.nocc, g,
+                                     precision='fp32')
+    acts = {}
+    for prec in set(precisions) | {'fp64'}:
+        act, d = grid_block_action(inst.eps, inst.nocc, inst.X, W, inst.V, precision=prec)
+        act.operator.screened_diagonal_device()
+        acts[prec] = act
+    net = LearnedPreconditioner(params=net_params, d=d)
+    net.residual_of_jd = residual_of_jd
+    out, ref = {}, None
+    for prec in ('fp64',) + tuple(p for p in precisions if p != 'fp64'):
+        for pre in ('screened', 'learned'):
+            refine = None if prec == 'fp64' else acts['fp64']
+            best = None
+            for r in range(repeats + 1):
+                tims = {}
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                w, _, _ = run_davidson(acts[prec], d, inst.nroots, inst.tol, 200, tda=True,
+                                       preconditioner=pre, learned=net, timings=tims,
+                                       learned_until=learned_until, refine_apply=refine)
+                torch.cuda.synchronize()
+                dt = time.perf_counter() - t0
+                if r > 0:
+                    best = dt if best is None else min(best, dt)
+            if ref is None:
+                ref = w
+            out[f'{pre}_{prec}'] = {
+                'cycles': int(tims['davidson_iterations']),
+                'vectors': int(tims['davidson_vectors']), 'seconds': best,
+                'max_err_meV': float(np.abs(w - ref).max() * HA_MEV),
+                'fallbacks': int('learned_fallback_cycle' in tims['davidson_flags'])}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--instances', type=int, default=512)
@@ -85,8 +129,29 @@ def main(argv=None):
     ap.add_argument('--eval', type=int, default=16)
     ap.add_argument('--out', default=DEFAULT_WEIGHTS)
     ap.add_argument('--target', default='jd', choices=['jd', 'span'])
+    ap.add_argument('--eval-only', action='store_true',
+                    help='evaluate the weights at --out (held-out batched instances '
+                         'and the 100/700/6000 instance) and rewrite the record\'s '
+                         'evaluation')
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
+    if args.eval_only:
+        net = LearnedPreconditioner.load(args.out)
+        sizes = synth.batched_sizes()
+        evals = [synth.batched_instance(i, sizes) for i in range(args.eval)]
+        res = evaluate(net.params(), evals, residual_of_jd=net.residual_of_jd)
+        print(f'[eval] held-out: {json.dumps(res)}', flush=True)
+        single = evaluate_single(net.params(), synth.config(4),
+                                 residual_of_jd=net.residual_of_jd)
+        print(f'[eval] 100/700/6000: {json.dumps(single)}', flush=True)
+        rec_path = os.path.splitext(args.out)[0] + '_train.json'
+        with open(rec_path) as f:
+            rec = json.load(f)
+        rec.update(eval_instances=args.eval, eval=res, eval_single={'instance': '100/700/6000',
+                                                                    'results': single})
+        with open(rec_path, 'w') as f:
+            json.dump(rec, f, indent=1)
+        return 0
     t0 = time.time()
     insts = training_instances(args.instances)
     print(f'[train] {len(insts)} instances generated in {time.time() - t0:.1f} s', flush=True)

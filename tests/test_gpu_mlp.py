@@ -118,3 +118,46 @@ def test_davidson_with_learned_correction_and_fallback():
     assert np.abs(w - gold['roots']).max() * 27211.386 < 1e-3
     assert net.calls >= 1
     assert 'learned_fallback_cycle' in tims['davidson_flags']
+
+
+def test_trained_gain():
+    """What the shipped network gains over the method without it -- the
+    Jacobi-Davidson correction on the screened diagonal, which it augments --
+    on 32 held-out instances of the batched workload (seed family 1; the
+    network was trained on family 2) and on the 100/700/6000 instance, FP64
+    sigma (and split-BF16 sigma with the FP64 recheck at 100/700/6000).
+
+    Asserted: every learned solve reaches the screened solve's roots
+    (<= 1e-3 meV); on the held-out set it takes fewer Davidson cycles than
+    the screened correction and than the bare diagonal.  Measured and
+    printed, not a gain: it applies sigma to more vectors than the screened
+    correction (its augment cycle adds a learned vector beside every
+    Jacobi-Davidson one), and at 100/700/6000 it takes one cycle more; the
+    counts are checked against the record shipped with the weights, so a
+    change of either shows."""
+    import json
+    import os
+    from mbptcode_amd import synth
+    from mbptcode_amd.mlp_precond import DEFAULT_WEIGHTS, LearnedPreconditioner
+    from mbptcode_amd.train_precond import evaluate, evaluate_single
+    net = LearnedPreconditioner.load()
+    sizes = synth.batched_sizes()
+    held = [synth.batched_instance(i, sizes) for i in range(32)]
+    res = evaluate(net.params(), held, residual_of_jd=net.residual_of_jd)
+    single = evaluate_single(net.params(), synth.config(4), repeats=1,
+                             residual_of_jd=net.residual_of_jd)
+    print('[trained gain] held-out ' + json.dumps(res), flush=True)
+    print('[trained gain] 100/700/6000 ' + json.dumps(single), flush=True)
+    for name, r in list(res.items()) + list(single.items()):
+        assert r['max_err_meV'] < 1e-3, name
+    L, S, B = res['learned'], res['screened'], res['bare']
+    assert L['cycles'] < S['cycles'] and L['cycles'] < B['cycles']
+    with open(os.path.splitext(DEFAULT_WEIGHTS)[0] + '_train.json') as f:
+        rec = json.load(f)
+    for mode in ('bare', 'screened', 'learned'):
+        for key in ('cycles', 'vectors'):
+            assert abs(res[mode][key] - rec['eval'][mode][key]) <= 0.03 * rec['eval'][mode][key], \
+                (mode, key, res[mode][key], rec['eval'][mode][key])
+    for mode, r in single.items():
+        ref = rec['eval_single']['results'][mode]
+        assert (r['cycles'], r['vectors']) == (ref['cycles'], ref['vectors']), (mode, r, ref)
