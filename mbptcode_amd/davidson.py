@@ -876,7 +876,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
             if (state['fallback'] or learned_until is not None and
                     float(R.norm(dim=0).max()) < learned_until * conv_tol):
                 return jd
-            t = learned.correction(R, U, theta, pdiag)
+            t = learned.correction(R, U, theta, pdiag, jd=jd)
             # 'augment': both directions enter the space (the diagonal one is
             # never lost); 'replace': the learned one alone
             return torch.cat([jd, t], dim=1) if learned_mode == 'augment' else t

@@ -164,12 +164,14 @@ class LearnedPreconditioner:
         return hist
 
     # -- the Davidson hook -----------------------------------------------------
-    def correction(self, R, U, theta, diag=None):
+    def correction(self, R, U, theta, diag=None, jd=None):
         """Corrections T (n, k) for residual columns R (n, k) at Ritz values
         theta (k): rms(r) * net(r / rms(r), d - theta, theta).  d is the pair
         energy on the Davidson's preconditioner diagonal (`diag`, the screened
         d - (ii|W|aa) by default, the features the network was trained on),
-        else the pair energies this object was built with."""
+        else the pair energies this object was built with.  A network trained
+        as a remainder over Jacobi-Davidson gets that correction added (`jd`
+        when the caller has it, else computed here)."""
         dvec = diag if diag is not None else self.d
         if dvec is None:
             raise ValueError('LearnedPreconditioner needs the pair energies d')
@@ -184,8 +186,10 @@ class LearnedPreconditioner:
                    'mbpt_mlp_correction')
         self.calls += 1
         if self.residual_of_jd and U is not None:
-            from .davidson import jacobi_davidson_correction
-            T += jacobi_davidson_correction(R, U, dvec, th)
+            if jd is None:
+                from .davidson import jacobi_davidson_correction
+                jd = jacobi_davidson_correction(R, U, dvec, th)
+            T += jd
         return T
 
     def correction_batched(self, R, U, theta, diag, npair, jd=None):

@@ -1,7 +1,8 @@
 """Train the learned Davidson preconditioner and compare it with the diagonal ones.
 
     python -m mbptcode_amd.train_precond [--instances 512] [--steps 3000]
-                                         [--eval 16] [--out PATH]
+                                         [--eval 32] [--target jd|span]
+                                         [--out PATH] [--eval-only]
 
 Training set: plain FP64 Davidson runs (screened Jacobi-Davidson correction,
 tol 1e-5) on `--instances` random instances sized like the batched workload
@@ -13,9 +14,13 @@ the features of 256 random pairs and their ideal correction (see
 65536-row minibatches) and saved with its training record.
 
 Evaluation: the first `--eval` instances of the batched workload, each solved
-with the 'bare', 'screened' and 'learned' corrections at FP64, the cycles,
-sigma-vectors and wall time of each reported, roots checked against the
-screened FP64 solve.
+with the 'bare', 'screened' and 'learned' corrections at FP64, and the
+100/700/6000 instance with the screened and learned corrections in FP64 and
+split-BF16 sigma; the cycles, sigma-vectors and wall time of each are
+written with the training figures to the record beside the weights
+(`<out>_train.json`), roots checked against the screened FP64 solve.
+`--eval-only` re-evaluates the weights at `--out` and rewrites the record's
+evaluation.
 """
 import argparse
 import json
@@ -126,7 +131,7 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument('--instances', type=int, default=512)
     ap.add_argument('--steps', type=int, default=3000)
-    ap.add_argument('--eval', type=int, default=16)
+    ap.add_argument('--eval', type=int, default=32)
     ap.add_argument('--out', default=DEFAULT_WEIGHTS)
     ap.add_argument('--target', default='jd', choices=['jd', 'span'])
     ap.add_argument('--eval-only', action='store_true',
@@ -135,35 +140,32 @@ def main(argv=None):
                          'evaluation')
     args = ap.parse_args(argv)
     torch.cuda.set_device(0)
+    rec_path = os.path.splitext(args.out)[0] + '_train.json'
     if args.eval_only:
         net = LearnedPreconditioner.load(args.out)
-        sizes = synth.batched_sizes()
-        evals = [synth.batched_instance(i, sizes) for i in range(args.eval)]
-        res = evaluate(net.params(), evals, residual_of_jd=net.residual_of_jd)
-        print(f'[eval] held-out: {json.dumps(res)}', flush=True)
-        single = evaluate_single(net.params(), synth.config(4),
-                                 residual_of_jd=net.residual_of_jd)
-        print(f'[eval] 100/700/6000: {json.dumps(single)}', flush=True)
-        rec_path = os.path.splitext(args.out)[0] + '_train.json'
         with open(rec_path) as f:
             rec = json.load(f)
-        rec.update(eval_instances=args.eval, eval=res, eval_single={'instance': '100/700/6000',
-                                                                    'results': single})
-        with open(rec_path, 'w') as f:
-            json.dump(rec, f, indent=1)
-        return 0
-    t0 = time.time()
-    insts = training_instances(args.instances)
-    print(f'[train] {len(insts)} instances generated in {time.time() - t0:.1f} s', flush=True)
-    net, info = train_on_instances(insts, steps=args.steps, log_every=500,
-                                   target=args.target)
-    print(f'[train] {json.dumps(info)}', flush=True)
-    net.save(args.out, rows=info['rows'], steps=args.steps, instances=args.instances,
-             loss_first=info['loss_first'], loss_last=info['loss_last'])
+    else:
+        t0 = time.time()
+        insts = training_instances(args.instances)
+        print(f'[train] {len(insts)} instances generated in {time.time() - t0:.1f} s',
+              flush=True)
+        net, info = train_on_instances(insts, steps=args.steps, log_every=500,
+                                       target=args.target)
+        print(f'[train] {json.dumps(info)}', flush=True)
+        net.save(args.out, rows=info['rows'], steps=args.steps, instances=args.instances,
+                 loss_first=info['loss_first'], loss_last=info['loss_last'])
+        rec = {'train': info}
     sizes = synth.batched_sizes()
     evals = [synth.batched_instance(i, sizes) for i in range(args.eval)]
     res = evaluate(net.params(), evals, residual_of_jd=net.residual_of_jd)
-    print(json.dumps({'train': info, 'eval_instances': args.eval, 'eval': res}), flush=True)
+    print(f'[eval] held-out: {json.dumps(res)}', flush=True)
+    single = evaluate_single(net.params(), synth.config(4), residual_of_jd=net.residual_of_jd)
+    print(f'[eval] 100/700/6000: {json.dumps(single)}', flush=True)
+    rec.update(eval_instances=args.eval, eval=res,
+               eval_single={'instance': '100/700/6000', 'results': single})
+    with open(rec_path, 'w') as f:
+        json.dump(rec, f, indent=1)
     return 0
 
 

@@ -132,7 +132,7 @@ def test_trained_gain():
     the screened correction and than the bare diagonal.  Measured and
     printed, not a gain: it applies sigma to more vectors than the screened
     correction (its augment cycle adds a learned vector beside every
-    Jacobi-Davidson one), and at 100/700/6000 it takes one cycle more; the
+    Jacobi-Davidson one), and at 100/700/6000 it takes as many cycles; the
     counts are checked against the record shipped with the weights, so a
     change of either shows."""
     import json
