@@ -336,7 +336,7 @@ def _jd(U, R, diag, theta):
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
              lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None,
-             keep=None, pick=None):
+             keep=None, pick=None, early=None, early_roots=0):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -368,6 +368,10 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     converge instead of the lowest (root following, `overlap_pick`): envs
     holds 'ref_coeff', V^T span (m, r), when the pick carries a `span`
     (n, r), else 'xs', the trial vectors as host arrays.
+    `early(X, w)` is called once, at the first cycle whose Ritz step finds
+    the leading `early_roots` pairs converged while a later tracked pair is
+    not, with those Ritz vectors (n, early_roots) on the device and their
+    values: a caller can start checking them while the iteration goes on.
     """
     last = [time.perf_counter()]
 
@@ -453,6 +457,12 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
                       'converged': int(conv.sum())})
         if conv.all():
             break
+        if early is not None and 0 < early_roots <= k and conv[:early_roots].all():
+            ye = torch.as_tensor(np.ascontiguousarray(y[:, :early_roots]), device=dev)
+            Xe = torch.empty((n, early_roots), dtype=torch.float64, device=dev)
+            _nn(Ranges(VA, (0, m)), ye, Xe, 0)
+            early(Xe, np.array(w[:early_roots], dtype=float))
+            early = None
         worst = float(rn[~conv].max())
         hist.append(worst)
         if stall_cycles and on_stall is not None and len(hist) > stall_cycles:
@@ -885,10 +895,36 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
             state['fallback'] = True
             counters['learned_fallback_cycle'] = cycle
             return None
+    early, checked = None, {}
+    if refine_apply is not None:
+        op64 = _operator_of(refine_apply)
+        mm64 = matmat_of(op64)
+        side = torch.cuda.Stream(device=dev) if dev.type == 'cuda' else None
+
+        def fp64_check(Xr):
+            """(theta, |A x - theta x|) of the columns of Xr with FP64 sigma."""
+            AX = mm64(Xr)
+            theta = (Xr * AX).sum(0)
+            return theta, (AX - Xr * theta[None, :]).norm(dim=0)
+
+        if side is not None:
+            def early(Xr, w):
+                # the requested roots have converged while tracked extras
+                # have not: check them in FP64 on a side stream while the
+                # iteration finishes the extras on this one
+                main = torch.cuda.current_stream(dev)
+                side.wait_stream(main)
+                with torch.cuda.stream(side):
+                    theta, rn_e = fp64_check(Xr)
+                Xr.record_stream(side)
+                for t in (theta, rn_e):
+                    t.record_stream(main)
+                checked.update(X=Xr, theta=theta, rn=rn_e)
     res = davidson(matmat, pdiag, track, x0, tol=conv_tol, max_cycle=max_cycle,
                    max_space=max_space, precond=precond,
                    stall_cycles=stall_cycles if preconditioner == 'learned' else None,
-                   on_stall=on_stall, phases=phases, keep=keep)
+                   on_stall=on_stall, phases=phases, keep=keep, early=early,
+                   early_roots=nroots if track > nroots else 0)
     cycles, vectors = res.cycles, res.vectors
     e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
     exhausted = res.exhausted
@@ -896,13 +932,21 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     if refine_apply is not None:
         # the FP64 check covers the requested roots; the extra tracked ones
         # only bound the unconfirmed-root test and keep their residuals
-        op64 = _operator_of(refine_apply)
-        mm64 = matmat_of(op64)
         Xr = X[:, :nroots].contiguous()
-        AX = mm64(Xr)
-        theta = (Xr * AX).sum(0)
-        R = AX - Xr * theta[None, :]
-        rn64 = R.norm(dim=0).cpu().numpy()
+        if checked:
+            torch.cuda.current_stream(dev).wait_stream(side)
+            rn64 = checked['rn'].cpu().numpy()
+            refine['early'] = bool((rn64 <= conv_tol).all())
+            if refine['early']:
+                # the roots as they were when they converged, checked in
+                # FP64 while the extras finished
+                Xr, theta = checked['X'], checked['theta']
+                X = torch.cat([Xr, X[:, nroots:]], dim=1)
+            else:
+                checked.clear()
+        if not checked:
+            theta, rn_t = fp64_check(Xr)
+            rn64 = rn_t.cpu().numpy()
         refine['rnorm_before'] = rn64.copy()
         if not (rn64 <= conv_tol).all():
             res2 = davidson(mm64, pdiag, track, X, tol=conv_tol, max_cycle=max_cycle,
