@@ -40,7 +40,8 @@ Simplifications against the reference:
 * Tracked extra pairs must converge; the reference also settles them by their
   residual interval clearing the highest requested root.
 * There are no pair-symmetry sectors.
-* There is no dense completion for a pair space the trial space exhausts.
+* A pair space the trial space exhausts is finished densely by the caller
+  (`davidson._casida_root` through `davidson.dense_casida`).
 * A projected a - b that is not positive definite raises LinAlgError here;
   `davidson.run_davidson` turns it into CasidaBreakdown or
   UnstableReference by the (A - B) probe (`probe.py`).
@@ -140,7 +141,16 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     (default 0; a warm start passes converged (X, Y) pairs).  Every
     returned root satisfies |[R_x; R_y]| <= tol where `conv` says so.
     `phases`, a dict, receives the seconds of each part of the cycle (the
-    device is synchronised at every boundary, so only for profiling)."""
+    device is synchronised at every boundary, so only for profiling).
+
+    The holders V, W, U1, U2 are four column blocks of one row-major buffer
+    (n, 4 cap), so [V | W] or [U1 | U2] is one operand of the two-range
+    reductions and products (dvd.hip).  A cycle's corrections are written
+    straight into V's and W's next columns and orthonormalised there: one
+    reduction and two products project them off the held pairs, one more
+    reduction gives the second projection and their Gram matrices together,
+    and one host transfer per cycle carries that (plus one for the new rows
+    of the projected blocks)."""
     last = [time.perf_counter()]
 
     def mark(name):
@@ -150,6 +160,8 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
             now = time.perf_counter()
             phases[name] = phases.get(name, 0.0) + now - last[0]
             last[0] = now
+    from .davidson import _nn, _tnr
+    from .subspace import Ranges
     dev = diag.device
     n = diag.numel()
     nroots = max(1, min(int(nroots), n))
@@ -160,77 +172,119 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         lindep = float(max(DAVIDSON_LINDEP_FLOOR, min(1e-14, 1e-2 * tol * tol)))
     floor = DAVIDSON_DENOMINATOR_FLOOR
     t0 = time.time()
-    cap = max_space + x0.shape[1] + nroots
-    Vh = torch.zeros((n, cap), dtype=torch.float64, device=dev)
-    Wh, U1h, U2h = torch.zeros_like(Vh), torch.zeros_like(Vh), torch.zeros_like(Vh)
+    cap = max_space + max(x0.shape[1], nroots)
+    oV, oW, oU1, oU2 = 0, cap, 2 * cap, 3 * cap
+    H4 = torch.zeros((n, 4 * cap), dtype=torch.float64, device=dev)   # [V | W | U1 | U2]
     a = np.zeros((cap, cap))
     b, sg, pi = np.zeros_like(a), np.zeros_like(a), np.zeros_like(a)
     m = 0
     vectors = 0
     exhausted = False
 
-    def act(V, W):
+    def pair(o1, o2, c0, c):
+        """The columns c0 .. c0 + c of the blocks at o1 and o2, one operand."""
+        return Ranges(H4, (o1 + c0, c), (o2 + c0, c))
+
+    def act(c0, q, cold):
+        """U1 = A V + B W, U2 = A W + B V of the new columns into their slots
+        (two block actions per batch, LR/davidson.py:1071-1088)."""
         nonlocal vectors
-        p = V.shape[1]
-        if not bool(torch.any(W)):
+        V, W = H4[:, oV + c0:oV + c0 + q], H4[:, oW + c0:oW + c0 + q]
+        if cold:
             # a cold start's pairs have no Y half: U1 = A V, U2 = B V from
             # one apply of V alone
             AZ, BZ = apply_ab(V.T.contiguous())
-            vectors += p
-            return AZ.T, BZ.T
-        AB = apply_ab(torch.cat([V, W], dim=1).T.contiguous())
-        AZ, BZ = AB[0].T, AB[1].T
-        vectors += 2 * p
-        return AZ[:, :p] + BZ[:, p:], AZ[:, p:] + BZ[:, :p]   # U1, U2
+            vectors += q
+            H4[:, oU1 + c0:oU1 + c0 + q] = AZ.T
+            H4[:, oU2 + c0:oU2 + c0 + q] = BZ.T
+            return
+        AZ, BZ = apply_ab(torch.cat([V, W], dim=1).T.contiguous())
+        vectors += 2 * q
+        H4[:, oU1 + c0:oU1 + c0 + q] = (AZ[:q] + BZ[q:]).T
+        H4[:, oU2 + c0:oU2 + c0 + q] = (AZ[q:] + BZ[:q]).T
 
-    def append(V, W, U1, U2):
-        """Hold the new columns and fill their rows of a, b, sigma, pi."""
+    def append(m0, q):
+        """Fill the rows of a, b, sigma and pi of the q columns held at m0."""
         nonlocal m
-        p = V.shape[1]
-        m0, m1 = m, m + p
-        Vh[:, m0:m1], Wh[:, m0:m1], U1h[:, m0:m1], U2h[:, m0:m1] = V, W, U1, U2
-        new = torch.cat([Vh[:, m0:m1], Wh[:, m0:m1], U1h[:, m0:m1], U2h[:, m0:m1]], dim=1)
-        old = torch.cat([Vh[:, :m1], Wh[:, :m1], U1h[:, :m1], U2h[:, :m1]], dim=1)
-        G = _tn(new, old).cpu().numpy()                          # (4p, 4 m1)
-        blk = lambda i, j: G[i * p:(i + 1) * p, j * m1:(j + 1) * m1]
-        nV, nW, nU1, nU2 = 0, 1, 2, 3
-        # new rows of V^T U1 and of U1^T V (the new columns of U1^T V are the
-        # transposes of the U1_new^T V rows taken against the old V rows)
-        a_rows = 0.5 * (blk(nV, nU1) + blk(nU1, nV)) + 0.5 * (blk(nW, nU2) + blk(nU2, nW))
-        b_rows = 0.5 * (blk(nV, nU2) + blk(nU2, nV)) + 0.5 * (blk(nW, nU1) + blk(nU1, nW))
-        s_rows = blk(nV, nV) - blk(nW, nW)
-        p_rows = blk(nV, nW) - blk(nW, nV)
-        for M_, rows_, sign in ((a, a_rows, 1), (b, b_rows, 1), (sg, s_rows, 1), (pi, p_rows, -1)):
+        m1 = m0 + q
+        stage = torch.empty((3, 2 * m1, 2 * q), dtype=torch.float64, device=dev)
+        _tnr(pair(oV, oW, 0, m1), pair(oV, oW, m0, q), out=stage[0])     # [V;W]^T [Vn|Wn]
+        _tnr(pair(oV, oW, 0, m1), pair(oU1, oU2, m0, q), out=stage[1])   # [V;W]^T [U1n|U2n]
+        _tnr(pair(oU1, oU2, 0, m1), pair(oV, oW, m0, q), out=stage[2])   # [U1;U2]^T [Vn|Wn]
+        G = stage.cpu().numpy()
+
+        def blk(s_, r, c):
+            """(new, old) block: the transpose of stage s_ rows r, columns c."""
+            return G[s_, r * m1:(r + 1) * m1, c * q:(c + 1) * q].T
+        # rows of new^T old: V_n^T V = blk(0,0,0), W_n^T W = blk(0,1,1),
+        # V_n^T W = blk(0,1,0), W_n^T V = blk(0,0,1); U1_n^T V = blk(1,0,0),
+        # U2_n^T V = blk(1,0,1), U1_n^T W = blk(1,1,0), U2_n^T W = blk(1,1,1);
+        # V_n^T U1 = blk(2,0,0), W_n^T U1 = blk(2,0,1), V_n^T U2 = blk(2,1,0),
+        # W_n^T U2 = blk(2,1,1)
+        a_rows = 0.5 * (blk(2, 0, 0) + blk(1, 0, 0)) + 0.5 * (blk(2, 1, 1) + blk(1, 1, 1))
+        b_rows = 0.5 * (blk(2, 1, 0) + blk(1, 0, 1)) + 0.5 * (blk(2, 0, 1) + blk(1, 1, 0))
+        s_rows = blk(0, 0, 0) - blk(0, 1, 1)
+        p_rows = blk(0, 1, 0) - blk(0, 0, 1)
+        for M_, rows_, sign in ((a, a_rows, 1), (b, b_rows, 1), (sg, s_rows, 1),
+                                (pi, p_rows, -1)):
             M_[m0:m1, :m1] = rows_
             M_[:m0, m0:m1] = sign * rows_[:, :m0].T
         m = m1
 
-    def orthonormal_new(tx, ty):
-        """The corrections projected twice off the held pairs and their
-        partners, then orthonormalised with their own partners."""
-        for _ in range(2):
-            if m:
-                ox = _tn(Vh[:, :m], tx) + _tn(Wh[:, :m], ty)
-                oy = _tn(Vh[:, :m], ty) + _tn(Wh[:, :m], tx)
-                tx = tx - Vh[:, :m] @ ox - Wh[:, :m] @ oy
-                ty = ty - Wh[:, :m] @ ox - Vh[:, :m] @ oy
-        s11 = (_tn(tx, tx) + _tn(ty, ty)).cpu().numpy()
-        s21 = (_tn(tx, ty) + _tn(ty, tx)).cpu().numpy()
+    def orthonormal_new(p):
+        """The p corrections held in V's and W's columns m .. m + p,
+        projected twice off the held pairs and their partners and
+        orthonormalised with their own partners, in place; returns how many
+        directions remain (in columns m .. m + q)."""
+        hx = lambda M_: torch.as_tensor(np.ascontiguousarray(M_), device=dev)
+        if m:
+            O = _tnr(pair(oV, oW, 0, m), pair(oV, oW, m, p))           # (2m, 2p)
+            ox = O[:m, :p] + O[m:, p:]                                 # V^T tx + W^T ty
+            oy = O[:m, p:] + O[m:, :p]                                 # V^T ty + W^T tx
+            _nn(pair(oV, oW, 0, m), torch.cat([ox, oy]).contiguous(), H4, oV + m, -1.0, 1.0)
+            _nn(pair(oV, oW, 0, m), torch.cat([oy, ox]).contiguous(), H4, oW + m, -1.0, 1.0)
+        # second projection and the Gram matrices in one reduction:
+        # [V tx | W ty]^T [tx | ty]
+        G = _tnr(Ranges(H4, (oV, m + p), (oW, m + p)), pair(oV, oW, m, p)).cpu().numpy()
+        r = m + p
+        Vt, Tx, Wt, Ty = G[:m], G[m:r], G[r:r + m], G[r + m:]
+        ox2 = Vt[:, :p] + Wt[:, p:]
+        oy2 = Vt[:, p:] + Wt[:, :p]
+        s11 = Tx[:, :p] + Ty[:, p:] - ox2.T @ ox2 - oy2.T @ oy2
+        s21 = Tx[:, p:] + Ty[:, :p] - ox2.T @ oy2 - oy2.T @ ox2
         with _host_lapack():
             coef = symmetric_coefficients(s11, s21, lindep)
         if coef is None:
-            return tx[:, :0], ty[:, :0]
-        ca = torch.as_tensor(coef[0], device=dev)
-        cb = torch.as_tensor(coef[1], device=dev)
-        return tx @ ca + ty @ cb, ty @ ca + tx @ cb
+            return 0
+        ca, cb = coef
+        q = ca.shape[1]
+        # t_x' = (t_x - V ox2 - W oy2) ca + (t_y - W ox2 - V oy2) cb, and its
+        # partner, each one product over [V t_x | W t_y]
+        Yx = np.concatenate([-(ox2 @ ca + oy2 @ cb), ca, -(oy2 @ ca + ox2 @ cb), cb])
+        Yy = np.concatenate([-(oy2 @ ca + ox2 @ cb), cb, -(ox2 @ ca + oy2 @ cb), ca])
+        out = torch.empty((n, 2 * q), dtype=torch.float64, device=dev)
+        A_ = Ranges(H4, (oV, m + p), (oW, m + p))
+        _nn(A_, hx(Yx), out, 0)
+        _nn(A_, hx(Yy), out, q)
+        H4[:, oV + m:oV + m + q] = out[:, :q]
+        H4[:, oW + m:oW + m + q] = out[:, q:]
+        return q
 
     x0 = x0.to(torch.float64)
-    y0 = torch.zeros_like(x0) if y0 is None else y0.to(device=x0.device, dtype=torch.float64)
-    V0, W0 = orthonormal_new(x0, y0)
-    append(V0, W0, *act(V0, W0))
+    p0 = x0.shape[1]
+    H4[:, oV:oV + p0] = x0
+    # a cold start's pairs have no Y half (y0 absent or zero): their W
+    # columns stay zero through the orthonormalisation, so one apply serves
+    cold = y0 is None or not bool(torch.any(y0))
+    if not cold:
+        H4[:, oW:oW + p0] = y0.to(device=x0.device, dtype=torch.float64)
+    q = orthonormal_new(p0)
+    act(0, q, cold)
+    append(0, q)
     trace = []
     conv = np.zeros(nroots, bool)
     w = x = y = rn = None
+    X = Y = None
     cycles = 0
     mark('setup')
     for cycle in range(max_cycle):
@@ -246,48 +300,63 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
             exc.space_exhausted = m >= n or exhausted
             raise
         mark('subspace')
-        xt = torch.as_tensor(x, device=dev)
-        yt = torch.as_tensor(y, device=dev)
+        kk = len(w)
+        xy = torch.as_tensor(np.ascontiguousarray(np.concatenate([x, y])), device=dev)
+        yx = torch.as_tensor(np.ascontiguousarray(np.concatenate([y, x])), device=dev)
         wt = torch.as_tensor(w, device=dev)
-        X = Vh[:, :m] @ xt + Wh[:, :m] @ yt
-        Y = Wh[:, :m] @ xt + Vh[:, :m] @ yt
-        Rx = U1h[:, :m] @ xt + U2h[:, :m] @ yt - X * wt
-        Ry = U2h[:, :m] @ xt + U1h[:, :m] @ yt + Y * wt
+        # X = V x + W y, Y = W x + V y; R_x = U1 x + U2 y - w X,
+        # R_y = U2 x + U1 y + w Y (`ritz_rows` :237-253)
+        XY = torch.empty((n, 4 * kk), dtype=torch.float64, device=dev)
+        _nn(pair(oV, oW, 0, m), xy, XY, 0)
+        _nn(pair(oV, oW, 0, m), yx, XY, kk)
+        _nn(pair(oU1, oU2, 0, m), xy, XY, 2 * kk)
+        _nn(pair(oU1, oU2, 0, m), yx, XY, 3 * kk)
+        X, Y = XY[:, :kk], XY[:, kk:2 * kk]
+        Rx = XY[:, 2 * kk:3 * kk] - X * wt
+        Ry = XY[:, 3 * kk:] + Y * wt
         rn = torch.sqrt((Rx * Rx).sum(0) + (Ry * Ry).sum(0)).cpu().numpy()
         conv = rn <= tol
         trace.append({'cycle': cycle, 'space': int(m), 'max_residual': float(rn.max()),
                       'converged': int(conv.sum())})
         mark('residuals')
-        if conv.all() or len(w) < k:
+        if conv.all() or kk < k:
             break
         idx = np.flatnonzero(~conv)
-        wi = wt[idx]
+        p = len(idx)
+        it = torch.as_tensor(idx, device=dev)
+        wi = wt[it]
         den_x = diag[:, None] - wi[None, :]
         den_y = diag[:, None] + wi[None, :]
         den_x = torch.where(den_x.abs() < floor, torch.full_like(den_x, floor), den_x)
         den_y = torch.where(den_y.abs() < floor, torch.full_like(den_y, floor), den_y)
-        tx, ty = Rx[:, idx] / den_x, Ry[:, idx] / den_y
+        tx, ty = Rx[:, it] / den_x, Ry[:, it] / den_y
         nrm = torch.sqrt((tx * tx).sum(0) + (ty * ty).sum(0)).clamp_min(1e-300)
+        H4[:, oV + m:oV + m + p] = tx / nrm
+        H4[:, oW + m:oW + m + p] = ty / nrm
         mark('correction')
-        tx, ty = orthonormal_new(tx / nrm, ty / nrm)
+        q = orthonormal_new(p)
         mark('orthonormal')
-        if tx.shape[1] == 0:
+        if q == 0:
             exhausted = True
             break
-        if m + tx.shape[1] > max_space:
+        if m + q > max_space:
             # restart on the lowest Ritz pairs, their products combined from
             # the held ones (the new directions stay orthogonal to them)
-            kk = len(w)
-            xk, yk = xt, yt
-            nV, nW = X[:, :kk].clone(), Y[:, :kk].clone()
-            nU1 = U1h[:, :m] @ xk + U2h[:, :m] @ yk
-            nU2 = U2h[:, :m] @ xk + U1h[:, :m] @ yk
-            m = 0
-            append(nV, nW, nU1, nU2)
+            keep = H4[:, [oV + m + j for j in range(q)] + [oW + m + j for j in range(q)]]
+            U12 = torch.empty((n, 2 * kk), dtype=torch.float64, device=dev)
+            _nn(pair(oU1, oU2, 0, m), xy, U12, 0)
+            _nn(pair(oU1, oU2, 0, m), yx, U12, kk)
+            H4[:, oV:oV + kk] = X
+            H4[:, oW:oW + kk] = Y
+            H4[:, oU1:oU1 + kk] = U12[:, :kk]
+            H4[:, oU2:oU2 + kk] = U12[:, kk:]
+            append(0, kk)
+            H4[:, oV + m:oV + m + q] = keep[:, :q]
+            H4[:, oW + m:oW + m + q] = keep[:, q:]
         mark('collapse')
-        prods = act(tx, ty)
+        act(m, q, False)
         mark('sigma')
-        append(tx, ty, *prods)
+        append(m, q)
         mark('append')
     if len(w) < nroots:
         warnings.warn(f'Casida Davidson: only {len(w)} roots in a space of {m}',
