@@ -320,14 +320,17 @@ int mbpt_batch_create(mbpt_batch **out, int n, const int *M, const int *nocc,
                       const double *const *V_dev, const double *eps_host,
                       int kmax, void *stream);
 
-/* Work decomposition (the tuning knobs; results do not depend on them
- * beyond rounding): grid rows per exchange work item (>= 64), K chunk of
- * the tail product T' X_v and of the Hartree product p V (>= 16), and the
- * persistent grid size of every launch (0: one workgroup per work item).
+/* Work decomposition and tile shape (the tuning knobs; results do not
+ * depend on them beyond rounding): grid rows per exchange work item
+ * (>= 64), K chunk of the tail product T' X_v and of the Hartree product
+ * p V (>= 16), the persistent grid size of every launch (0: one workgroup
+ * per work item), and the exchange's bucket width (16 or 32): each
+ * instance's n_o is padded to a multiple of it, which sets the K depth of
+ * the exchange's P product and the instances sharing one launch.
  * Synchronises the stream; reallocates the workspace, so a graph captured
  * before must be captured again. */
 int mbpt_batch_plan(mbpt_batch *h, int rows_per_split, int tail_kchunk,
-                    int hartree_kchunk, int grid_cap, void *stream);
+                    int hartree_kchunk, int grid_cap, int bucket_width, void *stream);
 
 /* out_dev (n, npair_max) FP64: row b, first nocc[b] nvir[b] entries, gets
  * instance b's screened diagonal d - (ii|W|aa) (`screened_diagonal()`,

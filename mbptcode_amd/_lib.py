@@ -102,7 +102,7 @@ _SIGNATURES = {
     'mbpt_batch_create': (_i, [ctypes.POINTER(_vp), _i, _ip, _ip, _ip, _i, _d,
                                ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                ctypes.POINTER(_vp), _dp, _i, _vp]),
-    'mbpt_batch_plan': (_i, [_vp, _i, _i, _i, _i, _vp]),
+    'mbpt_batch_plan': (_i, [_vp, _i, _i, _i, _i, _i, _vp]),
     'mbpt_batch_screened_diag': (_i, [_vp, _vp, _vp]),
     'mbpt_batch_npair_max': (_i, [_vp]),
     'mbpt_batch_held_bytes': (ctypes.c_size_t, [_vp]),

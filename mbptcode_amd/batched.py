@@ -47,11 +47,13 @@ _SLOT_SHIFT = 1e10
 _PAD_DIAG = 1e6
 MAX_SPACE = 64
 
-DEFAULT_PLAN = dict(rows_per_split=1024, tail_kchunk=512, hartree_kchunk=512, grid_cap=0)
+DEFAULT_PLAN = dict(rows_per_split=1024, tail_kchunk=512, hartree_kchunk=512, grid_cap=0,
+                    bucket_width=16)
 PLAN_SPACE = dict(rows_per_split=(256, 512, 1024, 2048, 4096),
                   tail_kchunk=(128, 256, 512, 1024, 4096),
                   hartree_kchunk=(256, 1024, 4096),
-                  grid_cap=(0, 2048, 4096, 8192))
+                  grid_cap=(0, 2048, 4096, 8192),
+                  bucket_width=(16, 32))
 SHIPPED_TUNING = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'data',
                               'batch_tune.json')
 
@@ -268,12 +270,14 @@ class BatchedSigma:
         return graphs[graph]
 
     # -- work -----------------------------------------------------------------
-    def plan(self, rows_per_split, tail_kchunk, hartree_kchunk, grid_cap=0):
+    def plan(self, rows_per_split, tail_kchunk, hartree_kchunk, grid_cap=0, bucket_width=16):
         _lib.check(_lib.lib().mbpt_batch_plan(self._h, int(rows_per_split), int(tail_kchunk),
                                               int(hartree_kchunk), int(grid_cap),
-                                              _lib.stream_ptr()), 'mbpt_batch_plan')
+                                              int(bucket_width), _lib.stream_ptr()),
+                   'mbpt_batch_plan')
         self.plan_args = dict(rows_per_split=int(rows_per_split), tail_kchunk=int(tail_kchunk),
-                              hartree_kchunk=int(hartree_kchunk), grid_cap=int(grid_cap))
+                              hartree_kchunk=int(hartree_kchunk), grid_cap=int(grid_cap),
+                              bucket_width=int(bucket_width))
         self.generation += 1
 
     def apply_into(self, z, out, kact=None):

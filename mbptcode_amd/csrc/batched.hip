@@ -27,7 +27,8 @@ namespace {
 
 struct Inst {
   int id;                      // position in the caller's instance order
-  int M, no, nv, nopad;
+  int M, no, nv, nopad;        // nopad: n_o padded to the plan's exchange bucket width
+  int ldxo;                    // row stride of the pool's X_o (n_o padded to 32)
   long xo, xv, xot, w, v;      // element offsets into the operand pool
   long d;                      // offset into the pair energies (FP64)
   long zxv, p, u, tp, tc, ep;  // element offsets into the workspace
@@ -238,7 +239,8 @@ __global__ __launch_bounds__(256) void g_exchange(Tables t, int k, const int *__
                                                   const T *__restrict__ pool,
                                                   T *__restrict__ ws) {
   // X_o of an instance sits in the pool zero-padded to (M rounded to 32) x
-  // NOPAD, and streams through double-buffered 32-row LDS stages by direct
+  // ldxo (>= NOPAD); its first NOPAD columns stream through double-buffered
+  // 32-row LDS stages by direct
   // 16-byte global-to-LDS loads (one instruction per row, a stage ahead); the
   // next 16 rows' W are prefetched.  Rows past M are zero rows (P = 0 there,
   // W read at a clamped row).
@@ -278,7 +280,7 @@ __global__ __launch_bounds__(256) void g_exchange(Tables t, int k, const int *__
         const int r = 4 * q + wave;
         if (lane < NL)
           __builtin_amdgcn_global_load_lds(
-              (const void *)((const char *)(Xp + (long)(ROWS * st + r) * NOPAD) + 16 * lane),
+              (const void *)((const char *)(Xp + (long)(ROWS * st + r) * I.ldxo) + 16 * lane),
               (lds_ptr *)&xs[n][r * LD], 16, 0, 0);
       }
     };
@@ -495,7 +497,7 @@ struct mbpt_batch {
   double *dpool;        // pair energies
   // plan
   int planned;
-  int rows_target, tail_kchunk, hartree_kchunk, grid_cap;
+  int rows_target, tail_kchunk, hartree_kchunk, grid_cap, bucket_width;
   Inst *dinst;          // instances in input order
   Inst *dinst_bucket;   // instances regrouped by occupied bucket (exchange)
   int *pre;             // S_N x (n + 1) prefix sums
@@ -525,7 +527,7 @@ template <typename T> int upload(mbpt_batch *h, const double *const *X, const do
   for (int b = 0; b < h->n; ++b) {
     const Inst &I = h->hinst[b];
     const long M = I.M;
-    g_split_X<T><<<grid1(M * (I.no + I.nv)), 256, 0, st>>>(I.M, I.no, I.nv, I.nopad, X[b],
+    g_split_X<T><<<grid1(M * (I.no + I.nv)), 256, 0, st>>>(I.M, I.no, I.nv, I.ldxo, X[b],
                                                            pool + I.xo,
                                                            pool + I.xv, pool + I.xot);
     g_convert<T><<<grid1(M * M), 256, 0, st>>>(M * M, W[b], pool + I.w);
@@ -593,20 +595,25 @@ int mbpt_batch_destroy(mbpt_batch *h) {
 }
 
 int mbpt_batch_plan(mbpt_batch *h, int rows_per_split, int tail_kchunk, int hartree_kchunk,
-                    int grid_cap, void *stream) {
+                    int grid_cap, int bucket_width, void *stream) {
   MBPT_REQUIRE(h, "mbpt_batch_plan: NULL handle");
   MBPT_REQUIRE(rows_per_split >= 64 && tail_kchunk >= 16 && hartree_kchunk >= 16 && grid_cap >= 0,
                "mbpt_batch_plan: rows_per_split >= 64, K chunks >= 16, grid_cap >= 0");
+  MBPT_REQUIRE(bucket_width == 16 || bucket_width == 32,
+               "mbpt_batch_plan: bucket_width is 16 or 32");
   hipStream_t st = (hipStream_t)stream;
   MBPT_HIP_TRY(hipStreamSynchronize(st));
   free_plan(h);
   const int n = h->n, k = h->kmax;
   h->rows_target = rows_per_split; h->tail_kchunk = tail_kchunk;
-  h->hartree_kchunk = hartree_kchunk; h->grid_cap = grid_cap;
+  h->hartree_kchunk = hartree_kchunk; h->grid_cap = grid_cap; h->bucket_width = bucket_width;
   std::vector<int> pre(S_N * (n + 1), 0);
   long wso = 0;
   for (int b = 0; b < n; ++b) {
     Inst &I = h->hinst[b];
+    // the exchange's K padding: n_o up to the bucket width (the pool's X_o
+    // rows are padded to 32, so either width reads them in place)
+    I.nopad = (I.no + bucket_width - 1) / bucket_width * bucket_width;
     const long Mb = I.M;
     const int Mr = (I.M + 63) / 64 * 64;
     I.rps = std::min(Mr, (rows_per_split + 63) / 64 * 64);
@@ -705,9 +712,10 @@ int mbpt_batch_create(mbpt_batch **out, int n, const int *M, const int *nocc, co
     I.id = b;
     I.M = M[b]; I.no = nocc[b]; I.nv = nvir[b];
     I.nopad = (I.no + 15) / 16 * 16;
+    I.ldxo = (I.no + 31) / 32 * 32;
     npmax = std::max(npmax, I.no * I.nv);
     const long Mb = I.M;
-    I.xo = fo; fo += (Mb + 31) / 32 * 32 * I.nopad;   // zero-padded for the exchange
+    I.xo = fo; fo += (Mb + 31) / 32 * 32 * I.ldxo;   // zero-padded for the exchange
     I.xv = fo; fo += Mb * I.nv;
     I.xot = fo; fo += Mb * I.no;
     I.w = fo; fo += Mb * Mb;
@@ -738,7 +746,7 @@ int mbpt_batch_create(mbpt_batch **out, int n, const int *M, const int *nocc, co
   }
   int rc = precision == MBPT_FP64 ? upload<double>(h, X_dev, W_dev, V_dev, st)
                                   : upload<float>(h, X_dev, W_dev, V_dev, st);
-  if (!rc) rc = mbpt_batch_plan(h, 1024, 512, 512, 0, stream);
+  if (!rc) rc = mbpt_batch_plan(h, 1024, 512, 512, 0, 16, stream);
   if (rc) {
     mbpt_batch_destroy(h);
     return rc;

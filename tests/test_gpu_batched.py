@@ -143,7 +143,11 @@ def test_plans_agree(batch):
     op = make_op(insts, Ws, 'fp64')
     z = random_z(op, seed=9)
     ref = op.apply(z).cpu().numpy()
-    for plan in [(64, 16, 16, 8), (128, 48, 32, 0), (4096, 4096, 4096, 64), (256, 128, 1024, 16)]:
+    # the last knob is the exchange bucket width: 32 pads n_o = 5, 9, 17 to
+    # 32 and 33, 40 to 64 (the K depth of the P product, and which
+    # instances share an exchange launch)
+    for plan in [(64, 16, 16, 8, 32), (128, 48, 32, 0, 16), (4096, 4096, 4096, 64, 32),
+                 (256, 128, 1024, 16, 16), (1024, 512, 512, 0, 32)]:
         op.plan(*plan)
         got = op.apply(z).cpu().numpy()
         for b in range(op.n):
