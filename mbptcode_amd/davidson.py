@@ -816,14 +816,16 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
                  learned=None, stall_cycles=3, learned_until=1000.0,
-                 learned_mode='augment', phases=None, keep=None):
+                 learned_mode='augment', phases=None, keep=None, learned_gate='root'):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
 
     preconditioner: 'bare' (d), 'screened' (d - (ii|W|aa)) or 'learned' (the
     MLP correction of `learned`, a `LearnedPreconditioner`, with the screened
     diagonal as its fallback whenever the residual stalls for `stall_cycles`
-    cycles).  refine_apply: an FP64 block action of the same operator; the
+    cycles).  The learned correction is applied to the roots whose residual
+    is at least learned_until * conv_tol (`learned_gate='root'`), or to all
+    while the largest one is (`'max'`); below, Jacobi-Davidson alone.  refine_apply: an FP64 block action of the same operator; the
     roots are then re-checked and polished in FP64 (the FP64 final residual
     check).  Returns NumPy omega (nroots,), X (n_pair, nroots), Y = 0.
     """
@@ -881,9 +883,24 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
         def precond(R, U, theta, cycle):
             # near convergence the useful (out-of-space) part of a correction
             # is small next to the network's BF16 output noise: the
-            # Jacobi-Davidson correction takes over below learned_until * tol
+            # Jacobi-Davidson correction takes over below learned_until * tol,
+            # root by root
             jd = jacobi_davidson_correction(R, U, pdiag, theta)
-            if (state['fallback'] or learned_until is not None and
+            if state['fallback']:
+                return jd
+            if learned_gate == 'root' and learned_until is not None:
+                big = torch.nonzero(R.norm(dim=0) >= learned_until * conv_tol).squeeze(1)
+                if big.numel() == 0:
+                    return jd
+                if big.numel() < R.shape[1]:
+                    t = learned.correction(R[:, big], U[:, big], theta[big], pdiag,
+                                           jd=jd[:, big])
+                    if learned_mode == 'augment':
+                        return torch.cat([jd, t], dim=1)
+                    jd = jd.clone()
+                    jd[:, big] = t
+                    return jd
+            elif (learned_until is not None and
                     float(R.norm(dim=0).max()) < learned_until * conv_tol):
                 return jd
             t = learned.correction(R, U, theta, pdiag, jd=jd)
