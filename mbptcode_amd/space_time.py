@@ -519,7 +519,10 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     """E_c with the interaction on the grid: log det(I - L^T chi0 L) with
     L = chol(V), Tr(L^T chi0 L) = sum(chi0 o V); every proj(tau) built once
     and transformed to the frequencies a batch at a time (`FrequencySweep`),
-    FP64 GEMMs and Cholesky log-dets.
+    Cholesky log-dets in FP64.  precision 'fp32': the terms (where their
+    weights allow) and the products L^T chi0 L in FP32 (library GEMMs, the
+    result widened to FP64 before I - K is formed and factorized); 'fp64':
+    everything in FP64.
 
     The matrices of up to `factor_batch_size` frequencies are factorized in
     one batched Cholesky; the log-dets and traces stay on the device until
@@ -548,11 +551,22 @@ def rpa_correlation_energy_grid(X, V, eps, nocc, grid, mu=None, precision='fp32'
     g = factor_batch_size(n, nf, per_matrix=3)
     chis = torch.empty((g, n, n), dtype=torch.float64, device=dev)
     A = torch.empty((g, n, n), dtype=torch.float64, device=dev)
+    if precision == 'fp32':
+        L32 = L.float()
+        L32t = L32.T.contiguous()
+        c32, Y32, K32 = (torch.empty((n, n), dtype=torch.float32, device=dev)
+                         for _ in range(3))
     for k0 in range(0, nf, g):
         ks = list(range(k0, min(nf, k0 + g)))
         sweep.build(ks, out=chis[:len(ks)])
         for j, k in enumerate(ks):
-            K = L.T @ (chis[j] @ L)
+            if precision == 'fp32':
+                c32.copy_(chis[j])
+                torch.matmul(c32, L32, out=Y32)
+                torch.matmul(L32t, Y32, out=K32)
+                K = K32.double()
+            else:
+                K = L.T @ (chis[j] @ L)
             trace[k] = torch.trace(K)
             torch.sub(eye, 0.5 * (K + K.T), out=A[j])
         C, inf = torch.linalg.cholesky_ex(A[:len(ks)])
