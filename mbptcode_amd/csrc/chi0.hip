@@ -206,19 +206,21 @@ int grow(float *&buf, size_t &cap, size_t bytes) {
   return MBPT_OK;
 }
 
-// time-point chunks per tile: the split whose item count best fills whole
-// rounds of one workgroup per CU.  Two fit per CU (the kernel is held to 128
-// VGPRs), but counting 512 slots chose more chunks and measured 3-9% slower:
-// more partial sums to write and reduce, for a tail the second resident
-// workgroup already shortens.
+// time-point chunks per tile.  A split into c chunks takes
+// ceil(ntiles c / 256) rounds of one workgroup per CU, each ceil(ntau / c)
+// time points long, plus c partial planes to write and sum (about half a
+// time-point round each).  The split with the least of that is taken.  Two
+// workgroups fit per CU (the kernel is held to 128 VGPRs), but counting 512
+// slots chose more chunks and measured 3-9% slower: more partial sums for a
+// tail the second resident workgroup already shortens.
 int chi0_chunks(int ntiles, int ntau) {
   const int slots = 256;
   int best = 1;
-  double best_eff = 0.0;
+  double best_cost = 1e300;
   for (int c = 1; c <= std::min(ntau, 8); ++c) {
-    const long items = (long)ntiles * c;
-    const double eff = (double)items / (double)(((items + slots - 1) / slots) * slots);
-    if (eff > best_eff + 0.02) { best_eff = eff; best = c; }
+    const long rounds = ((long)ntiles * c + slots - 1) / slots;
+    const double cost = (double)rounds * ((ntau + c - 1) / c) + 0.5 * c;
+    if (cost < best_cost - 1e-9) { best_cost = cost; best = c; }
   }
   return best;
 }
@@ -248,7 +250,8 @@ int build_f32(mbpt_chi0 *h, int ntau, const double *tau, const double *w, double
   k_chi0_f32<<<nblk, 512, 0, st>>>(Mp, ko, kv, nt, nchunk, tpc, h->Yt, tw, h->part, nblk);
   if ((rc = launch_status("k_chi0_f32"))) return rc;
   const int nto = (M + 63) / 64;
-  k_chi0_reduce<<<nto * nto, 256, 0, st>>>(M, Mp, nchunk, nto, h->part, out, accumulate);
+  k_chi0_reduce<<<nto * (nto + 1) / 2, 256, 0, st>>>(M, Mp, nchunk, nto, h->part, out,
+                                                       accumulate);
   if ((rc = launch_status("k_chi0_reduce"))) return rc;
   if (h->prof) {
     MBPT_HIP_TRY(hipEventRecord(h->e1, st));

@@ -174,16 +174,22 @@ __global__ __launch_bounds__(512, 4) void k_chi0_f32(int Mp, int ko, int kv, int
       }
 }
 
-// out = [acc ? out : 0] + sum_chunks part, one 64 x 64 output tile per block:
-// tiles below the diagonal read the mirrored upper tile and transpose it in
-// LDS, so loads and stores stay coalesced.
+// out = [acc ? out : 0] + sum_chunks part, one pair of mirrored 64 x 64 output
+// tiles per block: the upper tile's partials are read once, and the sum is
+// written to the upper tile and, transposed in LDS, to the lower one, so
+// loads and stores stay coalesced and the result is exactly symmetric.
 __global__ __launch_bounds__(256) void k_chi0_reduce(int M, int Mp, int nchunk, int nto,
                                                      const float *__restrict__ part,
                                                      double *__restrict__ out,
                                                      int accumulate) {
-  const int bi = blockIdx.x / nto, bj = blockIdx.x % nto;
-  const bool upper = (bi * 64) / kCT <= (bj * 64) / kCT;
-  const int sr0 = upper ? bi * 64 : bj * 64, sc0 = upper ? bj * 64 : bi * 64;
+  // block pair p -> 64-row blocks (bi, bj), bi <= bj: the partials hold every
+  // such block (it lies on or above the 128-tile diagonal); the sum is
+  // written there and, mirrored through LDS, at (bj, bi)
+  const int p = blockIdx.x;
+  int bi = 0, base = 0;
+  while (p >= base + (nto - bi)) { base += nto - bi; ++bi; }
+  const int bj = bi + (p - base);
+  const int sr0 = bi * 64, sc0 = bj * 64;
   __shared__ float t[64][65];
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
     const int r = e >> 6, c = e & 63;
@@ -195,11 +201,18 @@ __global__ __launch_bounds__(256) void k_chi0_reduce(int M, int Mp, int nchunk, 
   __syncthreads();
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
     const int r = e >> 6, c = e & 63;
-    const int row = bi * 64 + r, col = bj * 64 + c;
-    if (row >= M || col >= M) continue;
-    const double v = upper ? t[r][c] : t[c][r];
-    const long o = (long)row * M + col;
-    out[o] = accumulate ? out[o] + v : v;
+    int row = sr0 + r, col = sc0 + c;
+    if (row < M && col < M) {
+      const long o = (long)row * M + col;
+      out[o] = accumulate ? out[o] + (double)t[r][c] : (double)t[r][c];
+    }
+    if (bi == bj) continue;
+    row = sc0 + r;
+    col = sr0 + c;
+    if (row < M && col < M) {
+      const long o = (long)row * M + col;
+      out[o] = accumulate ? out[o] + (double)t[c][r] : (double)t[c][r];
+    }
   }
 }
 
