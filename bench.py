@@ -221,6 +221,9 @@ def exchange_roofline(prof, precision, nocc, M):
            'launches': prof['launches'], 'flops_per_launch': ex_flops,
            'flop_definition': 'k (4 nocc M^2 + M^2) per launch, k = trial vectors in '
                               'the launch (algorithmic FP32-equivalent flops)',
+           'launch_set': 'the timed steps\' launches that have the device to themselves: '
+                         'the last cycle\'s launch of each solve, which runs beside the '
+                         'FP64 recheck on a second stream, is left out',
            'stage_ms_per_apply': {k: v / prof['launches'] for k, v in prof['ms'].items()}}
     if precision == 'bf16x3':
         # MFMAs the kernel issues per (vector, 32-row chunk, 16-column tile):

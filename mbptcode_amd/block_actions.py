@@ -60,6 +60,15 @@ def _as_device(a, dev):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
 
 
+def _add_counters(a, b):
+    """The sum of two `profile_read` records (a may be None)."""
+    if a is None:
+        return b
+    return {'ms': {k: a['ms'][k] + b['ms'][k] for k in b['ms']},
+            'launches': a['launches'] + b['launches'],
+            'exchange_flops': a['exchange_flops'] + b['exchange_flops']}
+
+
 class SigmaOperator:
     """A z for batches of Tamm-Dancoff trial vectors, on one GPU.
 
@@ -203,9 +212,9 @@ class SigmaOperator:
         """Start (and zero) or stop per-stage device timing of `apply`."""
         _lib.check(self._lib.mbpt_sigma_profile(self._h, int(enable)),
                    'mbpt_sigma_profile')
+        self._prof = {'on': bool(enable), 'held': None}
 
-    def profile_read(self):
-        """{'ms': {head, hartree, exchange, tail}, 'launches', 'exchange_flops'}."""
+    def _profile_counters(self):
         ms = (ctypes.c_double * 4)()
         n = ctypes.c_long()
         fl = ctypes.c_double()
@@ -213,6 +222,32 @@ class SigmaOperator:
             self._h, ms, ctypes.byref(n), ctypes.byref(fl)), 'mbpt_sigma_profile_read')
         return {'ms': dict(zip(('head', 'hartree', 'exchange', 'tail'), list(ms))),
                 'launches': int(n.value), 'exchange_flops': float(fl.value)}
+
+    def profile_pause(self):
+        """Stop timing while keeping what was timed: applies that share the
+        device with other work (a check running on a second stream) are
+        left out of the figures.  `profile_resume` goes on."""
+        st = getattr(self, '_prof', None)
+        if not st or not st['on'] or st['held'] is not None:
+            return
+        st['held'] = self._profile_counters()
+        _lib.check(self._lib.mbpt_sigma_profile(self._h, 0), 'mbpt_sigma_profile')
+
+    def profile_resume(self):
+        st = getattr(self, '_prof', None)
+        if not st or st['held'] is None:
+            return
+        held = st['held']
+        _lib.check(self._lib.mbpt_sigma_profile(self._h, 1), 'mbpt_sigma_profile')
+        st['held'] = None
+        st['base'] = _add_counters(st.get('base'), held)
+
+    def profile_read(self):
+        """{'ms': {head, hartree, exchange, tail}, 'launches', 'exchange_flops'}
+        of the timed applies since `profile(True)`."""
+        st = getattr(self, '_prof', None) or {}
+        now = st['held'] if st.get('held') is not None else self._profile_counters()
+        return _add_counters(st.get('base'), now)
 
     def screened_diagonal_device(self):
         """d - (ii|W|aa), (nocc, nvir) FP64 on the device; formed once per

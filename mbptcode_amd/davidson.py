@@ -931,6 +931,8 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                 # iteration finishes the extras on this one
                 main = torch.cuda.current_stream(dev)
                 side.wait_stream(main)
+                if hasattr(op, 'profile_pause'):
+                    op.profile_pause()     # this operator's applies now share the chip
                 with torch.cuda.stream(side):
                     theta, rn_e = fp64_check(Xr)
                 Xr.record_stream(side)
@@ -952,6 +954,8 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
         Xr = X[:, :nroots].contiguous()
         if checked:
             torch.cuda.current_stream(dev).wait_stream(side)
+            if hasattr(op, 'profile_resume'):
+                op.profile_resume()
             rn64 = checked['rn'].cpu().numpy()
             refine['early'] = bool((rn64 <= conv_tol).all())
             if refine['early']:
