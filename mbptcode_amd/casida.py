@@ -39,7 +39,8 @@ Simplifications against the reference:
 
 * Tracked extra pairs must converge; the reference also settles them by their
   residual interval clearing the highest requested root.
-* There are no pair-symmetry sectors.
+* Pair-symmetry sectors (`x0sym`) label the trial pairs; the warm start
+  takes none (as the reference refuses one with them).
 * A pair space the trial space exhausts is finished densely by the caller
   (`davidson._casida_root` through `davidson.dense_casida`).
 * A projected a - b that is not positive definite raises LinAlgError here;
@@ -124,15 +125,45 @@ def symmetric_coefficients(s11, s21, lindep):
 
 class CasidaResult:
     __slots__ = ('e', 'X', 'Y', 'conv', 'rnorm', 'cycles', 'vectors', 'seconds',
-                 'trace', 'exhausted')
+                 'trace', 'exhausted', 'labels')
 
     def __init__(self, **kw):
         for k in self.__slots__:
             setattr(self, k, kw.get(k))
 
 
+def sector_solve(a, b, sigma, pi, labels, k):
+    """(omega, x, y, labels): `subspace_solve` sector by sector when the
+    trial pairs carry symmetry labels (`labels`, one per trial pair): the
+    projected problem of each sector is solved apart, its coefficients
+    embedded in the whole space (zero on the other sectors), and the roots of
+    all sectors merged in ascending order; the k lowest are kept with the
+    labels of their sectors (trial_space.py:624-648).  labels None: one
+    solve."""
+    if labels is None:
+        w, x, y = subspace_solve(a, b, sigma, pi, k)
+        return w, x, y, None
+    m = a.shape[0]
+    ws, xs, ys, ls = [], [], [], []
+    for lab in np.unique(labels):
+        idx = np.flatnonzero(labels == lab)
+        sub = np.ix_(idx, idx)
+        w, x, y = subspace_solve(a[sub], b[sub], sigma[sub], pi[sub], idx.size)
+        xe = np.zeros((m, w.size))
+        ye = np.zeros((m, w.size))
+        xe[idx], ye[idx] = x, y
+        ws.append(w)
+        xs.append(xe)
+        ys.append(ye)
+        ls.append(np.full(w.size, lab))
+    w, x, y, lab = (np.concatenate(ws), np.concatenate(xs, 1), np.concatenate(ys, 1),
+                    np.concatenate(ls))
+    order = np.argsort(w, kind='stable')[:k]
+    return w[order], x[:, order], y[:, order], lab[order]
+
+
 def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=None,
-                    lindep=None, y0=None, phases=None):
+                    lindep=None, y0=None, phases=None, x0sym=None):
     """Paired Davidson for the `nroots` lowest Casida roots, on device (or
     CPU) tensors.
 
@@ -142,6 +173,13 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     returned root satisfies |[R_x; R_y]| <= tol where `conv` says so.
     `phases`, a dict, receives the seconds of each part of the cycle (the
     device is synchronised at every boundary, so only for profiling).
+    x0sym: a symmetry label for each guess column (the irrep of its pair);
+    the trial pairs then keep labels, the projected problem is solved sector
+    by sector (`sector_solve`), a root's correction takes the root's label
+    and the corrections are orthonormalised sector by sector (against the
+    whole held space), as the reference's symmetry-blocked loop does
+    (trial_space.py:424-452, :514-648); `labels` of the result are the
+    roots' sectors.
 
     The holders V, W, U1, U2 are four column blocks of one row-major buffer
     (n, 4 cap), so [V | W] or [U1 | U2] is one operand of the two-range
@@ -180,6 +218,7 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     m = 0
     vectors = 0
     exhausted = False
+    lab = None if x0sym is None else np.zeros(cap, dtype=np.int64)
 
     def pair(o1, o2, c0, c):
         """The columns c0 .. c0 + c of the blocks at o1 and o2, one operand."""
@@ -231,11 +270,12 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
             M_[:m0, m0:m1] = sign * rows_[:, :m0].T
         m = m1
 
-    def orthonormal_new(p):
+    def orthonormal_new(p, tlab=None):
         """The p corrections held in V's and W's columns m .. m + p,
         projected twice off the held pairs and their partners and
-        orthonormalised with their own partners, in place; returns how many
-        directions remain (in columns m .. m + q)."""
+        orthonormalised with their own partners, in place (sector by sector
+        when `tlab` labels them); returns how many directions remain (in
+        columns m .. m + q, their labels in lab[m:m + q])."""
         hx = lambda M_: torch.as_tensor(np.ascontiguousarray(M_), device=dev)
         if m:
             O = _tnr(pair(oV, oW, 0, m), pair(oV, oW, m, p))           # (2m, 2p)
@@ -252,12 +292,26 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         oy2 = Vt[:, p:] + Wt[:, :p]
         s11 = Tx[:, :p] + Ty[:, p:] - ox2.T @ ox2 - oy2.T @ oy2
         s21 = Tx[:, p:] + Ty[:, :p] - ox2.T @ oy2 - oy2.T @ ox2
+        groups = ([np.arange(p)] if tlab is None else
+                  [np.flatnonzero(tlab == g) for g in np.unique(tlab)])
+        blocks = []
         with _host_lapack():
-            coef = symmetric_coefficients(s11, s21, lindep)
-        if coef is None:
+            for gi in groups:
+                coef = symmetric_coefficients(s11[np.ix_(gi, gi)], s21[np.ix_(gi, gi)],
+                                              lindep)
+                if coef is not None:
+                    blocks.append((gi, coef))
+        if not blocks:
             return 0
-        ca, cb = coef
-        q = ca.shape[1]
+        q = sum(c[0].shape[1] for _, c in blocks)
+        ca, cb = np.zeros((p, q)), np.zeros((p, q))
+        j = 0
+        for gi, (cga, cgb) in blocks:
+            ca[gi, j:j + cga.shape[1]] = cga
+            cb[gi, j:j + cga.shape[1]] = cgb
+            if tlab is not None:
+                lab[m + j:m + j + cga.shape[1]] = tlab[gi[0]]
+            j += cga.shape[1]
         # t_x' = (t_x - V ox2 - W oy2) ca + (t_y - W ox2 - V oy2) cb, and its
         # partner, each one product over [V t_x | W t_y]
         Yx = np.concatenate([-(ox2 @ ca + oy2 @ cb), ca, -(oy2 @ ca + ox2 @ cb), cb])
@@ -278,13 +332,14 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
     cold = y0 is None or not bool(torch.any(y0))
     if not cold:
         H4[:, oW:oW + p0] = y0.to(device=x0.device, dtype=torch.float64)
-    q = orthonormal_new(p0)
+    q = orthonormal_new(p0, None if x0sym is None else np.asarray(x0sym, dtype=np.int64))
     act(0, q, cold)
     append(0, q)
     trace = []
     conv = np.zeros(nroots, bool)
     w = x = y = rn = None
     X = Y = None
+    wlab = None
     cycles = 0
     mark('setup')
     for cycle in range(max_cycle):
@@ -292,7 +347,8 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         k = min(nroots, m)
         try:
             with _host_lapack():
-                w, x, y = subspace_solve(a[:m, :m], b[:m, :m], sg[:m, :m], pi[:m, :m], k)
+                w, x, y, wlab = sector_solve(a[:m, :m], b[:m, :m], sg[:m, :m],
+                                             pi[:m, :m], None if lab is None else lab[:m], k)
         except np.linalg.LinAlgError as exc:
             # whether the trial space had spanned the pair space when the
             # projected (A - B) block broke down (the caller may then solve
@@ -334,7 +390,7 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
         H4[:, oV + m:oV + m + p] = tx / nrm
         H4[:, oW + m:oW + m + p] = ty / nrm
         mark('correction')
-        q = orthonormal_new(p)
+        q = orthonormal_new(p, None if wlab is None else wlab[idx])
         mark('orthonormal')
         if q == 0:
             exhausted = True
@@ -343,6 +399,7 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
             # restart on the lowest Ritz pairs, their products combined from
             # the held ones (the new directions stay orthogonal to them)
             keep = H4[:, [oV + m + j for j in range(q)] + [oW + m + j for j in range(q)]]
+            keep_lab = None if lab is None else lab[m:m + q].copy()
             U12 = torch.empty((n, 2 * kk), dtype=torch.float64, device=dev)
             _nn(pair(oU1, oU2, 0, m), xy, U12, 0)
             _nn(pair(oU1, oU2, 0, m), yx, U12, kk)
@@ -350,9 +407,13 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
             H4[:, oW:oW + kk] = Y
             H4[:, oU1:oU1 + kk] = U12[:, :kk]
             H4[:, oU2:oU2 + kk] = U12[:, kk:]
+            if lab is not None:
+                lab[:kk] = wlab
             append(0, kk)
             H4[:, oV + m:oV + m + q] = keep[:, :q]
             H4[:, oW + m:oW + m + q] = keep[:, q:]
+            if lab is not None:
+                lab[m:m + q] = keep_lab
         mark('collapse')
         act(m, q, False)
         mark('sigma')
@@ -363,4 +424,4 @@ def casida_davidson(apply_ab, diag, nroots, x0, tol, max_cycle=100, max_space=No
                       RuntimeWarning, stacklevel=2)
     return CasidaResult(e=np.asarray(w), X=X, Y=Y, conv=conv, rnorm=rn, cycles=cycles,
                         vectors=vectors, seconds=time.time() - t0, trace=trace,
-                        exhausted=exhausted or m >= n)
+                        exhausted=exhausted or m >= n, labels=wlab)

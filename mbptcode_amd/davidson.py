@@ -830,15 +830,13 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     check).  Returns NumPy omega (nroots,), X (n_pair, nroots), Y = 0.
     """
     if not tda:
-        if x_sym is not None:
-            raise NotImplementedError('pair-symmetry sectors of the Casida Davidson '
-                                      'are not built')
         return _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
                             guess_factor=guess_factor, stats=stats, comm=comm,
                             timings=timings, refuse_unconverged=refuse_unconverged,
                             preconditioner=preconditioner, guess=guess,
                             tracked=tracked, extra_roots=extra_roots,
-                            max_space=max_space, refine_apply=refine_apply)
+                            max_space=max_space, refine_apply=refine_apply,
+                            x_sym=x_sym)
     if x_sym is not None:
         raise ValueError('the Tamm-Dancoff Davidson takes no pair symmetry')
     if comm is not None and comm.Get_size() > 1:
@@ -858,6 +856,15 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     t_precond = time.time() - t0
     order = guess_indices(diag_d, nroots, guess_factor, counters=counters)
     track = tracked_roots(nroots, npair, extra_roots)
+    x0sym = None
+    if x_sym is not None:
+        x_sym = np.asarray(x_sym).ravel()
+        if x_sym.size != npair:
+            raise ValueError(f'x_sym has {x_sym.size} labels for {npair} pairs')
+        if guess is not None:
+            raise ValueError('a warm-started Davidson takes no pair symmetry: the guess '
+                             'vectors carry no irrep label')
+        x0sym = x_sym[np.asarray(order)]
     x0 = torch.zeros((npair, len(order)), dtype=torch.float64, device=dev)
     x0[torch.as_tensor(order, device=dev), torch.arange(len(order), device=dev)] = 1.0
     if guess is not None:
@@ -1032,14 +1039,17 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
                  guess_factor=DAVIDSON_GUESS_FACTOR, stats=None, comm=None, timings=None,
                  refuse_unconverged=False, preconditioner=DAVIDSON_PRECONDITIONER,
                  guess=None, tracked=None, extra_roots=None, max_space=None,
-                 refine_apply=None):
+                 refine_apply=None, x_sym=None):
     """(omega, X, Y): the full Casida roots of a GPU block action, the
     reference's `_davidson_root` (LR/davidson.py:998) with its paired loop
     (`casida.casida_davidson`): the same guess pairs and tracked roots as the
     Tamm-Dancoff solve, the unit guesses with Y = 0 and a warm `guess`
     (X, Y) ahead of them.  refine_apply: an FP64 action of the same operator; the
     requested roots' residuals are recomputed on it, and the solve continues
-    on it from those roots where one is above the tolerance."""
+    on it from those roots where one is above the tolerance.  x_sym: the
+    symmetry label of every pair (`orbital_spaces.pair_symmetry`); the guess
+    pairs carry their pair's label and the loop keeps the sectors apart
+    (LR/davidson.py:1127-1130)."""
     from .casida import casida_davidson
     if comm is not None and comm.Get_size() > 1:
         raise NotImplementedError('one GPU: no pair-row split')
@@ -1059,6 +1069,15 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
         pdiag = op.screened_diagonal_device().reshape(-1)
     order = guess_indices(diag_d, nroots, guess_factor, counters=counters)
     track = tracked_roots(nroots, npair, extra_roots)
+    x0sym = None
+    if x_sym is not None:
+        x_sym = np.asarray(x_sym).ravel()
+        if x_sym.size != npair:
+            raise ValueError(f'x_sym has {x_sym.size} labels for {npair} pairs')
+        if guess is not None:
+            raise ValueError('a warm-started Davidson takes no pair symmetry: the guess '
+                             'vectors carry no irrep label')
+        x0sym = x_sym[np.asarray(order)]
     x0 = torch.zeros((npair, len(order)), dtype=torch.float64, device=dev)
     x0[torch.as_tensor(order, device=dev), torch.arange(len(order), device=dev)] = 1.0
     y0 = torch.zeros_like(x0)
@@ -1088,7 +1107,8 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
     dense = None
     try:
         res = casida_davidson(ab_of(op), pdiag, track, x0, tol=conv_tol,
-                              max_cycle=max_cycle, max_space=max_space, y0=y0)
+                              max_cycle=max_cycle, max_space=max_space, y0=y0,
+                              x0sym=x0sym)
     except np.linalg.LinAlgError as exc:
         if not (getattr(exc, 'space_exhausted', False) and npair <= BSE_DENSE_MAX_NOV):
             raise casida_breakdown(probe_on, diag_d, conv_tol) from exc
@@ -1117,7 +1137,7 @@ def _casida_root(apply_AB, diag_d, nroots, conv_tol, max_cycle,
             try:
                 res2 = casida_davidson(ab64, pdiag, track, X.contiguous(), tol=conv_tol,
                                        max_cycle=max_cycle, max_space=max_space,
-                                       y0=Y.contiguous())
+                                       y0=Y.contiguous(), x0sym=res.labels)
             except np.linalg.LinAlgError as exc:
                 raise casida_breakdown(refine_apply, diag_d, conv_tol) from exc
             e, X, Y, conv, rn = np.asarray(res2.e, float), res2.X, res2.Y, res2.conv, res2.rnorm
@@ -1205,9 +1225,6 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='RPA',
     if orbsym is not None and tda:
         raise ValueError('the Tamm-Dancoff Davidson takes no pair symmetry: its '
                          'trial space is one Hermitian block')
-    if orbsym is not None:
-        raise NotImplementedError('pair-symmetry sectors of the Casida Davidson '
-                                  'are not built')
     if one_body is not None:
         raise NotImplementedError('one_body coupling is not built')
     from .orbital_spaces import bse_pair_diagonal
@@ -1220,7 +1237,10 @@ def solve_casida_davidson(lr_solver, nocc, numerics=None, polarizability='RPA',
         refine_apply = block_action(lr_solver, nocc, polarizability, W_aux,
                                     isdf_factors, spin=spin, comm=comm, tda=tda,
                                     precision='fp64')[0]
+    from .orbital_spaces import get_occ_virt_indices, pair_symmetry
+    occ, virt = get_occ_virt_indices(lr_solver.eps, nocc)
     omega, X, Y = run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle,
+                               pair_symmetry(orbsym, occ, virt),
                                guess_factor=guess_factor, stats=stats, comm=comm,
                                timings=timings, refuse_unconverged=refuse_unconverged,
                                preconditioner=preconditioner, guess=guess, tda=tda,

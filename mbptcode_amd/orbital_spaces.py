@@ -29,3 +29,13 @@ def ov_energies(eps, nocc):
 def bse_pair_diagonal(eps, nocc):
     """eps_a - eps_i as (n_occ, n_vir) (block_actions.py:99-115)."""
     return ov_energies(eps, nocc).reshape(nocc, -1)
+
+
+def pair_symmetry(orbsym, occ, virt):
+    """The irrep label of every occupied-virtual pair, occupied-slow: the XOR
+    of the two orbitals' D2h labels (pyscf's irrep IDs modulo 10); None
+    without orbsym (LR/davidson.py:623-628)."""
+    if orbsym is None:
+        return None
+    d2h = np.asarray(orbsym) % 10
+    return (d2h[np.asarray(occ)][:, None] ^ d2h[np.asarray(virt)][None, :]).ravel()

@@ -91,6 +91,28 @@ CONFIGS = {
 }
 
 
+def inversion_instance(nocc, nvir, M_half, seed, nroots=5, ntau=8, tol=1e-5):
+    """An instance with an exact inversion symmetry, for the pair-symmetry
+    sectors: the points come in pairs r, -r (about the box centre), each
+    orbital is even or odd (X[-r] = +-X[r]), and V, hence W, is invariant
+    under the swap.  A pair (i, a) is even when i and a have the same
+    parity, and A and B couple only pairs of the same parity.
+    `extra['orbsym']` holds D2h-style labels (0 even, 1 odd) per orbital."""
+    rng = np.random.default_rng(seed)
+    e_o = np.sort(rng.uniform(-1.0, -0.3, nocc))
+    e_v = np.sort(rng.uniform(0.05, 3.0, nvir))
+    M = 2 * M_half
+    Xh = rng.normal(0.0, 1.0 / np.sqrt(M), (M_half, nocc + nvir))
+    parity = rng.integers(0, 2, nocc + nvir)
+    sign = 1.0 - 2.0 * parity
+    half = rng.uniform(-0.5 * BOX, 0.5 * BOX, (M_half, 3))
+    pts = np.vstack([half, -half]) + 0.5 * BOX
+    X = np.vstack([Xh, Xh * sign[None, :]])
+    return Instance(nocc, nvir, M, np.concatenate([e_o, e_v]), X, pts,
+                    coulomb_kernel(pts), nroots=nroots, ntau=ntau, tol=tol,
+                    name=f'inversion[{seed}]', extra={'orbsym': parity})
+
+
 def config(cfg, seed=0):
     """The single-instance workload `cfg` (0, 1, 2 or 4) from `seed`."""
     c = dict(CONFIGS[cfg])

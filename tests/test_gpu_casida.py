@@ -202,3 +202,28 @@ def test_dense_completion_against_the_oracle(tda):
         assert np.abs(A @ Xh - Xh * w).max() < 1e-10
     else:
         check_pairs(A, B, w, Xh, Yh, 1e-10)
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_pair_symmetry_sectors_on_the_device(precision):
+    """An instance with an inversion symmetry (`synth.inversion_instance`):
+    the device loop with the pairs' parity labels (x_sym) reaches the dense
+    roots of the oracle, each returned root inside its own sector: its weight
+    on the other sector is round-off (FP64) or the split-BF16 action's
+    rounding of the cancelling mirror terms (below 1e-4 of the root's
+    largest component)."""
+    from mbptcode_amd.orbital_spaces import get_occ_virt_indices, pair_symmetry
+    inst = synth.inversion_instance(12, 40, 240, seed=5, nroots=6)
+    print(f'inversion instance {precision}: 12 x 40 pairs, M = 480', flush=True)
+    D, W_aux, W_grid = screened(inst)
+    A, B = dense(inst, D, W_aux)
+    lab = pair_symmetry(inst.extra['orbsym'], *get_occ_virt_indices(inst.eps, inst.nocc))
+    ref, _, _ = O.dense_casida_roots(A, B)
+    w, X, Y = solve(inst, W_grid, precision, 1e-7, x_sym=lab)[:3]
+    assert np.abs(w - ref[:6]).max() * HA_MEV < (1e-5 if precision == 'fp64' else 1e-3)
+    leak = 1e-9 if precision == 'fp64' else 1e-4
+    for j in range(6):
+        top = np.abs(X[:, j]).max()
+        sect = [L for L in (0, 1) if np.abs(X[lab != L, j]).max() < leak * top
+                and np.abs(Y[lab != L, j]).max() < leak * top]
+        assert len(sect) == 1, j
