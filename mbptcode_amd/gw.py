@@ -59,7 +59,7 @@ def screened_correlation_grid(L, chi0, V=None, infos=None):
     return 0.5 * (Wc + Wc.T)
 
 
-def screened_correlation_frequencies(builder, V, grid):
+def screened_correlation_frequencies(builder, V, grid, products='fp32'):
     """(W - V)(i w_k) on the grid for every frequency of `grid`, (nfreq, M, M)
     FP64: proj(tau) on grid's time axis once (an FP64 builder), its cosine
     transform, Dyson.  The frequencies' I - L^T chi0 L are factorized in
@@ -69,7 +69,14 @@ def screened_correlation_frequencies(builder, V, grid):
     and the factorizations' status is checked once, after all of them.  The
     solves are not run concurrently: on several streams, from one host
     thread or from one thread per stream, torch's rocBLAS / rocSOLVER calls
-    corrupted some frequencies' results (1e-4 relative, at random)."""
+    corrupted some frequencies' results (1e-4 relative, at random).
+    products 'fp32': the products K = L^T chi0 L in FP32 (library GEMMs,
+    widened before I - K is formed); the Cholesky, the triangular solve and
+    Y^T Y - V stay FP64: W - V is a difference of two matrices of V's size,
+    and Y^T Y in FP32 moved Sigma by 3e-5 relative where FP32 K moves it by
+    1e-8.  'fp64': every product in FP64."""
+    if products not in ('fp32', 'fp64'):
+        raise ValueError(f"products={products!r}: 'fp32' or 'fp64'")
     if builder.precision != 'fp64':
         raise ValueError('the screening frequencies need FP64 proj(tau): the '
                          'cosine weights onto them cancel over many digits')
@@ -85,11 +92,21 @@ def screened_correlation_frequencies(builder, V, grid):
     eye = torch.eye(n, dtype=chi.dtype, device=dev)
     Lt = L.T.contiguous()
     A = torch.empty((g, n, n), dtype=chi.dtype, device=dev)
+    if products == 'fp32':
+        L32, Lt32 = L.float(), Lt.float()
+        c32, Y32, K32 = (torch.empty((n, n), dtype=torch.float32, device=dev)
+                         for _ in range(3))
     infos = []
     for k0 in range(0, nf, g):
         ks = range(k0, min(nf, k0 + g))
         for j, k in enumerate(ks):
-            K = L.T @ (chi[k] @ L)
+            if products == 'fp32':
+                c32.copy_(chi[k])
+                torch.matmul(c32, L32, out=Y32)
+                torch.matmul(Lt32, Y32, out=K32)
+                K = K32.double()
+            else:
+                K = L.T @ (chi[k] @ L)
             torch.sub(eye, 0.5 * (K + K.T), out=A[j])
         C, info = torch.linalg.cholesky_ex(A[:len(ks)])
         infos.append(info)
@@ -134,13 +151,15 @@ def diagonal_from_branches(sums, C, S):
 def self_energy_diagonal_space_time(X, V, eps, nocc, states, ntau=None,
                                     npade=grids.DEFAULT_NPADE,
                                     w0=grids.GL_FREQUENCY_SCALE, omega_out=None,
-                                    timings=None):
+                                    timings=None, products='fp32'):
     """(sigma, axes): Sigma^c_pp(i w) (nstates, nfreq) complex of the states
     `states` on the Pade frequencies (or `omega_out`), from the collocation X
     (M, nocc + nvir), the grid Coulomb V (M, M) and the orbital energies.
 
     timings, a dict, receives the seconds of each stage (device synchronised)
-    and 'sigma_kernel' ({'ms', 'launches', 'flops'} of the fused sums)."""
+    and 'sigma_kernel' ({'ms', 'launches', 'flops'} of the fused sums).
+    products: the precision of the screening's products L^T chi0 L
+    (`screened_correlation_frequencies`)."""
     dev = _device()
     eps = np.asarray(eps, float)
     states = np.atleast_1d(np.asarray(states, dtype=int))
@@ -160,7 +179,7 @@ def self_energy_diagonal_space_time(X, V, eps, nocc, states, ntau=None,
     Ctw, C, S = self_energy_transforms(axes, omega_out)
     lap('transforms')
     b64 = Chi0Builder(X, eps, nocc, mu=axes['mu'], precision='fp64', device=dev)
-    Wc = screened_correlation_frequencies(b64, V, axes['grid'])
+    Wc = screened_correlation_frequencies(b64, V, axes['grid'], products=products)
     del b64
     lap('screening')
     b = Chi0Builder(X, eps, nocc, mu=axes['mu'], precision='fp32', device=dev)
