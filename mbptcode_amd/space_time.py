@@ -412,7 +412,10 @@ def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
     """(W, chi0) on the grid: W = (I - V chi0(0))^-1 V computed as
     L (I - L^T chi0 L)^-1 L^T with L = chol(V) -- two FP64 Cholesky
     factorisations, a triangular solve and a symmetric product on the device;
-    chi0(0) from the fused kernel at `precision`."""
+    chi0(0) from the fused kernel at `precision`.  precision 'fp32': the
+    products L^T chi0 L and Y^T Y (library GEMMs) in FP32 as well, widened
+    to FP64 before I - K is factorized and for W; the factorizations and the
+    solve stay FP64."""
     dev = _device()
     grid = static_screening_grid(eps, nocc) if grid is None else grid
     Vd = _dev(V, dev)
@@ -420,7 +423,11 @@ def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
         chi0 = chi0_grid(X, eps, nocc, grid.tau_points, grid.cosft_wt[0], mu=mu,
                          precision=precision)
     L = torch.linalg.cholesky(Vd)
-    K = L.T @ (chi0 @ L)
+    if precision == 'fp32':
+        L32 = L.float()
+        K = (L32.T @ (chi0.float() @ L32)).double()
+    else:
+        K = L.T @ (chi0 @ L)
     n = K.shape[0]
     A = torch.eye(n, dtype=K.dtype, device=dev) - 0.5 * (K + K.T)
     C, info = torch.linalg.cholesky_ex(A)
@@ -428,7 +435,11 @@ def screened_interaction_grid(X, V, eps, nocc, grid=None, mu=None,
         raise np.linalg.LinAlgError('I - L^T chi0 L is not positive definite')
     # W = L A^-1 L^T = (C^-1 L^T)^T (C^-1 L^T)
     Y = solve_lower_blocked(C, L.T.contiguous())
-    W = Y.T @ Y
+    if precision == 'fp32':
+        Y32 = Y.float()
+        W = (Y32.T @ Y32).double()
+    else:
+        W = Y.T @ Y
     return 0.5 * (W + W.T), chi0
 
 
