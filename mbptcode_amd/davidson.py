@@ -336,7 +336,7 @@ def _jd(U, R, diag, theta):
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
              lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None,
-             keep=None, pick=None, early=None, early_roots=0):
+             keep=None, pick=None, early=None, early_roots=0, precond_roots=False):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -368,6 +368,8 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     converge instead of the lowest (root following, `overlap_pick`): envs
     holds 'ref_coeff', V^T span (m, r), when the pick carries a `span`
     (n, r), else 'xs', the trial vectors as host arrays.
+    precond_roots: call precond(R, U, theta, cycle, roots) with the tracked
+    roots' indices of the columns (ascending).
     `early(X, w)` is called once, at the first cycle whose Ritz step finds
     the leading `early_roots` pairs converged while a later tracked pair is
     not, with those Ritz vectors (n, early_roots) on the device and their
@@ -487,6 +489,8 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         def correction(pre, jd):
             if jd:
                 return _jd(Xu, Ru, diag, wu)
+            if precond_roots:
+                return pre(Ru, Xu, wu, cycle, idx).contiguous()
             return pre(Ru, Xu, wu, cycle).contiguous()
         Tc = correction(precond, use_jd)
         mark('precond')
@@ -816,7 +820,8 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                  read_tol=None, guess=None, tda=False, tracked=None,
                  extra_roots=None, max_space=None, refine_apply=None,
                  learned=None, stall_cycles=3, learned_until=1000.0,
-                 learned_mode='augment', phases=None, keep=None, learned_gate='root'):
+                 learned_mode='augment', phases=None, keep=None, learned_gate='root',
+                 learned_roots='requested'):
     """(omega, X, Y): the TDA roots from a GPU block action
     (LR/davidson.py:631 with tda=True -> `_tda_davidson_root` :462).
 
@@ -825,7 +830,10 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     diagonal as its fallback whenever the residual stalls for `stall_cycles`
     cycles).  The learned correction is applied to the roots whose residual
     is at least learned_until * conv_tol (`learned_gate='root'`), or to all
-    while the largest one is (`'max'`); below, Jacobi-Davidson alone.  refine_apply: an FP64 block action of the same operator; the
+    while the largest one is (`'max'`); below, Jacobi-Davidson alone.  With
+    learned_roots='requested' only the nroots requested roots get it (the
+    extra tracked roots, which only guard against a skipped root, keep the
+    Jacobi-Davidson correction); 'all': every tracked root.  refine_apply: an FP64 block action of the same operator; the
     roots are then re-checked and polished in FP64 (the FP64 final residual
     check).  Returns NumPy omega (nroots,), X (n_pair, nroots), Y = 0.
     """
@@ -887,7 +895,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                              '(a LearnedPreconditioner)')
         state = {'fallback': False}
 
-        def precond(R, U, theta, cycle):
+        def precond(R, U, theta, cycle, roots=None):
             # near convergence the useful (out-of-space) part of a correction
             # is small next to the network's BF16 output noise: the
             # Jacobi-Davidson correction takes over below learned_until * tol,
@@ -896,7 +904,10 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
             if state['fallback']:
                 return jd
             if learned_gate == 'root' and learned_until is not None:
-                big = torch.nonzero(R.norm(dim=0) >= learned_until * conv_tol).squeeze(1)
+                sel = R.norm(dim=0) >= learned_until * conv_tol
+                if learned_roots == 'requested' and roots is not None:
+                    sel &= torch.as_tensor(np.asarray(roots) < nroots, device=sel.device)
+                big = torch.nonzero(sel).squeeze(1)
                 if big.numel() == 0:
                     return jd
                 if big.numel() < R.shape[1]:
@@ -950,7 +961,8 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                    max_space=max_space, precond=precond,
                    stall_cycles=stall_cycles if preconditioner == 'learned' else None,
                    on_stall=on_stall, phases=phases, keep=keep, early=early,
-                   early_roots=nroots if track > nroots else 0)
+                   early_roots=nroots if track > nroots else 0,
+                   precond_roots=precond is not None)
     cycles, vectors = res.cycles, res.vectors
     e, X, conv, rn = res.e, res.X, res.conv, res.rnorm
     exhausted = res.exhausted
