@@ -124,25 +124,54 @@ __global__ void k_pack_bf16_xo(int M, int no, const float *__restrict__ Xo,
   }
 }
 
+// ---- W in 32 x 64 tiles ---------------------------------------------------------------
+// wtile[(c * nvb + vb) * 512 + 256 tt + 64 w + 16 q + r] holds, as one 16-byte
+// slot, W[32 c + 16 tt + 4 q + 0..3][64 vb + 16 w + r]: the four W values lane
+// 16 q + r of wave w multiplies its P tile tt by (P's result rows 4 q + 0..3),
+// zero outside the M x M matrix.  A tile is 8 KB in the order the exchange
+// reads it from LDS, so it streams in with the X_o chunk's own loads.
+__global__ void k_pack_w_tiles(int M, const float *__restrict__ W, f32x4 *__restrict__ wt) {
+  const int nvb = (M + 63) / 64;
+  const long n = (long)((M + 31) / 32) * nvb * 512;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
+       e += (long)gridDim.x * blockDim.x) {
+    const long tile = e / 512;
+    const int slot = (int)(e % 512);
+    const int c = (int)(tile / nvb), vb = (int)(tile % nvb);
+    const int tt = slot >> 8, w = (slot >> 6) & 3, q = (slot >> 4) & 3, r = slot & 15;
+    const int row0 = 32 * c + 16 * tt + 4 * q, col = 64 * vb + 16 * w + r;
+    f32x4 v;
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr)
+      v[rr] = (row0 + rr < M && col < M) ? W[(long)(row0 + rr) * M + col] : 0.f;
+    wt[e] = v;
+  }
+}
+
 // ---- the exchange ----------------------------------------------------------------
 typedef __attribute__((address_space(3))) void lds_void;
 // Workgroup = 4 waves; a wave computes NV vectors x NC 16-column tiles (NV = 2,
 // NC = 1 shares every W value and every X_o fragment between two vectors; NV =
 // 1, NC = 2 serves a single vector).  The packed X_o chunks stream through a
 // double-buffered LDS stage with direct global-to-LDS 16-byte loads
-// (global_load_lds_dwordx4: no staging registers, two waves fit per SIMD), and
-// the next chunk's W values are loaded into registers while the current chunk
-// computes.  W is read at clamped indices: a row >= M meets a zero row of the
-// packed X_o (P = 0 there) and a column >= M a zero column of zXv.
-template <int NP, int NIE, int TAIL, int NV, int NC>
+// (global_load_lds_dwordx4: no staging registers, two waves fit per SIMD).
+// WT: the chunk's W tile (k_pack_w_tiles) streams into the same stage and each
+// lane reads its eight W values from LDS, so the only wait on global memory is
+// the one at the chunk's end.  Without WT (NC = 2) the next chunk's W values
+// are loaded into registers while the current chunk computes, at clamped
+// indices: a row >= M meets a zero row of the packed X_o (P = 0 there) and a
+// column >= M a zero column of zXv.
+template <int NP, int NIE, int TAIL, int NV, int NC, bool WT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_exchange_bf16(
     int M, int no, int k, const bf16x8 *__restrict__ frag, const float *__restrict__ W,
-    const float *__restrict__ zXv, float *__restrict__ Tpart, int nsplit,
-    int rows_per_split, int nvb, int nblk) {
+    const f32x4 *__restrict__ wtile, const float *__restrict__ zXv, float *__restrict__ Tpart,
+    int nsplit, int rows_per_split, int nvb, int nblk) {
   constexpr int NKB = NP / 32, NFB = TAIL == 3 ? NKB : NKB - 1, NT = NV * NC;
   constexpr int TP = TAIL == 3 ? 2 : TAIL, FP = 2 * (NKB - 1) + TP;
   constexpr int SPP = 128 * FP, SPC = SPP + 128 * NIE, COLS = 64 * NC;
-  __shared__ bf16x8 buf[2][SPC];
+  static_assert(!WT || NC == 1, "a W tile holds 64 columns");
+  constexpr int SPW = WT ? 512 : 0;   // W tile slots per stage
+  __shared__ bf16x8 buf[2][SPC + SPW];
   const int kp = (k + NV - 1) / NV;
   const int lid = xcd_logical(blockIdx.x, nblk);
   if (lid >= nvb * kp * nsplit) return;
@@ -230,10 +259,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
       if (SPC % 256 == 0 || 256 * q + 64 * wave < SPC)
         __builtin_amdgcn_global_load_lds((const void *)(src + 256 * q + tid),
                                          (lds_void *)&buf[n][256 * q + 64 * wave], 16, 0, 0);
+    if (WT) {
+      const f32x4 *wsrc = wtile + ((long)c * nvb + vb) * 512;
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+        __builtin_amdgcn_global_load_lds((const void *)(wsrc + 256 * q + tid),
+                                         (lds_void *)&buf[n][SPC + 256 * q + 64 * wave], 16, 0,
+                                         0);
+    }
+  };
+  // this lane's W values from stage n's tile: rows 16 tt + 4 lq + rr, column colv[0]
+  auto lds_w = [&](int n, float (&w)[NC][2][4]) {
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt) {
+      const f32x4 v = *(const f32x4 *)&buf[n][SPC + 256 * tt + 64 * wave + lane];
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) w[0][tt][rr] = v[rr];
+    }
   };
   if (c_beg < c_end) fetch(c_beg, 0);
   float w[NC][2][4];
-  load_w(c_beg, w);
+  if (!WT) load_w(c_beg, w);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   int cur = 0;
@@ -242,9 +288,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     float wn[NC][2][4];
     if (more) {
       fetch(c + 1, cur ^ 1);   // buf[cur ^ 1] was last read before the previous barrier
-      load_w(c + 1, wn);
+      if (!WT) load_w(c + 1, wn);
     }
     const bf16x8 *fb = buf[cur];
+    // WT: all of the P products' fragments (and the W values) are read up
+    // front, so the chunk waits on LDS once before its P products rather than
+    // once per K block; the scheduling barrier keeps the compiler from sinking
+    // the reads back to their uses.  (Without WT the one-vector kernel has no
+    // registers to spare and reads each fragment where it is used.)
+    bf16x8 fa[2][FP];
+    auto pf = [&](int tt, int f) { return WT ? fa[tt][f] : fb[(tt * FP + f) * 64 + lane]; };
+    if constexpr (WT) {
+#pragma unroll
+      for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int f = 0; f < FP; ++f) fa[tt][f] = fb[(tt * FP + f) * 64 + lane];
+      lds_w(cur, w);
+      __builtin_amdgcn_sched_barrier(0);
+    }
     f32x4 P[2][NT];
 #pragma unroll
     for (int tt = 0; tt < 2; ++tt)
@@ -253,20 +314,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int tt = 0; tt < 2; ++tt) {
 #pragma unroll
-      for (int kb = 0; kb < NFB; ++kb) {
-        const bf16x8 ah = fb[(tt * FP + 2 * kb + 0) * 64 + lane];
-        const bf16x8 al = fb[(tt * FP + 2 * kb + 1) * 64 + lane];
+      for (int kb = 0; kb < NFB; ++kb)
 #pragma unroll
-        for (int u = 0; u < NT; ++u) P[tt][u] = mma3(ah, al, bzh[u][kb], bzl[u][kb], P[tt][u]);
-      }
+        for (int u = 0; u < NT; ++u)
+          P[tt][u] = mma3(pf(tt, 2 * kb), pf(tt, 2 * kb + 1), bzh[u][kb], bzl[u][kb], P[tt][u]);
       if (TAIL < 3) {
 #pragma unroll
-        for (int tf = 0; tf < (TAIL == 3 ? 1 : TAIL); ++tf) {
-          const bf16x8 a = fb[(tt * FP + 2 * (NKB - 1) + tf) * 64 + lane];
+        for (int tf = 0; tf < (TAIL == 3 ? 1 : TAIL); ++tf)
 #pragma unroll
-          for (int u = 0; u < NT; ++u) P[tt][u] = mma_bf16(a, bzt[u][tf], P[tt][u]);
-        }
+          for (int u = 0; u < NT; ++u)
+            P[tt][u] = mma_bf16(pf(tt, 2 * (NKB - 1) + tf), bzt[u][tf], P[tt][u]);
       }
+    }
+    // WT: the T products' fragments are in flight while S is formed and split
+    bf16x8 ft[NIE][2];
+    auto tf2 = [&](int it, int hl) { return WT ? ft[it][hl] : fb[SPP + (it * 2 + hl) * 64 + lane]; };
+    if constexpr (WT) {
+#pragma unroll
+      for (int it = 0; it < NIE; ++it) {
+        ft[it][0] = fb[SPP + (it * 2 + 0) * 64 + lane];
+        ft[it][1] = fb[SPP + (it * 2 + 1) * 64 + lane];
+      }
+      __builtin_amdgcn_sched_barrier(0);
     }
     bf16x8 sh[NT], sl[NT];
 #pragma unroll
@@ -281,13 +350,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         split8(sv, sh[v * NC + ct], sl[v * NC + ct]);
       }
 #pragma unroll
-    for (int it = 0; it < NIE; ++it) {
-      const bf16x8 ah = fb[SPP + (it * 2 + 0) * 64 + lane];
-      const bf16x8 al = fb[SPP + (it * 2 + 1) * 64 + lane];
+    for (int it = 0; it < NIE; ++it)
 #pragma unroll
-      for (int u = 0; u < NT; ++u) acc[it][u] = mma3(ah, al, sh[u], sl[u], acc[it][u]);
-    }
-    if (more) {
+      for (int u = 0; u < NT; ++u) acc[it][u] = mma3(tf2(it, 0), tf2(it, 1), sh[u], sl[u], acc[it][u]);
+    if (!WT && more) {
 #pragma unroll
       for (int ct = 0; ct < NC; ++ct)
 #pragma unroll

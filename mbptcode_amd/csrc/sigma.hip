@@ -682,6 +682,7 @@ struct mbpt_sigma {
   void *Xo, *Xv, *XoT, *W, *V;
   void *Xp;                // X_o zero-padded to (M rounded to 32) x nopad (FP32 / FP64)
   bf16x8 *frag;            // X_o split and packed in MFMA fragment order (MBPT_BF16X3)
+  f32x4 *wtile;            // W in the exchange's 32 x 64 LDS tiles (MBPT_BF16X3)
   // FP32 working precision: K-major, zero-padded copies for the head and tail
   // GEMMs, X_v^T (nvp x Mq) and X_v (Mp x nvq); p pads to 16, q pads to 128
   float *XvT, *XvP;
@@ -845,13 +846,15 @@ void launch_exchange_b(const mbpt_sigma *h, int k, int nsplit, int rps, T *Upart
                                                Upart, nsplit, rps, nmb, nblk);
 }
 
+// two vectors per wave: W streams in its packed tiles; one vector, two column
+// tiles: W from global memory
 template <int NP, int NIE, int TAIL, int NV>
 void launch_exchange_bf16_nv(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
   constexpr int NC = NV == 2 ? 1 : 2;
   const int nvb = (h->M + 64 * NC - 1) / (64 * NC), kp = (k + NV - 1) / NV;
   const int nblk = pad_grid((long)nvb * kp * nsplit);
-  k_exchange_bf16<NP, NIE, TAIL, NV, NC><<<nblk, 256, 0, st>>>(
-      h->M, h->no, k, h->frag, (const float *)h->W, (const float *)h->ws[WS_ZXV],
+  k_exchange_bf16<NP, NIE, TAIL, NV, NC, NV == 2><<<nblk, 256, 0, st>>>(
+      h->M, h->no, k, h->frag, (const float *)h->W, h->wtile, (const float *)h->ws[WS_ZXV],
       (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
 }
 
@@ -1234,6 +1237,15 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
     h->held += sizeof(bf16x8) * slots;
     k_pack_bf16_xo<<<grid1(slots), 256, 0, st>>>(M, nocc, (const float *)h->Xo, h->frag);
     if ((rc = launch_status("k_pack_bf16_xo"))) return fail(rc);
+    const long wslots = (long)((M + 31) / 32) * ((M + 63) / 64) * 512;
+    he = hipMalloc((void **)&h->wtile, sizeof(f32x4) * wslots);
+    if (he != hipSuccess) {
+      set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
+      return fail(MBPT_ERR_HIP);
+    }
+    h->held += sizeof(f32x4) * wslots;
+    k_pack_w_tiles<<<grid1(wslots), 256, 0, st>>>(M, (const float *)h->W, h->wtile);
+    if ((rc = launch_status("k_pack_w_tiles"))) return fail(rc);
   }
   *out = h;
   return MBPT_OK;
@@ -1318,7 +1330,7 @@ int mbpt_sigma_destroy(mbpt_sigma *h) {
     for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
   free_ws(h);
   (void)hipFree(h->Xo); (void)hipFree(h->Xp); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
-  (void)hipFree(h->frag); (void)hipFree(h->XvT); (void)hipFree(h->XvP);
+  (void)hipFree(h->frag); (void)hipFree(h->wtile); (void)hipFree(h->XvT); (void)hipFree(h->XvP);
   for (int hl = 0; hl < 2; ++hl) { (void)hipFree(h->XvB[hl]); (void)hipFree(h->XvTB[hl]); }
   (void)hipFree(h->d);
   delete h;
