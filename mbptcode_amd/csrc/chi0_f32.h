@@ -184,34 +184,60 @@ __global__ __launch_bounds__(256) void k_chi0_reduce(int M, int Mp, int nchunk, 
                                                      int accumulate) {
   // block pair p -> 64-row blocks (bi, bj), bi <= bj: the partials hold every
   // such block (it lies on or above the 128-tile diagonal); the sum is
-  // written there and, mirrored through LDS, at (bj, bi)
+  // written there and, mirrored through LDS, at (bj, bi).  A thread sums
+  // four consecutive columns (16-byte loads, Mp a multiple of 128), the
+  // chunk loop unrolled so that several planes' loads are in flight, and
+  // stores pairs of doubles where M is even and out 16-byte aligned.
   const int p = blockIdx.x;
   int bi = 0, base = 0;
   while (p >= base + (nto - bi)) { base += nto - bi; ++bi; }
   const int bj = bi + (p - base);
   const int sr0 = bi * 64, sc0 = bj * 64;
+  const long plane = (long)Mp * Mp;
   __shared__ float t[64][65];
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    const long src = (long)(sr0 + r) * Mp + sc0 + c;
-    float s = 0.f;
-    for (int q = 0; q < nchunk; ++q) s += part[(long)q * Mp * Mp + src];
-    t[r][c] = s;
+  for (int e = threadIdx.x; e < 64 * 16; e += 256) {
+    const int r = e >> 4, c = 4 * (e & 15);
+    const float *src = part + (long)(sr0 + r) * Mp + sc0 + c;
+    f32x4 s = {0.f, 0.f, 0.f, 0.f};
+    int q = 0;
+    for (; q + 4 <= nchunk; q += 4) {
+      const f32x4 a0 = *(const f32x4 *)(src + q * plane);
+      const f32x4 a1 = *(const f32x4 *)(src + (q + 1) * plane);
+      const f32x4 a2 = *(const f32x4 *)(src + (q + 2) * plane);
+      const f32x4 a3 = *(const f32x4 *)(src + (q + 3) * plane);
+      s += a0;
+      s += a1;
+      s += a2;
+      s += a3;
+    }
+    for (; q < nchunk; ++q) s += *(const f32x4 *)(src + q * plane);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[r][c + u] = s[u];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
-    const int r = e >> 6, c = e & 63;
-    int row = sr0 + r, col = sc0 + c;
-    if (row < M && col < M) {
-      const long o = (long)row * M + col;
-      out[o] = accumulate ? out[o] + (double)t[r][c] : (double)t[r][c];
-    }
-    if (bi == bj) continue;
-    row = sc0 + r;
-    col = sr0 + c;
-    if (row < M && col < M) {
-      const long o = (long)row * M + col;
-      out[o] = accumulate ? out[o] + (double)t[c][r] : (double)t[c][r];
+  const bool pairs = (M & 1) == 0 && ((unsigned long)out & 15) == 0;
+  for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+    const int r = e >> 5, c = 2 * (e & 31);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      if (side == 1 && bi == bj) break;
+      const int row = side ? sc0 + r : sr0 + r, col = side ? sr0 + c : sc0 + c;
+      const double v0 = side ? (double)t[c][r] : (double)t[r][c];
+      const double v1 = side ? (double)t[c + 1][r] : (double)t[r][c + 1];
+      if (row >= M) continue;
+      double *o = out + (long)row * M + col;
+      if (pairs && col + 1 < M) {
+        double2 v = {v0, v1};
+        if (accumulate) {
+          const double2 w = *(const double2 *)o;
+          v.x += w.x;
+          v.y += w.y;
+        }
+        *(double2 *)o = v;
+      } else {
+        if (col < M) o[0] = accumulate ? o[0] + v0 : v0;
+        if (col + 1 < M) o[1] = accumulate ? o[1] + v1 : v1;
+      }
     }
   }
 }
