@@ -191,39 +191,87 @@ __global__ __launch_bounds__(256) void k_exchange(
 #pragma unroll
     for (int r = 0; r < 4; ++r) w[r] = W[(long)min(mu0 + MF<T>::orow(lq, r), M - 1) * M + colc];
   };
-  if (st_beg < st_end) {
-    fetch(st_beg, 0);
-    T w[4];
-    load_w(ROWS * st_beg, w);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    int cur = 0;
-    for (int st = st_beg; st < st_end; ++st) {
-      if (st + 1 < st_end) fetch(st + 1, cur ^ 1);   // xs[cur ^ 1] last read before the barrier
-      const T *x = xs[cur];
+  // The W values of the 32 rows of stage st + 1 are loaded at the top of
+  // stage st and moved into place after the stage's closing wait, so no wait
+  // on them falls inside the stage (FP32: 3-4% at n_o = 100, M = 6000).  In
+  // FP64 the 16 extra VGPRs cost occupancy, and W is loaded 16 rows ahead.
+  if constexpr (sizeof(T) == 4) {
+    if (st_beg < st_end) {
+      fetch(st_beg, 0);
+      T w[2][4], wn[2][4];
+      load_w(ROWS * st_beg, w[0]);
+      load_w(ROWS * st_beg + 16, w[1]);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      int cur = 0;
+      for (int st = st_beg; st < st_end; ++st) {
+        const int sn = min(st + 1, st_end - 1);
+        fetch(sn, cur ^ 1);   // the last stage reloads itself into the idle buffer
+        load_w(ROWS * sn, wn[0]);
+        load_w(ROWS * sn + 16, wn[1]);
+        const T *x = xs[cur];
 #pragma unroll
-      for (int ch = 0; ch < ROWS / 16; ++ch) {
-        const int r0 = ch * 16;
-        T wn[4];
-        load_w(ROWS * st + r0 + 16, wn);
-        typename MF<T>::acc P;
-        zero(P);
+        for (int ch = 0; ch < ROWS / 16; ++ch) {
+          const int r0 = ch * 16;
+          typename MF<T>::acc P;
+          zero(P);
 #pragma unroll
-        for (int q = 0; q < NS; ++q) P = MF<T>::mma(x[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
-        T S[4];
+          for (int q = 0; q < NS; ++q) P = MF<T>::mma(x[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
+          T S[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) S[r] = P[r] * w[r];
+          for (int r = 0; r < 4; ++r) S[r] = P[r] * w[ch][r];
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+          for (int r = 0; r < 4; ++r)
 #pragma unroll
-          for (int it = 0; it < NI; ++it)
-            acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r], acc[it]);
+            for (int it = 0; it < NI; ++it)
+              acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r],
+                                   acc[it]);
+        }
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's stage and W loads landed
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r) w[r] = wn[r];
+        for (int ch = 0; ch < 2; ++ch)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) w[ch][r] = wn[ch][r];
+        __syncthreads();                 // ... and every other wave's
+        cur ^= 1;
       }
-      __builtin_amdgcn_s_waitcnt(0);   // this wave's stage loads (and W loads) landed
-      __syncthreads();                 // ... and every other wave's
-      cur ^= 1;
+    }
+  } else {
+    if (st_beg < st_end) {
+      fetch(st_beg, 0);
+      T w[4];
+      load_w(ROWS * st_beg, w);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      int cur = 0;
+      for (int st = st_beg; st < st_end; ++st) {
+        if (st + 1 < st_end) fetch(st + 1, cur ^ 1);   // xs[cur ^ 1] last read before the barrier
+        const T *x = xs[cur];
+  #pragma unroll
+        for (int ch = 0; ch < ROWS / 16; ++ch) {
+          const int r0 = ch * 16;
+          T wn[4];
+          load_w(ROWS * st + r0 + 16, wn);
+          typename MF<T>::acc P;
+          zero(P);
+  #pragma unroll
+          for (int q = 0; q < NS; ++q) P = MF<T>::mma(x[(r0 + lr) * LD + 4 * q + lq], bz[q], P);
+          T S[4];
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) S[r] = P[r] * w[r];
+  #pragma unroll
+          for (int r = 0; r < 4; ++r)
+  #pragma unroll
+            for (int it = 0; it < NI; ++it)
+              acc[it] = MF<T>::mma(x[(r0 + MF<T>::orow(lq, r)) * LD + 16 * it + lr], S[r], acc[it]);
+  #pragma unroll
+          for (int r = 0; r < 4; ++r) w[r] = wn[r];
+        }
+        __builtin_amdgcn_s_waitcnt(0);   // this wave's stage loads (and W loads) landed
+        __syncthreads();                 // ... and every other wave's
+        cur ^= 1;
+      }
     }
   }
   if (colok) {
