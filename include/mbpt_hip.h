@@ -94,12 +94,13 @@ size_t mbpt_sigma_workspace_bytes(const mbpt_sigma *h, int k, int with_b);
 /* Per-stage device timing, the counterpart of the action's `piece_clock`
  * (block_actions.py:56-70).  enable != 0 starts (and zeroes) accumulation:
  * every later apply records HIP events around its four launches on its
- * stream and waits for the last one.  ms_out[4] receives the accumulated
+ * stream without waiting for them; mbpt_sigma_profile_read waits for the
+ * last timed apply and adds up every one since.  ms_out[4] receives the accumulated
  * milliseconds of head, hartree, exchange and tail; launches the number of
  * applies timed; exchange_flops the algorithmic flops of the exchange
  * launches, sum of k (4 nocc M^2 + M^2). */
 int mbpt_sigma_profile(mbpt_sigma *h, int enable);
-int mbpt_sigma_profile_read(const mbpt_sigma *h, double *ms_out, long *launches,
+int mbpt_sigma_profile_read(mbpt_sigma *h, double *ms_out, long *launches,
                             double *exchange_flops);
 
 int mbpt_sigma_destroy(mbpt_sigma *h);

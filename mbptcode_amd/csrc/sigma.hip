@@ -745,7 +745,10 @@ struct mbpt_sigma {
   size_t ws_cap[WS_N];     // bytes
   size_t held;
   int prof;                // stage timing on
-  hipEvent_t ev[5];
+  // timing events, five per timed apply, read back (and reused) only when
+  // the figures are asked for, so a timed apply does not wait for the device
+  std::vector<hipEvent_t> ev;
+  std::vector<double> ev_flops;   // exchange flops of each pending apply
   double prof_ms[4], prof_flops;
   long prof_launches;
 };
@@ -756,6 +759,45 @@ constexpr int kMaxNocc = 128;
 constexpr int kTargetBlocks = 2048;
 
 int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
+
+// Timed applies waiting to be read: at most this many, then the oldest
+// figures are collected (which waits for them) before more are recorded.
+constexpr size_t kMaxPendingApplies = 512;
+
+// Adds the stage times of every pending timed apply to the totals (waiting
+// for each one's closing event: the applies may have run on several
+// streams) and frees their events for reuse.
+int prof_collect(mbpt_sigma *h) {
+  const size_t n = h->ev_flops.size();
+  for (size_t a = 0; a < n; ++a) {
+    MBPT_HIP_TRY(hipEventSynchronize(h->ev[5 * a + 4]));
+    for (int q = 0; q < 4; ++q) {
+      float ms = 0.f;
+      MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->ev[5 * a + q], h->ev[5 * a + q + 1]));
+      h->prof_ms[q] += ms;
+    }
+    h->prof_flops += h->ev_flops[a];
+    h->prof_launches += 1;
+  }
+  h->ev_flops.clear();
+  return MBPT_OK;
+}
+
+// The five events of the next timed apply (created as the pool grows).
+int prof_events(mbpt_sigma *h, hipEvent_t **out) {
+  if (h->ev_flops.size() >= kMaxPendingApplies) {
+    const int rc = prof_collect(h);
+    if (rc) return rc;
+  }
+  const size_t need = 5 * (h->ev_flops.size() + 1);
+  while (h->ev.size() < need) {
+    hipEvent_t e;
+    MBPT_HIP_TRY(hipEventCreate(&e));
+    h->ev.push_back(e);
+  }
+  *out = &h->ev[need - 5];
+  return MBPT_OK;
+}
 
 void free_ws(mbpt_sigma *h) {
   for (int q = 0; q < WS_N; ++q) {
@@ -980,7 +1022,11 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hi
   // rows per split, a multiple of the 64-row LDS stage
   const int rps = (((M + nsplit - 1) / nsplit) + 63) / 64 * 64;
   const int nsplit_eff = (M + rps - 1) / rps;
-  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[0], st));
+  hipEvent_t *ev = nullptr;   // this apply's five timing events
+  if (h->prof) {
+    if ((rc = prof_events(h, &ev))) return rc;
+    MBPT_HIP_TRY(hipEventRecord(ev[0], st));
+  }
 
   if (b3) {  // head, split BF16: z as hi / lo rows, times X_v^T
     __bf16 *Zh = (__bf16 *)h->ws[WS_ZT], *Zl = Zh + (size_t)ldr * h->nvp32;
@@ -1011,7 +1057,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hi
                                     nblk);
     if ((rc = launch_status("k_head"))) return rc;
   }
-  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[1], st));
+  if (ev) MBPT_HIP_TRY(hipEventRecord(ev[1], st));
   if (h->kappa != 0.0) {  // hartree
     k_hartree_p<T><<<grid1((long)k * M), 256, 0, st>>>(M, no, k, (const T *)h->XoT,
                                                         zXv, p);
@@ -1027,7 +1073,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hi
       if ((rc = launch_status("k_sum_partials"))) return rc;
     }
   }
-  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[2], st));
+  if (ev) MBPT_HIP_TRY(hipEventRecord(ev[2], st));
   if (h->prec == MBPT_BF16X3) {  // exchange, split BF16
     switch (h->nopad) {
       case 32: rc = launch_exchange_bf16<32>(h, k, nsplit_eff, rps, st); break;
@@ -1073,7 +1119,7 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hi
     }
     if ((rc = launch_status("k_exchange_b"))) return rc;
   }
-  if (h->prof) MBPT_HIP_TRY(hipEventRecord(h->ev[3], st));
+  if (ev) MBPT_HIP_TRY(hipEventRecord(ev[3], st));
   // tail: out = dd o z - (sum_s partials_s - kappa X_o^T o u) X_v (dd NULL: no
   // diagonal term, the B block)
   auto tail = [&](const T *src, const double *dd, double *out) -> int {
@@ -1120,16 +1166,9 @@ int apply_impl(mbpt_sigma *h, int k, const double *z, double *Az, double *Bz, hi
     return MBPT_OK;
   };
   if ((rc = tail(Tpart, h->d, Az))) return rc;
-  if (h->prof) {
-    MBPT_HIP_TRY(hipEventRecord(h->ev[4], st));
-    MBPT_HIP_TRY(hipEventSynchronize(h->ev[4]));
-    for (int q = 0; q < 4; ++q) {
-      float ms = 0.f;
-      MBPT_HIP_TRY(hipEventElapsedTime(&ms, h->ev[q], h->ev[q + 1]));
-      h->prof_ms[q] += ms;
-    }
-    h->prof_flops += (double)k * (4.0 * no * (double)M * M + (double)M * M);
-    h->prof_launches += 1;
+  if (ev) {
+    MBPT_HIP_TRY(hipEventRecord(ev[4], st));
+    h->ev_flops.push_back((double)k * (4.0 * no * (double)M * M + (double)M * M));
   }
   if (Bz && (rc = tail(Upart, nullptr, Bz))) return rc;
   return MBPT_OK;
@@ -1349,22 +1388,19 @@ size_t mbpt_sigma_held_bytes(const mbpt_sigma *h) {
 
 int mbpt_sigma_profile(mbpt_sigma *h, int enable) {
   MBPT_REQUIRE(h != nullptr, "mbpt_sigma_profile: NULL handle");
-  if (enable && !h->prof) {
-    for (int q = 0; q < 5; ++q) MBPT_HIP_TRY(hipEventCreate(&h->ev[q]));
-  }
-  if (!enable && h->prof) {
-    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
-  }
   h->prof = enable ? 1 : 0;
+  h->ev_flops.clear();          // pending applies are dropped with the figures
   for (int q = 0; q < 4; ++q) h->prof_ms[q] = 0.0;
   h->prof_flops = 0.0;
   h->prof_launches = 0;
   return MBPT_OK;
 }
 
-int mbpt_sigma_profile_read(const mbpt_sigma *h, double *ms_out, long *launches,
+int mbpt_sigma_profile_read(mbpt_sigma *h, double *ms_out, long *launches,
                             double *exchange_flops) {
   MBPT_REQUIRE(h != nullptr, "mbpt_sigma_profile_read: NULL handle");
+  int rc = prof_collect(h);
+  if (rc) return rc;
   if (ms_out)
     for (int q = 0; q < 4; ++q) ms_out[q] = h->prof_ms[q];
   if (launches) *launches = h->prof_launches;
@@ -1374,8 +1410,7 @@ int mbpt_sigma_profile_read(const mbpt_sigma *h, double *ms_out, long *launches,
 
 int mbpt_sigma_destroy(mbpt_sigma *h) {
   if (!h) return MBPT_OK;
-  if (h->prof)
-    for (int q = 0; q < 5; ++q) (void)hipEventDestroy(h->ev[q]);
+  for (hipEvent_t e : h->ev) (void)hipEventDestroy(e);
   free_ws(h);
   (void)hipFree(h->Xo); (void)hipFree(h->Xp); (void)hipFree(h->Xv); (void)hipFree(h->XoT); (void)hipFree(h->W); (void)hipFree(h->V);
   (void)hipFree(h->frag); (void)hipFree(h->wtile); (void)hipFree(h->XvT); (void)hipFree(h->XvP);

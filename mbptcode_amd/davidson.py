@@ -42,6 +42,19 @@ from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
 
 
 _BLAS = {}
+_SIDE = {}
+
+
+def _side_stream(dev):
+    """The second stream of a device for the early FP64 check, made once and
+    kept: the caching allocator holds freed blocks per stream, so a stream
+    made anew for every solve would meet its first allocations (the check's
+    operands, tens of MB) with fresh device allocations each time."""
+    key = dev.index if dev.index is not None else torch.cuda.current_device()
+    s = _SIDE.get(key)
+    if s is None:
+        s = _SIDE[key] = torch.cuda.Stream(device=dev)
+    return s
 
 
 def _host_lapack():
@@ -934,7 +947,7 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
     if refine_apply is not None:
         op64 = _operator_of(refine_apply)
         mm64 = matmat_of(op64)
-        side = torch.cuda.Stream(device=dev) if dev.type == 'cuda' else None
+        side = _side_stream(dev) if dev.type == 'cuda' else None
 
         def fp64_check(Xr):
             """(theta, |A x - theta x|) of the columns of Xr with FP64 sigma."""

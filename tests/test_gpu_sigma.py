@@ -216,3 +216,41 @@ def test_bf16x3_golden_and_screened_diagonal():
     sd = op.screened_diagonal_device().cpu().numpy()
     d = O.pair_diagonal(inst.eps, inst.nocc)
     assert np.abs(sd - gold['sdiag']).max() < 1e-6 * np.abs(gold['sdiag'] - d).max() + 1e-12
+
+
+@pytest.mark.parametrize('precision', ['fp64', 'bf16x3'])
+def test_stage_timing_reads_every_timed_apply(precision):
+    """Stage timing records events without waiting: applies queued back to
+    back are all counted when read, a pause keeps what was timed and leaves
+    out what runs meanwhile, and the results match untimed applies."""
+    inst = synth.config(1)
+    print(f'sigma stage timing cfg1 {precision}', flush=True)
+    W, _, _ = grid_inputs(inst)
+    op = device_action(inst, W, precision)
+    rng = np.random.default_rng(5)
+    zs = [torch.as_tensor(rng.normal(size=(k, inst.nocc, inst.nvir)), device='cuda')
+          for k in (2, 3, 1, 4)]
+    ref = [op.apply(z) for z in zs]
+    op.profile(True)
+    got = [op.apply(z) for z in zs[:3]]
+    op.profile_pause()
+    op.apply(zs[3])
+    op.profile_resume()
+    got.append(op.apply(zs[3]))
+    pr = op.profile_read()
+    op.profile(False)
+    torch.cuda.synchronize()
+    for g, r in zip(got, ref):
+        assert float((g - r).abs().max() / r.abs().max()) < 1e-12
+    assert pr['launches'] == 4
+    flops = sum(k * (4.0 * inst.nocc * inst.M ** 2 + inst.M ** 2) for k in (2, 3, 1, 4))
+    assert pr['exchange_flops'] == pytest.approx(flops, rel=1e-12)
+    assert all(v > 0 for v in pr['ms'].values())
+    # many timed applies queued at once: the pending events are collected
+    # as they fill up and nothing is lost
+    op.profile(True)
+    for _ in range(600):
+        op.apply(zs[2])
+    pr = op.profile_read()
+    op.profile(False)
+    assert pr['launches'] == 600
