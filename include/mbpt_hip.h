@@ -237,6 +237,13 @@ int mbpt_gemm_tn_f64_noalloc(int n, int a, int b, const double *A, int lda,
  *     (k, c) row-major with ldy, c <= 64 (beta 0: C is not read).
  * Ritz: X = V Y and R = AV Y - X diag(theta), V and AV the m columns at
  *     voff / avoff, Y (m, k), k <= 64: the Ritz vectors and their residuals.
+ * Block scale: before the operator is applied to a correction block T (n,
+ *     p <= 64, row stride ldt) projected by the two Gram-Schmidt passes of
+ *     the orthonormalisation, pr ((3m + 2p + 2) x p, row-major) holding
+ *     S1 = V^T T, G0 = T^T T, AV^T T, S2 = V^T T1, G1 = T1^T T1 in its rows:
+ *     column j's norm a_j = sqrt(G1_jj - |S2_:j|^2) goes to row 3m + 2p,
+ *     its keep flag (a_j > drop |T_:j| before) to the last row, and T's
+ *     columns are scaled to unit norm (kept) or zeroed (dropped).
  * JD: T = K^-1 R - alpha K^-1 U per column, K = diag - theta floored at
  *     `floor` in magnitude, alpha = U.K^-1 R / U.K^-1 U; work:
  *     mbpt_jd_correction_f64_work(n, k) doubles.
@@ -253,11 +260,32 @@ int mbpt_ritz_residual_f64(int n, int m, int k, const double *V, int ldv, int vo
                            const double *AV, int ldav, int avoff, const double *Y,
                            int ldy, const double *theta, double *X, int ldx,
                            double *R, int ldr, void *stream);
+int mbpt_block_scale_f64(int n, int m, int p, double drop, double *pr, double *T,
+                         int ldt, void *stream);
 long mbpt_jd_correction_f64_work(int n, int k);
 int mbpt_jd_correction_f64(int n, int k, const double *U, int ldu, const double *R,
                            int ldr, const double *diag, const double *theta,
                            double floor, double *T, int ldt, double *work,
                            long work_elems, void *stream);
+
+/* ------------------------------------------------------------------------
+ * Device Rayleigh-Ritz step of the single-instance Davidson (FP64): the m x m
+ * eigh of pyscf's lib.davidson1 (called at src/Solvers/davidson.py:277) for
+ * the lowest k Ritz pairs, kept on the device.
+ *
+ * H (m x m, row stride ldh) is the Rayleigh matrix V^T A V, G (row stride
+ * ldg) = (AV)^T AV.  The symmetric part of H is reduced to tridiagonal form
+ * (Householder), its lowest k eigenvalues found by Sturm-count bisection and
+ * their eigenvectors by inverse iteration (orthogonalised within clusters
+ * closer than 1e-6 ||T||) and taken back: Y (m x k, row stride ldy) holds
+ * them as columns, wq (3k) the eigenvalues ascending, then y^T G y, then
+ * y^T H y per column (H as given).  1 <= m <= mbpt_ritz_max_m() (192),
+ * 1 <= k <= min(m, 64); work: mbpt_ritz_eigh_f64_work(m, k) doubles. */
+int mbpt_ritz_max_m(void);
+long mbpt_ritz_eigh_f64_work(int m, int k);
+int mbpt_ritz_eigh_f64(int m, int k, const double *H, int ldh, const double *G, int ldg,
+                       double *Y, int ldy, double *wq, double *work, long work_elems,
+                       void *stream);
 
 /* ------------------------------------------------------------------------
  * Batched small symmetric eigensolver (FP64): the Rayleigh-Ritz step of many

@@ -29,7 +29,9 @@ EXPORTS = ('mbpt_last_error', 'mbpt_abi_version', 'mbpt_sigma_create',
            'mbpt_chi0_sigma', 'mbpt_chi0_sigma_profile_read', 'mbpt_chi0_destroy', 'mbpt_gemm_tn_f64_work', 'mbpt_gemm_tn_f64',
            'mbpt_gemm_tn_f64_noalloc_max', 'mbpt_gemm_tn_f64_noalloc',
            'mbpt_gemm_tn_seg_f64_work', 'mbpt_gemm_tn_seg_f64', 'mbpt_gemm_nn_seg_f64',
-           'mbpt_ritz_residual_f64', 'mbpt_jd_correction_f64_work', 'mbpt_jd_correction_f64',
+           'mbpt_ritz_residual_f64', 'mbpt_block_scale_f64', 'mbpt_jd_correction_f64_work',
+           'mbpt_jd_correction_f64', 'mbpt_ritz_max_m', 'mbpt_ritz_eigh_f64_work',
+           'mbpt_ritz_eigh_f64',
            'mbpt_mlp_nparams', 'mbpt_mlp_create', 'mbpt_mlp_get_params',
            'mbpt_mlp_forward', 'mbpt_mlp_gradient', 'mbpt_mlp_train_step',
            'mbpt_mlp_correction', 'mbpt_mlp_correction_batched', 'mbpt_mlp_destroy',
@@ -79,6 +81,10 @@ _SIGNATURES = {
                                   _vp]),
     'mbpt_ritz_residual_f64': (_i, [_i, _i, _i, _vp, _i, _i, _vp, _i, _i, _vp, _i, _vp, _vp, _i,
                                     _vp, _i, _vp]),
+    'mbpt_block_scale_f64': (_i, [_i, _i, _i, _d, _vp, _vp, _i, _vp]),
+    'mbpt_ritz_max_m': (_i, []),
+    'mbpt_ritz_eigh_f64_work': (ctypes.c_long, [_i, _i]),
+    'mbpt_ritz_eigh_f64': (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _vp, ctypes.c_long, _vp]),
     'mbpt_jd_correction_f64_work': (ctypes.c_long, [_i, _i]),
     'mbpt_jd_correction_f64': (_i, [_i, _i, _vp, _i, _vp, _i, _vp, _vp, _d, _vp, _i, _vp,
                                     ctypes.c_long, _vp]),

@@ -60,6 +60,12 @@ DAVIDSON_DENOMINATOR_FLOOR = 1e-8       # :2193
 DAVIDSON_COMPLETION_SEED = 0            # :2124 random directions of the completion solve
 DAVIDSON_FOLLOW_OVERLAP_TIE = 1e-6      # :2129 overlaps this close are a tie
 
+#: Smallest Rayleigh matrix whose Ritz pairs the device eigensolver
+#: (ritz.hip) computes in place of host LAPACK; below it the host is faster
+#: (measured on one MI355X box: 0.99 against 1.06 ms at m = 161, 1.12 against
+#: 1.64 ms at m = 192).  Not a reference constant.
+DAVIDSON_DEVICE_RITZ_MIN = 176
+
 #: Largest pair space a Davidson whose trial space ran out is completed in
 #: by a dense solve (constants.py:590).
 BSE_DENSE_MAX_NOV = 12000

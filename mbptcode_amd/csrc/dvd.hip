@@ -180,6 +180,38 @@ __global__ __launch_bounds__(64) void k_nns(int n, int k, int c, Cols2 A, Cols2 
   }
 }
 
+// ---- Unit scaling of a projected correction block before the operator:
+// from the two Gram-Schmidt passes' projections (S1 = V^T T, G0 = T^T T,
+// S2 = V^T T1, G1 = T1^T T1, laid out as the rows of `pr`, (3m + 2p + 2) x p)
+// column j keeps norm a_j = sqrt(max(G1_jj - |S2_:j|^2, 0)) after the
+// projections against b_j = sqrt(max(G0_jj, 0)) before; it is kept when
+// a_j > drop max(b_j, 1e-300).  Rows 3m + 2p and 3m + 2p + 1 of `pr` receive
+// a_j and the 0 / 1 keep flag, and the block's columns are scaled by 1 / a_j
+// (kept) or 0 (dropped).
+__global__ __launch_bounds__(64) void k_block_norms(int m, int p, double drop,
+                                                    double *__restrict__ pr) {
+  const int j = threadIdx.x;
+  if (j >= p) return;
+  const double *S2 = pr + (long)(2 * m + p) * p, *G1 = S2 + (long)m * p;
+  double s2 = 0.0;
+  for (int i = 0; i < m; ++i) s2 += S2[(long)i * p + j] * S2[(long)i * p + j];
+  const double a = sqrt(fmax(G1[(long)j * p + j] - s2, 0.0));
+  const double b = sqrt(fmax(pr[(long)(m + j) * p + j], 0.0));
+  pr[(long)(3 * m + 2 * p) * p + j] = a;
+  pr[(long)(3 * m + 2 * p + 1) * p + j] = a > drop * fmax(b, 1e-300) ? 1.0 : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_block_scale(int n, int m, int p,
+                                                     const double *__restrict__ pr,
+                                                     double *__restrict__ T, int ldt) {
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e >= (long)n * p) return;
+  const int i = (int)(e / p), j = (int)(e % p);
+  const double a = pr[(long)(3 * m + 2 * p) * p + j];
+  const bool keep = pr[(long)(3 * m + 2 * p + 1) * p + j] > 0.5;
+  T[(long)i * ldt + j] *= keep ? 1.0 / fmax(a, 1e-300) : 0.0;
+}
+
 // ---- Jacobi-Davidson correction (Solvers/davidson.py:43-77) for k columns:
 // t = K^-1 r - alpha K^-1 u, K = diag - theta floored at `floor` in magnitude,
 // alpha = u.K^-1 r / u.K^-1 u.  Pass 1: per-block partial sums of u.K^-1 u
@@ -345,6 +377,21 @@ int mbpt_ritz_residual_f64(int n, int m, int k, const double *V, int ldv, int vo
                                             Cols2{AV, ldav, m, avoff, 0}, Y, ldy, theta, X,
                                             ldx, 0, R, ldr, 1.0, 0.0);
   return launch_status("k_nns<ritz>");
+}
+
+int mbpt_block_scale_f64(int n, int m, int p, double drop, double *pr, double *T, int ldt,
+                         void *stream) {
+  MBPT_REQUIRE(n > 0 && m >= 0 && p >= 0 && p <= 64, "mbpt_block_scale_f64: need n > 0, m >= 0, 0 <= p <= 64");
+  if (p == 0) return MBPT_OK;
+  MBPT_REQUIRE(pr && T && ldt >= p, "mbpt_block_scale_f64: NULL argument or ldt < p");
+  MBPT_REQUIRE(drop >= 0.0, "mbpt_block_scale_f64: drop must be >= 0");
+  hipStream_t st = (hipStream_t)stream;
+  k_block_norms<<<1, 64, 0, st>>>(m, p, drop, pr);
+  int rc = launch_status("k_block_norms");
+  if (rc) return rc;
+  const long total = (long)n * p;
+  k_block_scale<<<(int)((total + 255) / 256), 256, 0, st>>>(n, m, p, pr, T, ldt);
+  return launch_status("k_block_scale");
 }
 
 long mbpt_jd_correction_f64_work(int n, int k) {

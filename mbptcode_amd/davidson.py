@@ -30,7 +30,8 @@ import warnings
 import numpy as np
 import torch
 
-from .subspace import Ranges, jd_correction, nn_ranges, ritz_residual, tn, tn_ranges
+from .subspace import (Ranges, block_scale, jd_correction, nn_ranges, ritz_eigh, ritz_max_m,
+                       ritz_residual, tn, tn_ranges)
 from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
                         DAVIDSON_GUESS_DEG_TOL, DAVIDSON_GUESS_FACTOR,
                         DAVIDSON_LINDEP_CEILING, DAVIDSON_LINDEP_FLOOR,
@@ -38,7 +39,7 @@ from .constants import (DAVIDSON_DENOMINATOR_FLOOR, DAVIDSON_FLOOR_EPS_MULTIPLE,
                         DAVIDSON_PRECONDITIONERS, DAVIDSON_TOL_RESIDUAL,
                         DAVIDSON_EXTRA_ROOTS_DIVISOR, DAVIDSON_EXTRA_ROOTS_FLOOR,
                         DAVIDSON_COMPLETION_SEED, DAVIDSON_FOLLOW_OVERLAP_TIE,
-                        DAVIDSON_NULL_DIRECTION, BSE_DENSE_MAX_NOV)
+                        DAVIDSON_NULL_DIRECTION, BSE_DENSE_MAX_NOV, DAVIDSON_DEVICE_RITZ_MIN)
 
 
 _BLAS = {}
@@ -349,7 +350,8 @@ def _jd(U, R, diag, theta):
 def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
              max_cycle=DAVIDSON_MAX_CYCLE, max_space=None, precond=None,
              lindep=None, stall_cycles=None, on_stall=None, observe=None, phases=None,
-             keep=None, pick=None, early=None, early_roots=0, precond_roots=False):
+             keep=None, pick=None, early=None, early_roots=0, precond_roots=False,
+             ritz_min=DAVIDSON_DEVICE_RITZ_MIN):
     """Block Davidson for the `nroots` lowest eigenpairs of a symmetric
     operator, on device tensors.
 
@@ -381,8 +383,12 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     converge instead of the lowest (root following, `overlap_pick`): envs
     holds 'ref_coeff', V^T span (m, r), when the pick carries a `span`
     (n, r), else 'xs', the trial vectors as host arrays.
-    precond_roots: call precond(R, U, theta, cycle, roots) with the tracked
-    roots' indices of the columns (ascending).
+    precond_roots: call precond(R, U, theta, cycle, roots, rnorm) with the
+    tracked roots' indices of the columns (ascending) and their residual
+    norms from the Ritz step (a host array).
+    ritz_min: the Rayleigh matrices from this size on (up to 192, with at
+    most 64 Ritz pairs wanted and no `pick`) are solved on the device
+    (ritz.hip, the lowest pairs only); smaller ones by host LAPACK.
     `early(X, w)` is called once, at the first cycle whose Ritz step finds
     the leading `early_roots` pairs converged while a later tracked pair is
     not, with those Ritz vectors (n, early_roots) on the device and their
@@ -445,15 +451,25 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         hg = HG[:m].cpu().numpy()
         Hns, G = hg[:, :m], hg[:, cap:cap + m]
         mark('rr_transfer')
-        H = 0.5 * (Hns + Hns.T)
         G = 0.5 * (G + G.T)
-        with _host_lapack():
-            evals, evecs = np.linalg.eigh(H)
         k = min(nroots, m)
-        w, y, order = ritz_select(evals, evecs, k)
+        kk = min(m, max(k, keep or k))
+        if (pick is None and VA.is_cuda and ritz_min <= m <= ritz_max_m()
+                and kk <= 64):
+            # large subspaces: the Ritz pairs from the device eigensolver
+            Yd, wq = ritz_eigh(HG[:m, :m], HG[:m, cap:cap + m], kk)
+            wq = wq.cpu().numpy()
+            evals, evecs = wq[:kk], Yd.cpu().numpy()
+            w, y, order = evals[:k], evecs[:, :k], np.arange(k)
+            sq, xax = wq[kk:kk + k], wq[2 * kk:2 * kk + k]
+        else:
+            H = 0.5 * (Hns + Hns.T)
+            with _host_lapack():
+                evals, evecs = np.linalg.eigh(H)
+            w, y, order = ritz_select(evals, evecs, k)
+            sq = ((G @ y) * y).sum(0)
+            xax = ((Hns @ y) * y).sum(0)
         y_rows = m
-        sq = ((G @ y) * y).sum(0)
-        xax = ((Hns @ y) * y).sum(0)
         # |A x - w x|^2 = |A x|^2 - 2 w x^T A x + w^2 (|x| = 1), exact for a
         # non-symmetric A as well
         rn = np.sqrt(np.clip(sq - 2.0 * w * xax + w ** 2, 0.0, None))
@@ -503,7 +519,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             if jd:
                 return _jd(Xu, Ru, diag, wu)
             if precond_roots:
-                return pre(Ru, Xu, wu, cycle, idx).contiguous()
+                return pre(Ru, Xu, wu, cycle, idx, rn[idx]).contiguous()
             return pre(Ru, Xu, wu, cycle).contiguous()
         Tc = correction(precond, use_jd)
         mark('precond')
@@ -526,7 +542,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             HG[:kk, cap:cap + kk] = torch.as_tensor(Gc, device=dev)
             m = kk
         mark('collapse')
-        q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop)
+        q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop, matmat)
         if q == 0 and on_stall is not None and not stalled:
             # the correction added nothing new: the same fallback as a stall
             stalled = True
@@ -534,15 +550,14 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             if new_pre is not None:
                 precond, use_jd = new_pre, False
             Tc = correction(precond, use_jd)
-            q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop)
-        mark('orthonormal')
+            q, C, Hrow = _orthonormalize_block(VA, m, cap, Tc, Hns, drop, matmat)
+        # the operator's pass on the new block ran inside (its image is in
+        # AV's next columns)
+        mark('orthonormal_sigma')
         if q == 0:
             exhausted = True
             break
-        AT = matmat(VA[:, m:m + q])
-        mark('sigma')
         vectors += q
-        VA[:, cap + m:cap + m + q] = AT
         P = _tnr(Ranges(VA, (0, m + q), (cap, m + q)), Ranges(VA, (cap + m, q)))
         HG[:m + q, m:m + q] = P[:m + q]
         HG[:m + q, cap + m:cap + m + q] = P[m + q:]
@@ -566,7 +581,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
                           stalled=stalled, exhausted=exhausted or m >= n)
 
 
-def _orthonormalize_block(VA, m, cap, Tc, Hns, drop):
+def _orthonormalize_block(VA, m, cap, Tc, Hns, drop, matmat=None):
     """Append the corrections Tc (n, p) orthonormalised against the m columns
     of V in VA (n, 2 cap) as V's next columns.  Returns (q, C, Hrow): the
     number of columns kept, the device (p, q) map from Tc's projection to
@@ -579,27 +594,66 @@ def _orthonormalize_block(VA, m, cap, Tc, Hns, drop):
     its norm before the projection is dropped, the rest are orthonormalised
     by G2's eigendecomposition (T2 C).  T^T AV follows from the algebra
     AV^T T2 = AV^T T - H^T (S1 + S2), H = V^T AV the whole Rayleigh matrix
-    (the host copy `Hns`)."""
+    (the host copy `Hns`).
+
+    With `matmat`, the operator is applied to the projected block as soon
+    as the second pass has formed it, while the host works out C: the image
+    of the kept columns is (A T2) C (A is linear), written into AV's next
+    columns beside them.  The block goes to the operator with its columns
+    scaled to unit norm (the dropped ones zeroed; the scale and the choice
+    made on the device travel with the projections), so a lower-precision
+    operator's rounding, relative to its input, is not magnified by C's
+    1 / |column| factors."""
     dev = VA.device
     n, p = Tc.shape
     VA[:, m:m + p] = Tc
     # pass 1: [V | T | AV]^T T -> S1 = V^T T, G0 = T^T T, Pav = AV^T T
-    stage = torch.empty((3 * m + 2 * p, p), dtype=torch.float64, device=dev)
+    stage = torch.empty((3 * m + 2 * p + 2, p), dtype=torch.float64, device=dev)
     P1 = stage[:2 * m + p]
     _tnr(Ranges(VA, (0, m + p), (cap, m)), Ranges(VA, (m, p)), out=P1)
     _nn(Ranges(VA, (0, m)), P1[:m], VA, m, alpha=-1.0, beta=1.0)
     # pass 2: [V | T1]^T T1 -> S2 = V^T T1, G1 = T1^T T1
-    P2 = stage[2 * m + p:]
+    P2 = stage[2 * m + p:3 * m + 2 * p]
     _tnr(Ranges(VA, (0, m + p)), Ranges(VA, (m, p)), out=P2)
     _nn(Ranges(VA, (0, m)), P2[:m], VA, m, alpha=-1.0, beta=1.0)
-    h = stage.cpu().numpy()
+    if matmat is not None and VA.is_cuda:
+        block_scale(stage, m, p, drop, VA, m)
+    elif matmat is not None:
+        # |T2 columns| = diag(G1 - S2^T S2); drop as below, scale the rest
+        g1 = torch.diagonal(P2[m:])
+        after = (g1 - (P2[:m] * P2[:m]).sum(0)).clamp_min(0.0).sqrt()
+        before = torch.diagonal(P1[m:m + p]).clamp_min(0.0).sqrt()
+        kept = after > drop * before.clamp_min(1e-300)
+        stage[-2] = after
+        stage[-1] = kept.to(torch.float64)
+        VA[:, m:m + p] *= torch.where(kept, 1.0 / after.clamp_min(1e-300),
+                                      torch.zeros_like(after))[None, :]
+    if VA.is_cuda:
+        # the projections leave first; the operator's pass on T2 is queued
+        # behind the copy and runs while the host solves the Gram problem
+        host = torch.empty(stage.shape, dtype=torch.float64, pin_memory=True)
+        host.copy_(stage, non_blocking=True)
+        landed = torch.cuda.Event()
+        landed.record()
+    else:
+        host = stage
+    if matmat is not None:
+        VA[:, cap + m:cap + m + p] = matmat(VA[:, m:m + p])
+    if VA.is_cuda:
+        landed.synchronize()
+    h = host.numpy()
     S1, G0, Pav = h[:m], h[m:m + p], h[m + p:2 * m + p]
-    S2, G1 = h[2 * m + p:3 * m + p], h[3 * m + p:]
+    S2, G1 = h[2 * m + p:3 * m + p], h[3 * m + p:3 * m + 2 * p]
     G2 = G1 - S2.T @ S2
     G2 = 0.5 * (G2 + G2.T)
-    before = np.sqrt(np.clip(np.diag(G0), 0.0, None))
-    after = np.sqrt(np.clip(np.diag(G2), 0.0, None))
-    keep = np.flatnonzero(after > drop * np.maximum(before, 1e-300))
+    if matmat is not None:
+        # the device's choice and scale (the block it sent to the operator)
+        scale, keep = h[-2], np.flatnonzero(h[-1] > 0.5)
+    else:
+        before = np.sqrt(np.clip(np.diag(G0), 0.0, None))
+        after = np.sqrt(np.clip(np.diag(G2), 0.0, None))
+        keep = np.flatnonzero(after > drop * np.maximum(before, 1e-300))
+        scale = np.ones(p)
     if keep.size == 0:
         return 0, None, None
     with _host_lapack():
@@ -609,10 +663,22 @@ def _orthonormalize_block(VA, m, cap, Tc, Hns, drop):
     C = np.zeros((p, q))
     C[keep] = U[:, good] / np.sqrt(s[good])
     Hrow = ((Pav - Hns.T @ (S1 + S2)) @ C).T
-    up = torch.as_tensor(np.concatenate([C.ravel(), Hrow.ravel()]), device=dev)
+    # T2 C = (T2 / scale) (scale C): the block in VA holds T2 / scale
+    up = np.concatenate([(scale[:, None] * C).ravel(), Hrow.ravel()])
+    if VA.is_cuda:
+        # queued behind the operator's pass without holding the host
+        pinned = torch.empty(up.size, dtype=torch.float64, pin_memory=True)
+        pinned.numpy()[:] = up
+        up = pinned.to(dev, non_blocking=True)
+    else:
+        up = torch.as_tensor(up)
     Cd, Hd = up[:p * q].reshape(p, q), up[p * q:].reshape(q, m)
     Tn = torch.empty((n, q), dtype=torch.float64, device=dev)
     _nn(Ranges(VA, (m, p)), Cd, Tn, 0)
+    if matmat is not None:
+        ATn = torch.empty((n, q), dtype=torch.float64, device=dev)
+        _nn(Ranges(VA, (cap + m, p)), Cd, ATn, 0)
+        VA[:, cap + m:cap + m + q] = ATn
     VA[:, m:m + q] = Tn
     return q, Cd, Hd
 
@@ -908,21 +974,26 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                              '(a LearnedPreconditioner)')
         state = {'fallback': False}
 
-        def precond(R, U, theta, cycle, roots=None):
+        def precond(R, U, theta, cycle, roots=None, rnorm=None):
             # near convergence the useful (out-of-space) part of a correction
             # is small next to the network's BF16 output noise: the
             # Jacobi-Davidson correction takes over below learned_until * tol,
             # root by root
-            jd = jacobi_davidson_correction(R, U, pdiag, theta)
+            jd = _jd(U, R, pdiag, theta)
             if state['fallback']:
                 return jd
             if learned_gate == 'root' and learned_until is not None:
-                sel = R.norm(dim=0) >= learned_until * conv_tol
+                # the gate reads the Ritz step's residual norms on the host
+                # when the iteration passes them (no device round trip)
+                rsel = (np.asarray(rnorm) if rnorm is not None
+                        else R.norm(dim=0).cpu().numpy())
+                sel = rsel >= learned_until * conv_tol
                 if learned_roots == 'requested' and roots is not None:
-                    sel &= torch.as_tensor(np.asarray(roots) < nroots, device=sel.device)
-                big = torch.nonzero(sel).squeeze(1)
-                if big.numel() == 0:
+                    sel &= np.asarray(roots) < nroots
+                big = np.flatnonzero(sel)
+                if big.size == 0:
                     return jd
+                big = torch.as_tensor(big, device=R.device)
                 if big.numel() < R.shape[1]:
                     t = learned.correction(R[:, big], U[:, big], theta[big], pdiag,
                                            jd=jd[:, big])
@@ -931,9 +1002,11 @@ def run_davidson(apply_AB, diag_d, nroots, conv_tol, max_cycle, x_sym=None,
                     jd = jd.clone()
                     jd[:, big] = t
                     return jd
-            elif (learned_until is not None and
-                    float(R.norm(dim=0).max()) < learned_until * conv_tol):
-                return jd
+            elif learned_until is not None:
+                worst = (float(np.max(rnorm)) if rnorm is not None
+                         else float(R.norm(dim=0).max()))
+                if worst < learned_until * conv_tol:
+                    return jd
             t = learned.correction(R, U, theta, pdiag, jd=jd)
             # 'augment': both directions enter the space (the diagonal one is
             # never lost); 'replace': the learned one alone

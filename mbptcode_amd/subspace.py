@@ -156,6 +156,48 @@ def ritz_residual(buf, m, v_off, av_off, Y, theta, X, R):
     return X, R
 
 
+def block_scale(stage, m, p, drop, buf, t_off):
+    """Unit scaling of the projected block buf[:, t_off : t_off + p] from the
+    orthonormalisation's projections `stage` ((3m + 2p + 2, p) contiguous):
+    its last two rows receive the columns' norms and keep flags, and the
+    block's columns are scaled to unit norm or zeroed (dvd.hip)."""
+    n = buf.shape[0]
+    if stage.shape != (3 * m + 2 * p + 2, p) or not stage.is_contiguous():
+        raise ValueError('stage must be a contiguous (3m + 2p + 2, p) buffer')
+    if not (buf.is_contiguous() and 0 <= t_off and t_off + p <= buf.shape[1]):
+        raise ValueError('the block must be p columns of a contiguous (n, ld) buffer')
+    _lib.check(_lib.lib().mbpt_block_scale_f64(
+        n, m, p, float(drop), _lib.dptr(stage),
+        ctypes.c_void_p(buf.data_ptr() + 8 * t_off), int(buf.shape[1]), _lib.stream_ptr()),
+        'mbpt_block_scale_f64')
+
+
+def ritz_max_m():
+    """The largest Rayleigh matrix `ritz_eigh` takes."""
+    return int(_lib.lib().mbpt_ritz_max_m())
+
+
+def ritz_eigh(H, G, k, Y=None):
+    """The lowest k eigenpairs of the symmetric part of H, on the device.
+    H and G are (m, m) FP64 views with unit column stride (blocks of the
+    Davidson's [H | G] buffer); returns Y (m, k) with the eigenvectors as
+    columns and wq (3k,): the eigenvalues ascending, y^T G y and y^T H y of
+    each column (ritz.hip)."""
+    m = H.shape[0]
+    if H.shape != (m, m) or G.shape != (m, m) or H.stride(1) != 1 or G.stride(1) != 1:
+        raise ValueError('H and G must be (m, m) views with unit column stride')
+    lib = _lib.lib()
+    if Y is None:
+        Y = torch.empty((m, k), dtype=torch.float64, device=H.device)
+    wq = torch.empty(3 * k, dtype=torch.float64, device=H.device)
+    work = _Work.get(H.device, int(lib.mbpt_ritz_eigh_f64_work(m, k)))
+    _lib.check(lib.mbpt_ritz_eigh_f64(
+        m, k, ctypes.c_void_p(H.data_ptr()), int(H.stride(0)), ctypes.c_void_p(G.data_ptr()),
+        int(G.stride(0)), _lib.dptr(Y), int(Y.stride(0)), _lib.dptr(wq), _lib.dptr(work),
+        work.numel(), _lib.stream_ptr()), 'mbpt_ritz_eigh_f64')
+    return Y, wq
+
+
 def jd_correction(U, R, diag, theta, T, t_off=0, floor=1e-8):
     """T[:, t_off : t_off + k] = K^-1 R - alpha K^-1 U per column (the
     Jacobi-Davidson correction, Solvers/davidson.py:43-77); U, R (n, k)

@@ -82,3 +82,42 @@ def test_jd_correction(n):
     scale = float(ref.abs().max())
     assert float((got - ref).abs().max()) <= 1e-12 * scale
     assert torch.isnan(T[:, :2]).all() and torch.isnan(T[:, 2 + k:]).all()
+
+
+@pytest.mark.parametrize('n,m,p', [(1, 0, 3), (1953, 40, 30), (70000, 161, 64)])
+def test_block_scale_against_numpy(n, m, p):
+    """The unit scaling of a projected correction block: norms after the
+    projections from G1 - S2^T S2, keep flags against the norms before, and
+    the block's columns scaled (kept) or zeroed (dropped)."""
+    from mbptcode_amd.subspace import block_scale
+    rng = np.random.default_rng(n + m + p)
+    S1 = rng.normal(size=(m, p))
+    T = rng.normal(size=(n, p))
+    G0 = T.T @ T
+    S2 = 1e-3 * rng.normal(size=(m, p))
+    T1 = T.copy()
+    T1[:, :2] *= 1e-9      # two columns that the projections all but removed
+    G1 = T1.T @ T1 + S2.T @ S2
+    stage = np.concatenate([S1, G0, rng.normal(size=(m, p)), S2, G1, np.zeros((2, p))])
+    buf = rng.normal(size=(n, 3 * p + 5))
+    off = p + 2
+    buf[:, off:off + p] = T1
+    drop = 1e-6
+    sd, bd = dev(stage), dev(buf)
+    print(f'block_scale n={n} m={m} p={p}', flush=True)
+    block_scale(sd, m, p, drop, bd, off)
+    got_s, got_b = sd.cpu().numpy(), bd.cpu().numpy()
+    after = np.sqrt(np.clip(np.diag(G1) - (S2 * S2).sum(0), 0, None))
+    before = np.sqrt(np.clip(np.diag(G0), 0, None))
+    keep = after > drop * np.maximum(before, 1e-300)
+    assert not keep[:2].any() and keep[2:].all()
+    # a dropped column's norm is a difference far below the terms it comes
+    # from: it matches to the rounding of those terms (sum order differs)
+    assert np.allclose(got_s[-2][keep], after[keep], rtol=1e-12, atol=0)
+    assert np.allclose(got_s[-2][~keep], after[~keep], rtol=0,
+                       atol=1e-14 * np.sqrt(np.diag(G1).max()))
+    assert np.array_equal(got_s[-1], keep.astype(float))
+    assert np.array_equal(got_s[:-2], stage[:-2])
+    ref = buf.copy()
+    ref[:, off:off + p] = T1 * np.where(keep, 1.0 / np.maximum(after, 1e-300), 0.0)
+    assert np.abs(got_b - ref).max() <= 1e-13 * np.abs(ref).max()
