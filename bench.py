@@ -60,7 +60,8 @@ def parse():
                    help='learned correction while the largest residual exceeds this x tol')
     p.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     p.add_argument('--no-legs', action='store_true', help='headline only')
-    p.add_argument('--leg', choices=['benzene', 'large', 'casida', 'gw', 'fit', 'batched'],
+    p.add_argument('--leg', choices=['benzene', 'large', 'casida', 'gw', 'fit', 'batched',
+                                     'ritz'],
                    help='run this side leg alone and print its JSON (no headline line)')
     p.add_argument('--kernel-probe', nargs='?', const='k20', choices=['k20', 'mix'],
                    help='counter-run workload, then one 256 MiB device copy: k20, three '
@@ -665,6 +666,45 @@ def leg_batched():
                                                'tail products)')}
 
 
+def leg_ritz(reps=20):
+    """The Davidson's Rayleigh-Ritz step at the subspace sizes of the
+    headline solves and above: the device eigensolver (ritz.hip, the lowest
+    30 pairs and their quadratic forms, timed with HIP events on its stream)
+    against host LAPACK's eigh on one thread (timed on the host clock),
+    random symmetric matrices with the Ritz matrices' scale."""
+    import torch
+    from threadpoolctl import threadpool_limits
+    from mbptcode_amd.subspace import ritz_eigh
+    rng = np.random.default_rng(0)
+    out = []
+    for m in (40, 70, 100, 130, 161, 192):
+        A = rng.normal(size=(m, m))
+        H = A + A.T
+        d = torch.as_tensor(np.concatenate([H, H @ H], 1), device='cuda')
+        k = 30
+        ritz_eigh(d[:, :m], d[:, m:], k)
+        sync()
+        s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        s.record()
+        for _ in range(reps):
+            _, wq = ritz_eigh(d[:, :m], d[:, m:], k)
+        e.record()
+        sync()
+        dev_us = s.elapsed_time(e) / reps * 1e3
+        err = float(np.abs(wq.cpu().numpy()[:k] - np.linalg.eigvalsh(H)[:k]).max())
+        with threadpool_limits(1):
+            np.linalg.eigh(H)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                np.linalg.eigh(H)
+            host_us = (time.perf_counter() - t0) / reps * 1e6
+        out.append({'m': m, 'k': k, 'device_us': dev_us, 'host_lapack_us': host_us,
+                    'max_eig_err': err})
+        log(f'ritz m={m}: device {dev_us:.0f} us, host LAPACK {host_us:.0f} us, '
+            f'eigenvalue err {err:.1e}')
+    return out
+
+
 def _blas_threads():
     try:
         from threadpoolctl import threadpool_info
@@ -746,7 +786,7 @@ def main():
         return
     if args.leg:
         legs = {'benzene': leg_benzene, 'large': leg_large, 'gw': leg_gw,
-                'fit': leg_fit, 'batched': leg_batched}
+                'fit': leg_fit, 'batched': leg_batched, 'ritz': leg_ritz}
         if args.leg == 'casida':
             inst, W, _ = build_problem(4)
             print(json.dumps({'casida': leg_full_action(inst, W)}), flush=True)

@@ -188,13 +188,18 @@ __global__ __launch_bounds__(64) void k_nns(int n, int k, int c, Cols2 A, Cols2 
 // a_j > drop max(b_j, 1e-300).  Rows 3m + 2p and 3m + 2p + 1 of `pr` receive
 // a_j and the 0 / 1 keep flag, and the block's columns are scaled by 1 / a_j
 // (kept) or 0 (dropped).
-__global__ __launch_bounds__(64) void k_block_norms(int m, int p, double drop,
-                                                    double *__restrict__ pr) {
-  const int j = threadIdx.x;
-  if (j >= p) return;
+__global__ __launch_bounds__(256) void k_block_norms(int m, int p, double drop,
+                                                     double *__restrict__ pr) {
+  __shared__ double part[4][64];
+  const int j = threadIdx.x & 63, q = threadIdx.x >> 6;   // column, quarter of S2's rows
   const double *S2 = pr + (long)(2 * m + p) * p, *G1 = S2 + (long)m * p;
   double s2 = 0.0;
-  for (int i = 0; i < m; ++i) s2 += S2[(long)i * p + j] * S2[(long)i * p + j];
+  if (j < p)
+    for (int i = q; i < m; i += 4) s2 += S2[(long)i * p + j] * S2[(long)i * p + j];
+  part[q][j] = s2;
+  __syncthreads();
+  if (q != 0 || j >= p) return;
+  s2 = (part[0][j] + part[1][j]) + (part[2][j] + part[3][j]);
   const double a = sqrt(fmax(G1[(long)j * p + j] - s2, 0.0));
   const double b = sqrt(fmax(pr[(long)(m + j) * p + j], 0.0));
   pr[(long)(3 * m + 2 * p) * p + j] = a;
@@ -386,7 +391,7 @@ int mbpt_block_scale_f64(int n, int m, int p, double drop, double *pr, double *T
   MBPT_REQUIRE(pr && T && ldt >= p, "mbpt_block_scale_f64: NULL argument or ldt < p");
   MBPT_REQUIRE(drop >= 0.0, "mbpt_block_scale_f64: drop must be >= 0");
   hipStream_t st = (hipStream_t)stream;
-  k_block_norms<<<1, 64, 0, st>>>(m, p, drop, pr);
+  k_block_norms<<<1, 256, 0, st>>>(m, p, drop, pr);
   int rc = launch_status("k_block_norms");
   if (rc) return rc;
   const long total = (long)n * p;
