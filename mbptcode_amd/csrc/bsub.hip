@@ -15,16 +15,16 @@
 namespace mbpt {
 namespace {
 
-// NN: 64 rows per block.  The rows' A values (m <= 64) are staged in LDS with
-// coalesced loads, S 16 columns at a time; thread (row tid/4, column
-// quarter tid%4) computes 4 outputs per column block.
+// NN on v_mfma_f64_16x16x4f64: 64 rows per block, 16 per wave, the output's
+// columns in groups of up to four 16-wide tiles.  Per 4-deep K step a lane
+// supplies A[row lr][k lq] (its row's 4 consecutive values) and S[k lq][col
+// lr] of each tile; S is shared by every block of the instance, so its reads
+// come from the caches.
 __global__ __launch_bounds__(256) void k_bnn(int P, int m, int c, const int *__restrict__ rows,
                                              const double *__restrict__ A, long sA, int lda,
                                              const double *__restrict__ S, long sS, int lds,
                                              double *__restrict__ Y, long sY, int ldy,
                                              int subtract) {
-  __shared__ double sa[64][65];
-  __shared__ double st[64][17];
   const int b = blockIdx.y, r0 = blockIdx.x * 64, tid = threadIdx.x;
   const int nr = rows[b];
   double *Yb = Y + b * sY;
@@ -37,31 +37,36 @@ __global__ __launch_bounds__(256) void k_bnn(int P, int m, int c, const int *__r
     return;
   }
   const double *Ab = A + b * sA, *Sb = S + b * sS;
-  for (int e = tid; e < 64 * 64; e += 256) {
-    const int r = e >> 6, i = e & 63;
-    sa[r][i] = (r0 + r < nr && i < m) ? Ab[(long)(r0 + r) * lda + i] : 0.0;
-  }
-  const int r = tid >> 2, jq = tid & 3, p = r0 + r;
-  for (int cb = 0; cb < c; cb += 16) {
-    __syncthreads();
-    for (int e = tid; e < 64 * 16; e += 256) {
-      const int i = e >> 4, j = e & 15;
-      st[i][j] = (i < m && cb + j < c) ? Sb[(long)i * lds + cb + j] : 0.0;
-    }
-    __syncthreads();
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = 0; i < m; ++i) {
-      const double a = sa[r][i];
+  const int lane = tid & 63, wave = tid >> 6, lr = lane & 15, lq = lane >> 4;
+  const int ra = r0 + 16 * wave + lr;          // the row this lane feeds to A
+  const bool rok = ra < nr;
+  for (int c0 = 0; c0 < c; c0 += 64) {
+    const int nct = min(4, (c - c0 + 15) / 16);
+    f64x4 acc[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] += a * st[i][jq + 4 * u];
-    }
-    if (p < P) {
+    for (int t = 0; t < 4; ++t) zero(acc[t]);
+    for (int k = 0; k < m; k += 4) {
+      const int ka = k + lq;
+      const double a = (rok && ka < m) ? Ab[(long)ra * lda + ka] : 0.0;
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int j = cb + jq + 4 * u;
-        if (j >= c) continue;
-        double *y = Yb + (long)p * ldy + j;
-        if (p < nr) *y = subtract ? *y - acc[u] : acc[u];
+      for (int t = 0; t < 4; ++t) {
+        if (t >= nct) break;   // uniform
+        const int col = c0 + 16 * t + lr;
+        const double sv = (ka < m && col < c) ? Sb[(long)ka * lds + col] : 0.0;
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sv, acc[t], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      if (t >= nct) break;
+      const int col = c0 + 16 * t + lr;
+      if (col >= c) continue;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int p = r0 + 16 * wave + MF<double>::orow(lq, r);
+        if (p >= P) continue;
+        double *y = Yb + (long)p * ldy + col;
+        if (p < nr) *y = subtract ? *y - acc[t][r] : acc[t][r];
         else if (!subtract) *y = 0.0;
       }
     }

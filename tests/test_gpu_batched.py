@@ -297,3 +297,29 @@ def test_batched_subspace_products_match_bmm():
     Z = Bm.clone()
     bnn(Av, S, rows, out=Z, subtract=True)
     assert torch.allclose(Z, Bm - ref_nn, rtol=1e-12, atol=1e-10)
+
+
+@pytest.mark.parametrize('m,c', [(64, 70), (4, 1), (17, 33)])
+def test_batched_nn_product_shapes(m, c):
+    """NN over column groups past 64, the narrowest product and ragged tiles;
+    padding rows zeroed by assignment and left alone by subtraction."""
+    from mbptcode_amd.batched import bnn
+    print(f'batched nn m={m} c={c}', flush=True)
+    rng = np.random.default_rng(m + c)
+    B, P = 3, 300
+    rows_h = np.array([300, 65, 1], np.int32)
+    rows = torch.as_tensor(rows_h, device='cuda')
+    A = torch.zeros((B, P, m), dtype=torch.float64, device='cuda')
+    for b in range(B):
+        A[b, :rows_h[b]] = torch.as_tensor(rng.normal(size=(rows_h[b], m)), device='cuda')
+    S = torch.as_tensor(rng.normal(size=(B, m, c)), device='cuda')
+    ref = torch.bmm(A, S)
+    Y = torch.full((B, P, c), 5.0, dtype=torch.float64, device='cuda')
+    bnn(A, S, rows, out=Y)
+    assert torch.allclose(Y, ref, rtol=1e-12, atol=1e-10)
+    Z = torch.full((B, P, c), 2.0, dtype=torch.float64, device='cuda')
+    bnn(A, S, rows, out=Z, subtract=True)
+    for b in range(B):
+        n = rows_h[b]
+        assert torch.allclose(Z[b, :n], 2.0 - ref[b, :n], rtol=1e-12, atol=1e-10)
+        assert (Z[b, n:] == 2.0).all()
