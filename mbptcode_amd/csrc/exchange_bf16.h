@@ -125,13 +125,15 @@ __global__ void k_pack_bf16_xo(int M, int no, const float *__restrict__ Xo,
 }
 
 // ---- W in 32 x 64 tiles ---------------------------------------------------------------
+// nvb >= ceil(M / 64) tiles per 32-row chunk (an even count, so that a
+// 128-column workgroup's two tiles always exist; the extra one is zero).
 // wtile[(c * nvb + vb) * 512 + 256 tt + 64 w + 16 q + r] holds, as one 16-byte
 // slot, W[32 c + 16 tt + 4 q + 0..3][64 vb + 16 w + r]: the four W values lane
 // 16 q + r of wave w multiplies its P tile tt by (P's result rows 4 q + 0..3),
 // zero outside the M x M matrix.  A tile is 8 KB in the order the exchange
 // reads it from LDS, so it streams in with the X_o chunk's own loads.
-__global__ void k_pack_w_tiles(int M, const float *__restrict__ W, f32x4 *__restrict__ wt) {
-  const int nvb = (M + 63) / 64;
+__global__ void k_pack_w_tiles(int M, int nvb, const float *__restrict__ W,
+                               f32x4 *__restrict__ wt) {
   const long n = (long)((M + 31) / 32) * nvb * 512;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < n;
        e += (long)gridDim.x * blockDim.x) {
@@ -161,16 +163,21 @@ typedef __attribute__((address_space(3))) void lds_void;
 // are loaded into registers while the current chunk computes, at clamped
 // indices: a row >= M meets a zero row of the packed X_o (P = 0 there) and a
 // column >= M a zero column of zXv.
-template <int NP, int NIE, int TAIL, int NV, int NC, bool WT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_exchange_bf16(
+// NW waves per workgroup (4, or 8 with W tiles: 128 columns, one workgroup
+// per CU, so the X_o stage is filled once per CU and chunk instead of twice).
+template <int NP, int NIE, int TAIL, int NV, int NC, bool WT, int NW = 4>
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(2))) void k_exchange_bf16(
     int M, int no, int k, const bf16x8 *__restrict__ frag, const float *__restrict__ W,
     const f32x4 *__restrict__ wtile, const float *__restrict__ zXv, float *__restrict__ Tpart,
-    int nsplit, int rows_per_split, int nvb, int nblk) {
+    int nsplit, int rows_per_split, int nvb, int nblk, int nvb64) {
   constexpr int NKB = NP / 32, NFB = TAIL == 3 ? NKB : NKB - 1, NT = NV * NC;
   constexpr int TP = TAIL == 3 ? 2 : TAIL, FP = 2 * (NKB - 1) + TP;
-  constexpr int SPP = 128 * FP, SPC = SPP + 128 * NIE, COLS = 64 * NC;
+  constexpr int NTH = 64 * NW;                    // threads
+  constexpr int SPP = 128 * FP, SPC = SPP + 128 * NIE, COLS = 16 * NC * NW;
   static_assert(!WT || NC == 1, "a W tile holds 64 columns");
-  constexpr int SPW = WT ? 512 : 0;   // W tile slots per stage
+  static_assert(NW == 4 || (NW == 8 && WT), "eight waves only with W tiles");
+  constexpr int NWT = NW / 4;                     // W tiles per stage
+  constexpr int SPW = WT ? 512 * NWT : 0;         // W tile slots per stage
   __shared__ bf16x8 buf[2][SPC + SPW];
   const int kp = (k + NV - 1) / NV;
   const int lid = xcd_logical(blockIdx.x, nblk);
@@ -250,21 +257,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                               min(colv[ct], M - 1)];
     }
   };
-  // chunk c into buf[n]: slot 256 q + tid from thread tid, each wave's
+  // chunk c into buf[n]: slot NTH q + tid from thread tid, each wave's
   // instruction filling 1 KB of LDS at base + 16 lane
   auto fetch = [&](int c, int n) {
     const bf16x8 *src = frag + (long)c * SPC;
 #pragma unroll
-    for (int q = 0; q < (SPC + 255) / 256; ++q)
-      if (SPC % 256 == 0 || 256 * q + 64 * wave < SPC)
-        __builtin_amdgcn_global_load_lds((const void *)(src + 256 * q + tid),
-                                         (lds_void *)&buf[n][256 * q + 64 * wave], 16, 0, 0);
-    if (WT) {
-      const f32x4 *wsrc = wtile + ((long)c * nvb + vb) * 512;
+    for (int q = 0; q < (SPC + NTH - 1) / NTH; ++q)
+      if (SPC % NTH == 0 || NTH * q + 64 * wave < SPC)
+        __builtin_amdgcn_global_load_lds((const void *)(src + NTH * q + tid),
+                                         (lds_void *)&buf[n][NTH * q + 64 * wave], 16, 0, 0);
+    if (WT) {   // the workgroup's NWT consecutive tiles
+      const f32x4 *wsrc = wtile + ((long)c * nvb64 + NWT * vb) * 512;
 #pragma unroll
-      for (int q = 0; q < 2; ++q)
-        __builtin_amdgcn_global_load_lds((const void *)(wsrc + 256 * q + tid),
-                                         (lds_void *)&buf[n][SPC + 256 * q + 64 * wave], 16, 0,
+      for (int q = 0; q < SPW / NTH; ++q)
+        __builtin_amdgcn_global_load_lds((const void *)(wsrc + NTH * q + tid),
+                                         (lds_void *)&buf[n][SPC + NTH * q + 64 * wave], 16, 0,
                                          0);
     }
   };
@@ -272,7 +279,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   auto lds_w = [&](int n, float (&w)[NC][2][4]) {
 #pragma unroll
     for (int tt = 0; tt < 2; ++tt) {
-      const f32x4 v = *(const f32x4 *)&buf[n][SPC + 256 * tt + 64 * wave + lane];
+      const f32x4 v = *(const f32x4 *)&buf[n][SPC + 512 * (wave >> 2) + 256 * tt +
+                                              64 * (wave & 3) + lane];
 #pragma unroll
       for (int rr = 0; rr < 4; ++rr) w[0][tt][rr] = v[rr];
     }

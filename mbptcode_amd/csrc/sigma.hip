@@ -760,6 +760,10 @@ constexpr int kTargetBlocks = 2048;
 
 int grid1(long n) { return (int)std::min<long>((n + 255) / 256, 65535L); }
 
+// W tiles (32 x 64) per 32-row chunk in the packed layout: ceil(M / 64)
+// rounded up to even, so that the 128-column workgroups' two tiles exist
+int w_tiles_per_chunk(int M) { return ((M + 63) / 64 + 1) / 2 * 2; }
+
 // Timed applies waiting to be read: at most this many, then the oldest
 // figures are collected (which waits for them) before more are recorded.
 constexpr size_t kMaxPendingApplies = 512;
@@ -940,12 +944,13 @@ void launch_exchange_b(const mbpt_sigma *h, int k, int nsplit, int rps, T *Upart
 // tiles: W from global memory
 template <int NP, int NIE, int TAIL, int NV>
 void launch_exchange_bf16_nv(const mbpt_sigma *h, int k, int nsplit, int rps, hipStream_t st) {
-  constexpr int NC = NV == 2 ? 1 : 2;
-  const int nvb = (h->M + 64 * NC - 1) / (64 * NC), kp = (k + NV - 1) / NV;
+  constexpr int NC = NV == 2 ? 1 : 2, NW = NV == 2 ? 8 : 4;
+  const int cols = 16 * NC * NW;
+  const int nvb = (h->M + cols - 1) / cols, kp = (k + NV - 1) / NV;
   const int nblk = pad_grid((long)nvb * kp * nsplit);
-  k_exchange_bf16<NP, NIE, TAIL, NV, NC, NV == 2><<<nblk, 256, 0, st>>>(
+  k_exchange_bf16<NP, NIE, TAIL, NV, NC, NV == 2, NW><<<nblk, 64 * NW, 0, st>>>(
       h->M, h->no, k, h->frag, (const float *)h->W, h->wtile, (const float *)h->ws[WS_ZXV],
-      (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk);
+      (float *)h->ws[WS_TPART], nsplit, rps, nvb, nblk, w_tiles_per_chunk(h->M));
 }
 
 template <int NP, int NIE, int TAIL>
@@ -1324,14 +1329,15 @@ int mbpt_sigma_create(mbpt_sigma **out, int M, int nocc, int nvir, int precision
     h->held += sizeof(bf16x8) * slots;
     k_pack_bf16_xo<<<grid1(slots), 256, 0, st>>>(M, nocc, (const float *)h->Xo, h->frag);
     if ((rc = launch_status("k_pack_bf16_xo"))) return fail(rc);
-    const long wslots = (long)((M + 31) / 32) * ((M + 63) / 64) * 512;
+    const long wslots = (long)((M + 31) / 32) * w_tiles_per_chunk(M) * 512;
     he = hipMalloc((void **)&h->wtile, sizeof(f32x4) * wslots);
     if (he != hipSuccess) {
       set_error(std::string("hipMalloc: ") + hipGetErrorString(he));
       return fail(MBPT_ERR_HIP);
     }
     h->held += sizeof(f32x4) * wslots;
-    k_pack_w_tiles<<<grid1(wslots), 256, 0, st>>>(M, (const float *)h->W, h->wtile);
+    k_pack_w_tiles<<<grid1(wslots), 256, 0, st>>>(M, w_tiles_per_chunk(M), (const float *)h->W,
+                                                   h->wtile);
     if ((rc = launch_status("k_pack_w_tiles"))) return fail(rc);
   }
   *out = h;
