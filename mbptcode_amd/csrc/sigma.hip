@@ -852,8 +852,19 @@ ApplyPlan plan_for(const mbpt_sigma *h, int k) {
   // tail: K = M split in nq chunks of kchunk (multiples of 16), enough
   // workgroups for the skinny output
   if (h->prec == MBPT_BF16X3) {
+    // K split by a cost model: rounds of the 512 resident workgroups times
+    // the K steps (32 deep) each one runs, plus two steps' worth for its
+    // partial tile; a split whose last round is nearly empty loses to a
+    // finer or a coarser one
     const int tiles = (a.ldr / 128) * (h->nvq / 128);
-    a.nq = std::max(1, std::min((512 + tiles - 1) / tiles, h->Mp32 / 64));
+    const int ksteps = h->Mp32 / 32;
+    double best = 1e300;
+    a.nq = 1;
+    for (int nq = 1; nq <= std::max(1, h->Mp32 / 64); ++nq) {
+      const long rounds = ((long)tiles * nq + 511) / 512;
+      const double cost = (double)rounds * ((ksteps + nq - 1) / nq + 2);
+      if (cost < best - 1e-9) { best = cost; a.nq = nq; }
+    }
     a.kchunk = ((h->Mp32 + a.nq - 1) / a.nq + 31) / 32 * 32;
     a.nq = (h->Mp32 + a.kchunk - 1) / a.kchunk;
   } else if (km) {
