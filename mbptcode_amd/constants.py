@@ -61,10 +61,12 @@ DAVIDSON_COMPLETION_SEED = 0            # :2124 random directions of the complet
 DAVIDSON_FOLLOW_OVERLAP_TIE = 1e-6      # :2129 overlaps this close are a tie
 
 #: Smallest Rayleigh matrix whose Ritz pairs the device eigensolver
-#: (ritz.hip) computes in place of host LAPACK; below it the host is faster
-#: (measured on one MI355X box: 0.99 against 1.06 ms at m = 161, 1.12 against
-#: 1.64 ms at m = 192).  Not a reference constant.
-DAVIDSON_DEVICE_RITZ_MIN = 176
+#: (ritz.hip) computes in place of host LAPACK.  Inside the Davidson the host
+#: step also pays for its products yTGy, yTHy and the upload of the vectors;
+#: on one MI355X box the headline plain FP64 solve (Rayleigh matrices of 40
+#: to 161) took 64.3 ms with the host step throughout and 63.2 ms with the
+#: device solver from 70 or from 40.  Not a reference constant.
+DAVIDSON_DEVICE_RITZ_MIN = 64
 
 #: Largest pair space a Davidson whose trial space ran out is completed in
 #: by a dense solve (constants.py:590).

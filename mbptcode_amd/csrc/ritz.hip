@@ -3,13 +3,13 @@
 // 1 <= m <= 192, 1 <= k <= min(m, 64), and the two quadratic forms the
 // iteration's residual norms come from, without a trip to the host.
 //
-//   k_ritz_tridiag   Householder reduction to tridiagonal form, one
-//                    workgroup of 1024 threads; the matrix lives in registers,
-//                    thread (r, c) of a 32 x 32 grid holding the 6 x 6 entries
-//                    (32 a + r, 32 b + c).  Step j: reflector from column j,
-//                    p = tau A v, w = p - (tau / 2)(p.v) v, A -= v w^T + w v^T
-//                    on the trailing block (Golub & Van Loan, "Matrix
-//                    Computations", 8.3.1).
+//   k_ritz_tridiag   Householder reduction to tridiagonal form in one
+//                    workgroup, the matrix in registers (a 16 x 16 or 32 x 16
+//                    grid of threads, each holding a tile of entries).  Step
+//                    j: reflector from column j, p = tau A v,
+//                    w = p - (tau / 2)(p.v) v, A -= v w^T + w v^T on the
+//                    trailing block (Golub & Van Loan, "Matrix Computations",
+//                    8.3.1).
 //   k_ritz_tri_eig   one 64-lane workgroup per wanted eigenvalue: Sturm-count
 //                    multisection on the Gershgorin interval (64 points per
 //                    round) to working precision, then two steps of inverse
@@ -25,6 +25,7 @@
 #include "common.h"
 
 #include <cfloat>
+#include <type_traits>
 
 namespace mbpt {
 namespace {
@@ -76,129 +77,15 @@ __device__ __forceinline__ void lds_sync() {
 }
 
 // entry (a, b) of a thread's register tile for run-time block indices
-template <int NB>
-__device__ __forceinline__ double tile_get(const double (&A)[NB][NB], int a, int b) {
+template <int NR, int NC>
+__device__ __forceinline__ double tile_get(const double (&A)[NR][NC], int a, int b) {
   double x = 0.0;
 #pragma unroll
-  for (int p = 0; p < NB; ++p)
+  for (int p = 0; p < NR; ++p)
 #pragma unroll
-    for (int q = 0; q < NB; ++q)
+    for (int q = 0; q < NC; ++q)
       if (p == a && q == b) x = A[p][q];
   return x;
-}
-
-// Column j of the register-held matrix below its diagonal into x, and its
-// diagonal entry (final from step j on) into dd[j]: by the 32 threads whose
-// column lane is j % 32.
-template <int NB>
-__device__ __forceinline__ void put_column(const double (&A)[NB][NB], int m, int j, int r,
-                                           int c, double *x, double *dd) {
-  if (c != (j & 31)) return;
-#pragma unroll
-  for (int a = 0; a < NB; ++a) {
-    const int i = 32 * a + r;
-    const double v = tile_get(A, a, j >> 5);
-    if (i > j && i < m) x[i] = v;
-    if (i == j) dd[j] = v;
-  }
-}
-
-// Three LDS-only barriers a step: every wave forms the reflector's scalars
-// from the column in LDS by itself and the first 192 threads write v to LDS
-// once; the rows of p = tau A v are summed across the 32 column lanes with
-// DPP; the owners of column j + 1 put it in the other LDS buffer as soon as
-// their update is done.  Blocks wholly above the active part or past m are
-// skipped (the tests are uniform over the workgroup).
-// NB = ceil(m / 32) blocks per dimension: the register tile is NB x NB.
-template <int NB>
-__global__ __launch_bounds__(1024) void k_ritz_tridiag(int m, const double *__restrict__ H,
-                                                       int ldh, double *__restrict__ work) {
-  __shared__ double xs[2][kRMax], vs[kRMax], ps[kRMax];
-  double *dd = work + kOffD, *ee = work + kOffE, *taus = work + kOffTau, *hv = work + kOffHv;
-  const int tid = threadIdx.x, r = tid >> 5, c = tid & 31, lane = tid & 63;
-  double A[NB][NB];
-#pragma unroll
-  for (int a = 0; a < NB; ++a)
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int i = 32 * a + r, l = 32 * b + c;
-      A[a][b] = (i < m && l < m) ? 0.5 * (H[(long)i * ldh + l] + H[(long)l * ldh + i]) : 0.0;
-    }
-  if (m > 2) put_column(A, m, 0, r, c, xs[0], dd);
-  lds_sync();
-  for (int j = 0; j + 2 < m; ++j) {
-    const double *x = xs[j & 1];
-    // the reflector I - tau v v^T with v[j + 1] = 1, H x = beta e_{j+1}
-    double s2 = 0.0;
-    for (int i = j + 2 + lane; i < m; i += 64) s2 += x[i] * x[i];
-    s2 = wave_sum(s2);
-    const double alpha = x[j + 1];
-    double t = 0.0, scale = 0.0, beta = alpha;
-    if (s2 > 0.0) {
-      beta = -copysign(sqrt(alpha * alpha + s2), alpha);
-      t = (beta - alpha) / beta;
-      scale = 1.0 / (alpha - beta);
-    }
-    if (tid < kRMax) {
-      const double v = t == 0.0 ? 0.0
-                     : (tid == j + 1 ? 1.0 : (tid > j + 1 && tid < m ? x[tid] * scale : 0.0));
-      vs[tid] = v;
-      hv[(long)j * kRMax + tid] = v;
-      if (tid == 0) {
-        ee[j] = beta;
-        taus[j] = t;
-      }
-    }
-    lds_sync();
-    double vc[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) vc[b] = vs[32 * b + c];
-    // p = tau A v over the rows below j
-#pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      if (32 * a + 31 <= j || 32 * a >= m) continue;
-      double s = 0.0;
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        if (32 * b + 31 > j && 32 * b < m) s += A[a][b] * vc[b];
-      s = sum32_tail(s);
-      if (c == 31) ps[32 * a + r] = t * s;
-    }
-    lds_sync();
-    // w = p - (tau / 2)(p.v) v, every wave for itself
-    double pv = 0.0;
-    for (int i = j + 1 + lane; i < m; i += 64) pv += ps[i] * vs[i];
-    const double K = -0.5 * t * wave_sum(pv);
-    double wc[NB];
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-      const int ic = 32 * b + c;
-      wc[b] = (ic > j && ic < m) ? ps[ic] + K * vc[b] : 0.0;
-    }
-    // A -= v w^T + w v^T on the trailing block
-#pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      if (32 * a + 31 <= j || 32 * a >= m) continue;
-      const int ir = 32 * a + r;
-      const double va = vs[ir], wa = (ir > j && ir < m) ? ps[ir] + K * va : 0.0;
-#pragma unroll
-      for (int b = 0; b < NB; ++b)
-        if (32 * b + 31 > j && 32 * b < m) A[a][b] -= va * wc[b] + wa * vc[b];
-    }
-    if (j + 3 < m) put_column(A, m, j + 1, r, c, xs[(j + 1) & 1], dd);
-    lds_sync();
-  }
-  // the last 2 x 2 (or 1 x 1) block
-  for (int q = 0; q < 3; ++q) {
-    const int i = q == 0 ? m - 1 : (q == 1 ? m - 2 : m - 1);
-    const int l = q == 0 ? m - 1 : (q == 1 ? m - 2 : m - 2);
-    if (i < 0 || l < 0) continue;
-    if ((i & 31) == r && (l & 31) == c) {
-      const double v = tile_get(A, i >> 5, l >> 5);
-      if (q < 2) dd[i] = v;
-      else ee[m - 2] = v;
-    }
-  }
 }
 
 // 1 / q to working precision: the hardware reciprocal and two Newton steps
@@ -206,6 +93,227 @@ __device__ __forceinline__ double recip(double q) {
   double y = __builtin_amdgcn_rcp(q);
   y = fma(y, fma(-q, y, 1.0), y);
   return fma(y, fma(-q, y, 1.0), y);
+}
+
+// f(std::integral_constant<int, v>) for a run-time 0 <= v < N, so that loops
+// bounded by v unroll without branches
+template <int N, class F> __device__ __forceinline__ void with_const(int v, F &&f) {
+  if constexpr (N > 0) {
+    if (v == N - 1) f(std::integral_constant<int, N - 1>{});
+    else with_const<N - 1>(v, f);
+  }
+}
+// with_const<N>(v, f) when TRIM, else f(0): the whole range, no branches
+template <bool TRIM, int N, class F> __device__ __forceinline__ void from_block(int v, F &&f) {
+  if constexpr (TRIM) with_const<N>(v, f);
+  else f(std::integral_constant<int, 0>{});
+}
+
+// a double rotated within each row of 16 lanes (every lane receives)
+template <int CTRL> __device__ __forceinline__ double ror(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// s[a], A0 <= a < N, summed over each row of TC lanes of the thread grid,
+// stage by stage so that the rows' chains interleave; every lane of the row
+// holds the sum for TC = 16, lanes 16-31 and 48-63 for TC = 32
+template <int TC, int A0, int N> __device__ __forceinline__ void row_sums(double (&s)[N]) {
+  static_assert(TC == 16 || TC == 32, "row_sums: 16 or 32 lanes");
+  double t[N];
+#pragma unroll
+  for (int a = A0; a < N; ++a) t[a] = ror<0x128>(s[a]);
+#pragma unroll
+  for (int a = A0; a < N; ++a) s[a] += t[a];
+#pragma unroll
+  for (int a = A0; a < N; ++a) t[a] = ror<0x124>(s[a]);
+#pragma unroll
+  for (int a = A0; a < N; ++a) s[a] += t[a];
+#pragma unroll
+  for (int a = A0; a < N; ++a) t[a] = ror<0x122>(s[a]);
+#pragma unroll
+  for (int a = A0; a < N; ++a) s[a] += t[a];
+#pragma unroll
+  for (int a = A0; a < N; ++a) t[a] = ror<0x121>(s[a]);
+#pragma unroll
+  for (int a = A0; a < N; ++a) s[a] += t[a];
+  if constexpr (TC == 32) {
+#pragma unroll
+    for (int a = A0; a < N; ++a) s[a] += dpp<0x142, 0xA>(s[a]);
+  }
+}
+
+// The reduction runs on a TR x TC grid of threads, thread (r, c) holding the
+// NR x NC entries (TR a + r, TC b + c) in registers.  Two LDS-only barriers a
+// step:
+//   1. every wave forms the reflector's scalars from column j in LDS by
+//      itself and the v entries it needs on demand (v_i = x_i / (alpha -
+//      beta) below j + 1); p = tau A v is summed across the TC column lanes
+//      with DPP and its rows written to LDS;
+//   2. every wave forms w = p - (tau / 2)(p.v) v for itself, updates its
+//      part of the trailing block A -= v w^T + w v^T, and the owners of
+//      column j + 1 put it in the other LDS buffer for the next step.
+// Blocks wholly above the active part or past m are skipped (the tests are
+// uniform over the workgroup).  A grid of 16 x 16 threads (one wave per
+// SIMD) serves m <= 144; 32 x 16 (two waves per SIMD) the larger sizes: each
+// wave repeats the reflector's scalar work, so fewer waves a SIMD make a
+// step shorter while the register tile still fits.
+template <int TR, int TC, int NR, int NC, bool TRIM>
+__global__ __launch_bounds__(TR *TC) void k_ritz_tridiag(int m, const double *__restrict__ H,
+                                                         int ldh, double *__restrict__ work) {
+  static_assert(TR * TC >= kRMax, "the reflector store needs a thread per row");
+  __shared__ double xs[2][kRMax], ps[kRMax];
+  double *dd = work + kOffD, *ee = work + kOffE, *taus = work + kOffTau, *hv = work + kOffHv;
+  const int tid = threadIdx.x, r = tid / TC, c = tid % TC, lane = tid & 63;
+  const int pown = TC == 16 ? 0 : TC - 1;   // a lane of the row that holds row_sum
+  double A[NR][NC];
+#pragma unroll
+  for (int a = 0; a < NR; ++a)
+#pragma unroll
+    for (int b = 0; b < NC; ++b) {
+      const int i = TR * a + r, l = TC * b + c;
+      A[a][b] = (i < m && l < m) ? 0.5 * (H[(long)i * ldh + l] + H[(long)l * ldh + i]) : 0.0;
+    }
+  if (m > 2 && c == 0) {   // column 0 below the diagonal, and d[0]
+#pragma unroll
+    for (int a = 0; a < NR; ++a) {
+      const int i = TR * a + r;
+      if (i > 0 && i < m) xs[0][i] = A[a][0];
+      if (i == 0) dd[0] = A[a][0];
+    }
+  }
+  lds_sync();
+  for (int j = 0; j + 2 < m; ++j) {
+    const double *x = xs[j & 1];
+    // every LDS value this thread needs, loaded at once (all indices are
+    // below kRMax; entries outside (j, m) are stale and selected away)
+    double xsq[3], xc[NC];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = j + 2 + lane + 64 * q;
+      xsq[q] = x[i < m ? i : 0];
+    }
+#pragma unroll
+    for (int b = 0; b < NC; ++b) xc[b] = x[TC * b + c];
+    const double xh = x[tid < kRMax ? tid : 0];
+    const double alpha = x[j + 1];
+    // the reflector I - tau v v^T with v[j + 1] = 1, H x = beta e_{j+1}
+    double s2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+      if (j + 2 + lane + 64 * q < m) s2 += xsq[q] * xsq[q];
+    s2 = wave_sum(s2);
+    double t = 0.0, scale = 0.0, beta = alpha;
+    if (s2 > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + s2), alpha);
+      t = (beta - alpha) * recip(beta);
+      scale = recip(alpha - beta);
+    }
+    // v_i from the column's entry x_i
+    auto vof = [&](int i, double xi) {
+      const double v = i == j + 1 ? 1.0 : xi * scale;
+      return (t == 0.0 || i <= j || i >= m) ? 0.0 : v;
+    };
+    if (tid < kRMax) {
+      hv[(long)j * kRMax + tid] = vof(tid, xh);
+      if (tid == 0) {
+        ee[j] = beta;
+        taus[j] = t;
+      }
+    }
+    double vc[NC];
+#pragma unroll
+    for (int b = 0; b < NC; ++b) vc[b] = vof(TC * b + c, xc[b]);
+    // p = tau A v over the rows below j: the blocks from B0 = (j + 1) / TC
+    // (column blocks) and A0 = B0 TC / TR (row blocks) on hold row j + 1
+    // and all later ones; v = 0 on the columns before them
+    const int B0 = (j + 1) / TC;
+    from_block<TRIM, NC>(B0, [&](auto bc) {
+      constexpr int Bc = decltype(bc)::value, Ac = Bc * TC / TR;
+      double s[NR];
+#pragma unroll
+      for (int a = Ac; a < NR; ++a) {
+        s[a] = 0.0;
+#pragma unroll
+        for (int b = Bc; b < NC; ++b) s[a] = fma(A[a][b], vc[b], s[a]);
+      }
+      row_sums<TC, Ac>(s);
+      if (c == pown) {
+#pragma unroll
+        for (int a = Ac; a < NR; ++a) ps[TR * a + r] = t * s[a];
+      }
+    });
+    lds_sync();
+    double psq[3], pc[NC], pr[NR], xr[NR];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = j + 1 + lane + 64 * q;
+      psq[q] = ps[i < m ? i : 0];
+      xsq[q] = x[i < m ? i : 0];
+    }
+#pragma unroll
+    for (int b = 0; b < NC; ++b) pc[b] = ps[TC * b + c];
+#pragma unroll
+    for (int a = 0; a < NR; ++a) {
+      pr[a] = ps[TR * a + r];
+      xr[a] = x[TR * a + r];
+    }
+    // w = p - (tau / 2)(p.v) v, every wave for itself
+    double pv = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int i = j + 1 + lane + 64 * q;
+      if (i < m) pv = fma(psq[q], vof(i, xsq[q]), pv);
+    }
+    const double K = -0.5 * t * wave_sum(pv);
+    double wc[NC];
+#pragma unroll
+    for (int b = 0; b < NC; ++b) {
+      const int ic = TC * b + c;
+      wc[b] = (ic > j && ic < m) ? fma(K, vc[b], pc[b]) : 0.0;
+    }
+    // A -= v w^T + w v^T on the trailing block (v = w = 0 outside it);
+    // then the owners of column j + 1 put it below the diagonal in the
+    // other LDS buffer and d[j + 1], final from here on, in the output
+    const int jn = j + 1;
+    double *xn = xs[jn & 1];
+    from_block<TRIM, NC>(B0, [&](auto bc) {
+      constexpr int Bc = decltype(bc)::value, Ac = Bc * TC / TR;
+#pragma unroll
+      for (int a = Ac; a < NR; ++a) {
+        const int ir = TR * a + r;
+        const double va = vof(ir, xr[a]);
+        const double wa = (ir > j && ir < m) ? fma(K, va, pr[a]) : 0.0;
+#pragma unroll
+        for (int b = Bc; b < NC; ++b) A[a][b] = fma(-va, wc[b], fma(-wa, vc[b], A[a][b]));
+      }
+    });
+    if (j + 3 < m && c == jn % TC) {
+      with_const<NC>(B0, [&](auto bc) {   // column j + 1 lies in block B0
+        constexpr int Bc = decltype(bc)::value, Ac = Bc * TC / TR;
+        double dj = 0.0;
+#pragma unroll
+        for (int a = Ac; a < NR; ++a) {
+          const int i = TR * a + r;
+          xn[i] = A[a][Bc];   // entries outside (jn, m) are never read as such
+          dj = i == jn ? A[a][Bc] : dj;
+        }
+        if (r == jn % TR) dd[jn] = dj;
+      });
+    }
+    lds_sync();
+  }
+  // the last 2 x 2 (or 1 x 1) block
+  for (int q = 0; q < 3; ++q) {
+    const int i = q == 0 ? m - 1 : (q == 1 ? m - 2 : m - 1);
+    const int l = q == 0 ? m - 1 : (q == 1 ? m - 2 : m - 2);
+    if (i < 0 || l < 0) continue;
+    if (i % TR == r && l % TC == c) {
+      const double v = tile_get(A, i / TR, l / TC);
+      if (q < 2) dd[i] = v;
+      else ee[m - 2] = v;
+    }
+  }
 }
 
 // Sturm count: eigenvalues of the tridiagonal (d, e^2) below x
@@ -288,10 +396,13 @@ __global__ __launch_bounds__(64) void k_ritz_tri_eig(int m, double *__restrict__
     if (g == 0) work[kOffNorm] = tnorm;
   }
   // inverse iteration on T - lam I: LU with partial pivoting (one lane),
-  // the pivots' reciprocals (all lanes), two solves (one lane)
+  // the pivots' reciprocals (all lanes), two solves (one lane, the running
+  // entries carried in registers, the next one loaded a step ahead) each
+  // followed by a scaling to unit max-norm (all lanes)
   const double tiny = DBL_EPSILON * fmax(tnorm, DBL_MIN);
   if (lane == 0) {
     double p0 = d[0] - lam, p1 = m > 1 ? e[0] : 0.0, p2 = 0.0;
+#pragma unroll 2
     for (int k = 0; k + 1 < m; ++k) {
       const double r0 = e[k], r1 = d[k + 1] - lam, r2 = k + 2 < m ? e[k + 1] : 0.0;
       if (fabs(p0) >= fabs(r0)) {
@@ -318,36 +429,50 @@ __global__ __launch_bounds__(64) void k_ritz_tri_eig(int m, double *__restrict__
     xv[k] = 0.5 + (double)((k * 7919 + g * 104729 + 13) % 1009) / 1009.0;
   }
   __syncthreads();
-  if (lane == 0) {
-    for (int step = 0; step < 2; ++step) {
+  for (int step = 0; step < 2; ++step) {
+    if (lane == 0) {
+      // L: the row swaps and eliminations in order
+      double cur = xv[0];
       for (int k = 0; k + 1 < m; ++k) {
+        double nxt = xv[k + 1];
         if (piv[k]) {
-          const double s = xv[k];
-          xv[k] = xv[k + 1];
-          xv[k + 1] = s;
+          const double s = cur;
+          cur = nxt;
+          nxt = s;
         }
-        xv[k + 1] -= ml[k] * xv[k];
+        xv[k] = cur;
+        cur = nxt - ml[k] * cur;
       }
+      xv[m - 1] = cur;
+      // U: back substitution; a growing solution is rescaled, the entries
+      // still to be solved (and the one already loaded) with it
       double x1 = 0.0, x2 = 0.0;   // x[k + 1], x[k + 2]
+      double xk = xv[m - 1];
       for (int k = m - 1; k >= 0; --k) {
-        double s = (xv[k] - u1[k] * x1 - u2[k] * x2) * u0[k];
-        if (fabs(s) > 1e100) {   // rescale what is solved and what is left
-          for (int i = 0; i < m; ++i) xv[i] *= 1e-100;
+        double xn = k > 0 ? xv[k - 1] : 0.0;
+        double s = (xk - u1[k] * x1 - u2[k] * x2) * u0[k];
+        if (fabs(s) > 1e100) {
+          for (int i = 0; i < k; ++i) xv[i] *= 1e-100;
+          for (int i = k + 1; i < m; ++i) xv[i] *= 1e-100;
           x1 *= 1e-100;
           x2 *= 1e-100;
           s *= 1e-100;
+          xn *= 1e-100;
         }
         xv[k] = s;
         x2 = x1;
         x1 = s;
+        xk = xn;
       }
-      double mx = 0.0;
-      for (int i = 0; i < m; ++i) mx = fmax(mx, fabs(xv[i]));
-      const double inv = mx > 0.0 ? 1.0 / mx : 1.0;
-      for (int i = 0; i < m; ++i) xv[i] *= inv;
     }
+    __syncthreads();
+    double mx = 0.0;
+    for (int i = lane; i < m; i += 64) mx = fmax(mx, fabs(xv[i]));
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    const double inv = mx > 0.0 ? 1.0 / mx : 1.0;
+    for (int i = lane; i < m; i += 64) xv[i] *= inv;
+    __syncthreads();
   }
-  __syncthreads();
   double nn = 0.0;
   for (int i = lane; i < m; i += 64) nn += xv[i] * xv[i];
   const double inv = 1.0 / sqrt(wave_sum(nn));
@@ -518,13 +643,19 @@ int mbpt_ritz_eigh_f64(int m, int k, const double *H, int ldh, const double *G, 
   MBPT_REQUIRE(ldh >= m && ldg >= m && ldy >= k, "mbpt_ritz_eigh_f64: leading dimension too small");
   MBPT_REQUIRE(work_elems >= kRitzWork, "mbpt_ritz_eigh_f64: work buffer too small");
   hipStream_t st = (hipStream_t)stream;
-  switch ((m + 31) / 32) {
-    case 1: k_ritz_tridiag<1><<<1, 1024, 0, st>>>(m, H, ldh, work); break;
-    case 2: k_ritz_tridiag<2><<<1, 1024, 0, st>>>(m, H, ldh, work); break;
-    case 3: k_ritz_tridiag<3><<<1, 1024, 0, st>>>(m, H, ldh, work); break;
-    case 4: k_ritz_tridiag<4><<<1, 1024, 0, st>>>(m, H, ldh, work); break;
-    case 5: k_ritz_tridiag<5><<<1, 1024, 0, st>>>(m, H, ldh, work); break;
-    default: k_ritz_tridiag<6><<<1, 1024, 0, st>>>(m, H, ldh, work); break;
+  switch ((m + 15) / 16) {
+    case 1: k_ritz_tridiag<16, 16, 1, 1, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 2: k_ritz_tridiag<16, 16, 2, 2, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 3: k_ritz_tridiag<16, 16, 3, 3, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 4: k_ritz_tridiag<16, 16, 4, 4, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 5: k_ritz_tridiag<16, 16, 5, 5, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 6: k_ritz_tridiag<16, 16, 6, 6, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 7: k_ritz_tridiag<16, 16, 7, 7, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 8: k_ritz_tridiag<16, 16, 8, 8, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 9: k_ritz_tridiag<16, 16, 9, 9, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 10: k_ritz_tridiag<16, 16, 10, 10, true><<<1, 256, 0, st>>>(m, H, ldh, work); break;
+    case 11: k_ritz_tridiag<32, 16, 6, 11, false><<<1, 512, 0, st>>>(m, H, ldh, work); break;
+    default: k_ritz_tridiag<32, 16, 6, 12, false><<<1, 512, 0, st>>>(m, H, ldh, work); break;
   }
   int rc = launch_status("k_ritz_tridiag");
   if (rc) return rc;

@@ -388,7 +388,10 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     norms from the Ritz step (a host array).
     ritz_min: the Rayleigh matrices from this size on (up to 192, with at
     most 64 Ritz pairs wanted and no `pick`) are solved on the device
-    (ritz.hip, the lowest pairs only); smaller ones by host LAPACK.
+    (ritz.hip, the lowest pairs only); smaller ones by host LAPACK, and so
+    are all once `early` has been called: its check then keeps the device
+    busy, and the one-workgroup solver would wait for it while the host
+    solve runs beside it.
     `early(X, w)` is called once, at the first cycle whose Ritz step finds
     the leading `early_roots` pairs converged while a later tracked pair is
     not, with those Ritz vectors (n, early_roots) on the device and their
@@ -442,6 +445,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
     w = rn = None
     hist = []
     stalled = exhausted = False
+    side_busy = False
     cycles = 0
     y = None
     y_rows = m
@@ -455,7 +459,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
         k = min(nroots, m)
         kk = min(m, max(k, keep or k))
         if (pick is None and VA.is_cuda and ritz_min <= m <= ritz_max_m()
-                and kk <= 64):
+                and kk <= 64 and not side_busy):
             # large subspaces: the Ritz pairs from the device eigensolver
             Yd, wq = ritz_eigh(HG[:m, :m], HG[:m, cap:cap + m], kk)
             wq = wq.cpu().numpy()
@@ -494,6 +498,7 @@ def davidson(matmat, diag, nroots, x0, tol=DAVIDSON_TOL_RESIDUAL,
             _nn(Ranges(VA, (0, m)), ye, Xe, 0)
             early(Xe, np.array(w[:early_roots], dtype=float))
             early = None
+            side_busy = True
         worst = float(rn[~conv].max())
         hist.append(worst)
         if stall_cycles and on_stall is not None and len(hist) > stall_cycles:

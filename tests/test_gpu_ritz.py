@@ -38,7 +38,8 @@ def check(H, k, G=None, tol=1e-12):
                        atol=1e-12 * np.abs(H).max() * m)
 
 
-@pytest.mark.parametrize('m', [1, 2, 3, 17, 40, 64, 100, 161, 192])
+@pytest.mark.parametrize('m', [1, 2, 3, 16, 17, 40, 64, 65, 100, 129, 144, 145, 160, 161, 176,
+                               177, 192])
 def test_random_symmetric(m):
     rng = np.random.default_rng(m)
     A = rng.normal(size=(m, m))
