@@ -28,23 +28,26 @@ __device__ __forceinline__ f64x4 mma64(double a, double b, f64x4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// ---- C (a x b) = A^T B over rows [r0, r1), one wave per (64 x 32 tile, chunk)
+// ---- C (a x b) = A^T B over rows [r0, r1): a workgroup of four waves per
+// (64 x 32 tile, chunk of rows), each wave a quarter of the chunk, their
+// tiles summed in LDS before the chunk's partial is written.
 //
 // Lane l holds columns i = 16 f + l % 16 of A (f < 4) and j = 16 g + l % 16
 // of B (g < 2).  A step covers 4 rows: lane l reads row base + l / 16, which
 // is the k index of its operand in all eight MFMAs (f, g) of the step.
-// Loads for the next step are issued before the current step's MFMAs.
-__global__ __launch_bounds__(64) void k_tns_partial(int n, int a, int b, int chunk, Cols2 A,
-                                                   Cols2 B, double *__restrict__ P, int nta,
-                                                   int ntb, int nblk) {
+__global__ __launch_bounds__(256) void k_tns_partial(int n, int a, int b, int chunk, Cols2 A,
+                                                    Cols2 B, double *__restrict__ P, int nta,
+                                                    int ntb, int nblk) {
+  __shared__ double red[3][32 * 64];
   const int lid = xcd_logical(blockIdx.x, nblk);
   const int ntiles = nta * ntb;
   const int c = lid / ntiles, t = lid % ntiles;
   const int ta = t / ntb, tb = t % ntb;
-  const int r0 = c * chunk;
-  if (r0 >= n) return;
-  const int r1 = min(n, r0 + chunk);
-  const int lane = threadIdx.x, lr = lane & 15, lq = lane >> 4;
+  if (c * chunk >= n) return;   // the whole workgroup
+  const int wave = threadIdx.x >> 6, sub = chunk / 4;
+  const int r0 = c * chunk + wave * sub;
+  const int r1 = min(n, r0 + sub);   // r1 <= r0: this wave has no rows
+  const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
   int ca[4], cb[2];
 #pragma unroll
   for (int f = 0; f < 4; ++f) ca[f] = A.col(min(a - 1, 64 * ta + 16 * f + lr));
@@ -55,30 +58,48 @@ __global__ __launch_bounds__(64) void k_tns_partial(int n, int a, int b, int chu
   for (int f = 0; f < 4; ++f)
 #pragma unroll
     for (int g = 0; g < 2; ++g) acc[f][g] = f64x4{0.0, 0.0, 0.0, 0.0};
-  double va[4], vb[2];
-  int r = r0 + lq;
-  auto load = [&](int row) {
-    const bool in = row < r1;
-    const long ra = (long)(in ? row : r0) * A.ld, rb = (long)(in ? row : r0) * B.ld;
+  // a pipeline of PD steps: the loads of step s + PD are issued as soon as
+  // step s's MFMAs have read their operands, so PD steps stay in flight.
+  // Rows past r1 are read at r1 - 1 (always in bounds, no branch around the
+  // loads) and their B operand is zeroed where it is used.
+  constexpr int PD = 8;
+  double va[PD][4], vb[PD][2];
+  bool ok[PD];
+  auto load = [&](int row, double (&xa)[4], double (&xb)[2], bool &in) {
+    in = row < r1;
+    const long rr = in ? row : r1 - 1;
+    const long ra = rr * A.ld, rb = rr * B.ld;
 #pragma unroll
-    for (int f = 0; f < 4; ++f) va[f] = in ? A.p[ra + ca[f]] : 0.0;
+    for (int f = 0; f < 4; ++f) xa[f] = A.p[ra + ca[f]];
 #pragma unroll
-    for (int g = 0; g < 2; ++g) vb[g] = in ? B.p[rb + cb[g]] : 0.0;
+    for (int g = 0; g < 2; ++g) xb[g] = B.p[rb + cb[g]];
   };
-  load(r);
-  for (int base = r0; base < r1; base += 4) {
-    double xa[4], xb[2];
 #pragma unroll
-    for (int f = 0; f < 4; ++f) xa[f] = va[f];
+  for (int d = 0; d < PD; ++d) load(r0 + 4 * d + lq, va[d], vb[d], ok[d]);
+  for (int base = r0; base < r1; base += 4 * PD) {
 #pragma unroll
-    for (int g = 0; g < 2; ++g) xb[g] = vb[g];
-    r += 4;
-    if (base + 4 < r1) load(r);
+    for (int d = 0; d < PD; ++d) {
+      double xb[2];
+#pragma unroll
+      for (int g = 0; g < 2; ++g) xb[g] = ok[d] ? vb[d][g] : 0.0;
+#pragma unroll
+      for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int g = 0; g < 2; ++g) acc[f][g] = mma64(va[d][f], xb[g], acc[f][g]);
+      load(base + 4 * (PD + d) + lq, va[d], vb[d], ok[d]);
+    }
+  }
+  // the four waves' tiles summed in a fixed order by wave 0
+  if (wave > 0) {
 #pragma unroll
     for (int f = 0; f < 4; ++f)
 #pragma unroll
-      for (int g = 0; g < 2; ++g) acc[f][g] = mma64(xa[f], xb[g], acc[f][g]);
+      for (int g = 0; g < 2; ++g)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[wave - 1][((f * 2 + g) * 4 + q) * 64 + lane] = acc[f][g][q];
   }
+  __syncthreads();
+  if (wave > 0) return;
   // D[l / 16 + 4 q][l % 16] of each 16 x 16 tile (the f64 MFMA output layout)
   double *out = P + (long)c * a * b;
 #pragma unroll
@@ -88,7 +109,9 @@ __global__ __launch_bounds__(64) void k_tns_partial(int n, int a, int b, int chu
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int i = 64 * ta + 16 * f + lq + 4 * q, j = 32 * tb + 16 * g + lr;
-        if (i < a && j < b) out[(long)i * b + j] = acc[f][g][q];
+        const int e = ((f * 2 + g) * 4 + q) * 64 + lane;
+        const double v = ((acc[f][g][q] + red[0][e]) + red[1][e]) + red[2][e];
+        if (i < a && j < b) out[(long)i * b + j] = v;
       }
 }
 
@@ -219,9 +242,12 @@ __global__ __launch_bounds__(256) void k_block_scale(int n, int m, int p,
 
 // ---- Jacobi-Davidson correction (Solvers/davidson.py:43-77) for k columns:
 // t = K^-1 r - alpha K^-1 u, K = diag - theta floored at `floor` in magnitude,
-// alpha = u.K^-1 r / u.K^-1 u.  Pass 1: per-block partial sums of u.K^-1 u
-// and u.K^-1 r (fixed order); pass 2: every block sums the partials itself
-// and writes t.
+// alpha = u.K^-1 r / u.K^-1 u.  Lane l of a workgroup takes column
+// j = 64 blockIdx.y + l, so a wave reads 64 adjacent entries of a row; the
+// four waves take every fourth of the workgroup's kJdRows rows.  Pass 1:
+// per-workgroup partial sums of u.K^-1 u and u.K^-1 r (fixed order: rows in
+// order within a wave, then the waves, then the workgroups); pass 2: alpha
+// per column; pass 3: t.
 constexpr int kJdRows = 256;
 
 __device__ __forceinline__ double jd_den(double d, double th, double floor) {
@@ -234,62 +260,71 @@ __global__ __launch_bounds__(256) void k_jd_partial(int n, int k, const double *
                                                     int ldr, const double *__restrict__ dg,
                                                     const double *__restrict__ theta,
                                                     double floor, double *__restrict__ part) {
-  __shared__ double su[256], sr[256];
-  const int j = blockIdx.y, tid = threadIdx.x;
-  const int i = blockIdx.x * kJdRows + tid;
+  __shared__ double red[2][4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = 64 * blockIdx.y + lane;
+  const bool col = j < k;
+  const double th = col ? theta[j] : 0.0;
+  const int i0 = blockIdx.x * kJdRows + wave, i1 = min(n, (blockIdx.x + 1) * kJdRows);
   double uku = 0.0, ukr = 0.0;
-  if (i < n) {
-    const double den = jd_den(dg[i], theta[j], floor);
-    const double u = U[(long)i * ldu + j], r = R[(long)i * ldr + j];
-    uku = u * u / den;
-    ukr = u * r / den;
-  }
-  su[tid] = uku;
-  sr[tid] = ukr;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (tid < h) {
-      su[tid] += su[tid + h];
-      sr[tid] += sr[tid + h];
+  if (col) {
+#pragma unroll 8
+    for (int i = i0; i < i1; i += 4) {
+      const double den = jd_den(dg[i], th, floor);
+      const double u = U[(long)i * ldu + j], r = R[(long)i * ldr + j];
+      uku += u * u / den;
+      ukr += u * r / den;
     }
-    __syncthreads();
   }
-  if (tid == 0) {
-    part[((long)j * gridDim.x + blockIdx.x) * 2] = su[0];
-    part[((long)j * gridDim.x + blockIdx.x) * 2 + 1] = sr[0];
+  red[0][wave][lane] = uku;
+  red[1][wave][lane] = ukr;
+  __syncthreads();
+  if (wave == 0 && col) {
+    const double a = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
+    const double b = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
+    part[((long)j * gridDim.x + blockIdx.x) * 2] = a;
+    part[((long)j * gridDim.x + blockIdx.x) * 2 + 1] = b;
   }
 }
 
-__global__ __launch_bounds__(256) void k_jd_apply(int n, int k, int nblk,
-                                                  const double *__restrict__ U, int ldu,
-                                                  const double *__restrict__ R, int ldr,
+// alpha_j from the nblk partials of column j, into alpha[j]
+__global__ __launch_bounds__(256) void k_jd_alpha(int k, int nblk, const double *__restrict__ part,
+                                                  double *__restrict__ alpha) {
+  __shared__ double red[2][4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = 64 * blockIdx.x + lane;
+  double a = 0.0, b = 0.0;
+  if (j < k)
+    for (int q = wave; q < nblk; q += 4) {
+      a += part[((long)j * nblk + q) * 2];
+      b += part[((long)j * nblk + q) * 2 + 1];
+    }
+  red[0][wave][lane] = a;
+  red[1][wave][lane] = b;
+  __syncthreads();
+  if (wave == 0 && j < k) {
+    const double su = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
+    const double sr = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
+    alpha[j] = fabs(su) > 1e-300 ? sr / su : 0.0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_jd_apply(int n, int k, const double *__restrict__ U,
+                                                  int ldu, const double *__restrict__ R, int ldr,
                                                   const double *__restrict__ dg,
                                                   const double *__restrict__ theta,
                                                   double floor,
-                                                  const double *__restrict__ part,
+                                                  const double *__restrict__ alpha,
                                                   double *__restrict__ T, int ldt) {
-  __shared__ double su[256], sr[256];
-  const int j = blockIdx.y, tid = threadIdx.x;
-  double a = 0.0, b = 0.0;
-  for (int q = tid; q < nblk; q += 256) {
-    a += part[((long)j * nblk + q) * 2];
-    b += part[((long)j * nblk + q) * 2 + 1];
-  }
-  su[tid] = a;
-  sr[tid] = b;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if (tid < h) {
-      su[tid] += su[tid + h];
-      sr[tid] += sr[tid + h];
-    }
-    __syncthreads();
-  }
-  const double alpha = fabs(su[0]) > 1e-300 ? sr[0] / su[0] : 0.0;
-  const int i = blockIdx.x * kJdRows + tid;
-  if (i < n) {
-    const double den = jd_den(dg[i], theta[j], floor);
-    T[(long)i * ldt + j] = (R[(long)i * ldr + j] - alpha * U[(long)i * ldu + j]) / den;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int j = 64 * blockIdx.y + lane;
+  if (j >= k) return;
+  const double th = theta[j], al = alpha[j];
+  const int i0 = blockIdx.x * kJdRows + wave, i1 = min(n, (blockIdx.x + 1) * kJdRows);
+#pragma unroll 8
+  for (int i = i0; i < i1; i += 4) {
+    const double den = jd_den(dg[i], th, floor);
+    T[(long)i * ldt + j] = (R[(long)i * ldr + j] - al * U[(long)i * ldu + j]) / den;
   }
 }
 
@@ -299,12 +334,14 @@ using namespace mbpt;
 
 namespace {
 
+// Rows a workgroup of k_tns_partial covers: a multiple of 128 (four waves
+// of whole 32-row groups), about 2048 waves over all tiles and chunks.
 int tns_chunk(int n, int a, int b, int *nc_out) {
   const long tiles = (long)((a + 63) / 64) * ((b + 31) / 32);
-  long nc = std::max<long>(1, 4096 / tiles);
-  nc = std::min<long>(nc, (n + 63) / 64);
+  long nc = std::max<long>(1, 512 / tiles);
+  nc = std::min<long>(nc, (n + 127) / 128);
   int chunk = (int)((n + nc - 1) / nc);
-  chunk = (chunk + 3) / 4 * 4;
+  chunk = (chunk + 127) / 128 * 128;
   *nc_out = (n + chunk - 1) / chunk;
   return chunk;
 }
@@ -342,7 +379,7 @@ int mbpt_gemm_tn_seg_f64(int n, int a, int b, const double *A, int lda, int a1, 
   hipStream_t st = (hipStream_t)stream;
   const int nta = (a + 63) / 64, ntb = (b + 31) / 32;
   const int nblk = pad_grid((long)nta * ntb * nc);
-  k_tns_partial<<<nblk, 64, 0, st>>>(n, a, b, chunk, Cols2{A, lda, a1, aoff1, aoff2},
+  k_tns_partial<<<nblk, 256, 0, st>>>(n, a, b, chunk, Cols2{A, lda, a1, aoff1, aoff2},
                                      Cols2{B, ldb, b1, boff1, boff2}, work, nta, ntb, nblk);
   int rc = launch_status("k_tns_partial");
   if (rc) return rc;
@@ -400,7 +437,7 @@ int mbpt_block_scale_f64(int n, int m, int p, double drop, double *pr, double *T
 }
 
 long mbpt_jd_correction_f64_work(int n, int k) {
-  return 2L * k * ((n + kJdRows - 1) / kJdRows);
+  return 2L * k * ((n + kJdRows - 1) / kJdRows) + k;
 }
 
 int mbpt_jd_correction_f64(int n, int k, const double *U, int ldu, const double *R, int ldr,
@@ -411,14 +448,17 @@ int mbpt_jd_correction_f64(int n, int k, const double *U, int ldu, const double 
   MBPT_REQUIRE(U && R && diag && theta && T && work, "mbpt_jd_correction_f64: NULL argument");
   MBPT_REQUIRE(ldu >= k && ldr >= k && ldt >= k, "mbpt_jd_correction_f64: leading dimension too small");
   MBPT_REQUIRE(floor > 0.0, "mbpt_jd_correction_f64: floor must be positive");
-  const int nblk = (n + kJdRows - 1) / kJdRows;
-  MBPT_REQUIRE(work_elems >= 2L * k * nblk, "mbpt_jd_correction_f64: work buffer too small");
+  const int nblk = (n + kJdRows - 1) / kJdRows, ncg = (k + 63) / 64;
+  MBPT_REQUIRE(work_elems >= 2L * k * nblk + k, "mbpt_jd_correction_f64: work buffer too small");
+  double *alpha = work + 2L * k * nblk;
   hipStream_t st = (hipStream_t)stream;
-  k_jd_partial<<<dim3(nblk, k), 256, 0, st>>>(n, k, U, ldu, R, ldr, diag, theta, floor, work);
+  k_jd_partial<<<dim3(nblk, ncg), 256, 0, st>>>(n, k, U, ldu, R, ldr, diag, theta, floor, work);
   int rc = launch_status("k_jd_partial");
   if (rc) return rc;
-  k_jd_apply<<<dim3(nblk, k), 256, 0, st>>>(n, k, nblk, U, ldu, R, ldr, diag, theta, floor,
-                                            work, T, ldt);
+  k_jd_alpha<<<ncg, 256, 0, st>>>(k, nblk, work, alpha);
+  if ((rc = launch_status("k_jd_alpha"))) return rc;
+  k_jd_apply<<<dim3(nblk, ncg), 256, 0, st>>>(n, k, U, ldu, R, ldr, diag, theta, floor, alpha,
+                                              T, ldt);
   return launch_status("k_jd_apply");
 }
 

@@ -14,7 +14,7 @@ def dev(a):
     return torch.as_tensor(np.ascontiguousarray(a), device='cuda')
 
 
-@pytest.mark.parametrize('n', [1, 95, 1953, 70000])
+@pytest.mark.parametrize('n', [1, 95, 129, 1953, 4097, 70000])
 def test_tn_ranges(n):
     from mbptcode_amd.subspace import Ranges, tn_ranges
     rng = np.random.default_rng(n)
@@ -64,12 +64,11 @@ def test_nn_ranges_and_ritz(n):
     assert np.abs(R.cpu().numpy() - Rr).max() <= 1e-12 * np.abs(Rr).max()
 
 
-@pytest.mark.parametrize('n', [5, 1953, 70000])
-def test_jd_correction(n):
+@pytest.mark.parametrize('n,k', [(5, 23), (1953, 23), (70000, 23), (300, 1), (1953, 70)])
+def test_jd_correction(n, k):
     from mbptcode_amd.davidson import jacobi_davidson_correction
     from mbptcode_amd.subspace import jd_correction
     rng = np.random.default_rng(n + 2)
-    k = 23
     U, R = dev(rng.normal(size=(n, k))), dev(rng.normal(size=(n, k)))
     diag = dev(np.sort(rng.uniform(0.1, 3.0, n)))
     theta = dev(rng.uniform(0.1, 1.0, k))
