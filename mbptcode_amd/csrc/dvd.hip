@@ -142,18 +142,26 @@ __global__ __launch_bounds__(256) void k_tns_sum(int a, int b, int nc,
 // R = A2 Y - X diag(theta) where X = A Y is the first product (the Ritz
 // vectors and their residuals from one pass over V and AV).
 //
-// One wave per 16 rows and all of c (<= 64, four 16-wide tiles).  Lane l
-// reads row i0 + l % 16 at the four adjacent columns k0 + 4 (l / 16) + s and
-// uses column s in MFMA s; Y's row for that k comes from L1.
+// A workgroup of four waves takes 64 rows, a wave 16 rows and all of c
+// (<= 64, four 16-wide tiles); Y (k x c) is staged in LDS by the workgroup,
+// kNnsRows rows at a time, padded to c16 + 4 doubles so that the four k rows
+// one MFMA reads fall in different bank halves.  Lane l reads row i0 + l % 16
+// at the four adjacent columns k0 + 4 (l / 16) + s and uses column s in
+// MFMA s; the loads of two 16-wide k steps are issued together.
+constexpr int kNnsRows = 96;
 template <bool RITZ>
-__global__ __launch_bounds__(64) void k_nns(int n, int k, int c, Cols2 A, Cols2 A2,
-                                            const double *__restrict__ Y, int ldy,
-                                            const double *__restrict__ theta,
-                                            double *__restrict__ C, int ldc, int c_off,
-                                            double *__restrict__ R, int ldr, double alpha,
-                                            double beta) {
-  const int i0 = blockIdx.x * 16;
-  const int lane = threadIdx.x, lr = lane & 15, lq = lane >> 4;
+__global__ __launch_bounds__(256) void k_nns(int n, int k, int c, Cols2 A, Cols2 A2,
+                                             const double *__restrict__ Y, int ldy,
+                                             const double *__restrict__ theta,
+                                             double *__restrict__ C, int ldc, int c_off,
+                                             double *__restrict__ R, int ldr, double alpha,
+                                             double beta) {
+  extern __shared__ double ys[];
+  const int ntile = (c + 15) / 16, ldys = 16 * ntile + 4;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i0 = blockIdx.x * 64 + 16 * wave;
+  const bool active = i0 < n;   // waves past n still take part in the barriers
+  const int lr = lane & 15, lq = lane >> 4;
   const int row = min(n - 1, i0 + lr);
   const long ra = (long)row * A.ld, ra2 = (long)row * A2.ld;
   f64x4 acc[4], acc2[4];
@@ -162,29 +170,41 @@ __global__ __launch_bounds__(64) void k_nns(int n, int k, int c, Cols2 A, Cols2 
     acc[g] = f64x4{0.0, 0.0, 0.0, 0.0};
     acc2[g] = f64x4{0.0, 0.0, 0.0, 0.0};
   }
-  const int ntile = (c + 15) / 16;
-  for (int k0 = 0; k0 < k; k0 += 16) {
-    double va[4], va2[4];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const int kk = k0 + 4 * lq + s;
-      const bool in = kk < k;
-      va[s] = in ? A.p[ra + A.col(kk)] : 0.0;
-      if (RITZ) va2[s] = in ? A2.p[ra2 + A2.col(kk)] : 0.0;
+  for (int kb = 0; kb < k; kb += kNnsRows) {
+    const int kn = min(kNnsRows, k - kb), kn32 = (kn + 31) / 32 * 32;
+    __syncthreads();   // the previous rows of Y are no longer read
+    for (int e = threadIdx.x; e < kn32 * 16 * ntile; e += 256) {
+      const int kk = e / (16 * ntile), j = e % (16 * ntile);
+      ys[kk * ldys + j] = (kk < kn && j < c) ? Y[(long)(kb + kk) * ldy + j] : 0.0;
     }
+    __syncthreads();
+    if (!active) continue;
+    for (int k0 = 0; k0 < kn; k0 += 32) {
+      double va[2][4], va2[2][4];
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      if (g >= ntile) break;
-      const int j = 16 * g + lr;
+      for (int h = 0; h < 2; ++h)
 #pragma unroll
-      for (int s = 0; s < 4; ++s) {
-        const int kk = k0 + 4 * lq + s;
-        const double y = (kk < k && j < c) ? Y[(long)kk * ldy + j] : 0.0;
-        acc[g] = mma64(va[s], y, acc[g]);
-        if (RITZ) acc2[g] = mma64(va2[s], y, acc2[g]);
-      }
+        for (int s = 0; s < 4; ++s) {
+          const int kc = min(kb + k0 + 16 * h + 4 * lq + s, k - 1);   // past k: Y's row is 0
+          va[h][s] = A.p[ra + A.col(kc)];
+          if (RITZ) va2[h][s] = A2.p[ra2 + A2.col(kc)];
+        }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          if (g >= ntile) break;
+          const int j = 16 * g + lr;
+#pragma unroll
+          for (int s = 0; s < 4; ++s) {
+            const double y = ys[(k0 + 16 * h + 4 * lq + s) * ldys + j];
+            acc[g] = mma64(va[h][s], y, acc[g]);
+            if (RITZ) acc2[g] = mma64(va2[h][s], y, acc2[g]);
+          }
+        }
     }
   }
+  if (!active) return;
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     if (g >= ntile) break;
@@ -201,6 +221,12 @@ __global__ __launch_bounds__(64) void k_nns(int n, int k, int c, Cols2 A, Cols2 
       if (RITZ) R[(long)i * ldr + j] = acc2[g][q] - acc[g][q] * th;
     }
   }
+}
+
+// LDS bytes of k_nns's staged rows of Y (at most 96 x 68 doubles)
+inline size_t nns_lds(int k, int c) {
+  const int ntile = (c + 15) / 16;
+  return (size_t)std::min(kNnsRows, (k + 31) / 32 * 32) * (16 * ntile + 4) * sizeof(double);
 }
 
 // ---- Unit scaling of a projected correction block before the operator:
@@ -399,8 +425,8 @@ int mbpt_gemm_nn_seg_f64(int n, int k, int c, const double *A, int lda, int k1, 
   MBPT_REQUIRE(ldy >= c && c_off >= 0 && c_off + c <= ldc, "mbpt_gemm_nn_seg_f64: bad ldy, ldc or c_off");
   hipStream_t st = (hipStream_t)stream;
   const Cols2 a{A, lda, k1, aoff1, aoff2};
-  k_nns<false><<<(n + 15) / 16, 64, 0, st>>>(n, k, c, a, a, Y, ldy, nullptr, C, ldc, c_off,
-                                             nullptr, 0, alpha, beta);
+  k_nns<false><<<(n + 63) / 64, 256, nns_lds(k, c), st>>>(n, k, c, a, a, Y, ldy, nullptr, C, ldc,
+                                                          c_off, nullptr, 0, alpha, beta);
   return launch_status("k_nns");
 }
 
@@ -415,7 +441,7 @@ int mbpt_ritz_residual_f64(int n, int m, int k, const double *V, int ldv, int vo
                "mbpt_ritz_residual_f64: a column range runs past its row");
   MBPT_REQUIRE(ldy >= k && ldx >= k && ldr >= k, "mbpt_ritz_residual_f64: leading dimension too small");
   hipStream_t st = (hipStream_t)stream;
-  k_nns<true><<<(n + 15) / 16, 64, 0, st>>>(n, m, k, Cols2{V, ldv, m, voff, 0},
+  k_nns<true><<<(n + 63) / 64, 256, nns_lds(m, k), st>>>(n, m, k, Cols2{V, ldv, m, voff, 0},
                                             Cols2{AV, ldav, m, avoff, 0}, Y, ldy, theta, X,
                                             ldx, 0, R, ldr, 1.0, 0.0);
   return launch_status("k_nns<ritz>");

@@ -32,14 +32,14 @@ def test_tn_ranges(n):
         assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (ra, rb)
 
 
-@pytest.mark.parametrize('n', [7, 1953, 70000])
+@pytest.mark.parametrize('n', [7, 65, 1953, 70000])
 def test_nn_ranges_and_ritz(n):
     from mbptcode_amd.subspace import Ranges, nn_ranges, ritz_residual
     rng = np.random.default_rng(n + 1)
     buf = rng.normal(size=(n, 260))
     d = dev(buf)
     print(f'nn_ranges / ritz n={n}', flush=True)
-    for k, c in ((161, 30), (3, 1), (40, 64), (17, 17)):
+    for k, c in ((161, 30), (3, 1), (40, 64), (17, 17), (230, 64)):
         Y = rng.normal(size=(k, c))
         C = dev(rng.normal(size=(n, 70)))
         C0 = C.cpu().numpy()
