@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_block_scale(int n, int m, int p,
 // per-workgroup partial sums of u.K^-1 u and u.K^-1 r (fixed order: rows in
 // order within a wave, then the waves, then the workgroups); pass 2: alpha
 // per column; pass 3: t.
-constexpr int kJdRows = 256;
+constexpr int kJdRows = 128;
 
 __device__ __forceinline__ double jd_den(double d, double th, double floor) {
   const double x = d - th;
@@ -313,24 +313,31 @@ __global__ __launch_bounds__(256) void k_jd_partial(int n, int k, const double *
   }
 }
 
-// alpha_j from the nblk partials of column j, into alpha[j]
-__global__ __launch_bounds__(256) void k_jd_alpha(int k, int nblk, const double *__restrict__ part,
-                                                  double *__restrict__ alpha) {
-  __shared__ double red[2][4][64];
+// alpha_j from the nblk partials of column j, into alpha[j]: sixteen waves,
+// each summing every sixteenth partial, then the waves in order
+__global__ __launch_bounds__(1024) void k_jd_alpha(int k, int nblk,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ alpha) {
+  __shared__ double red[2][16][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int j = 64 * blockIdx.x + lane;
   double a = 0.0, b = 0.0;
-  if (j < k)
-    for (int q = wave; q < nblk; q += 4) {
+  if (j < k) {
+#pragma unroll 4
+    for (int q = wave; q < nblk; q += 16) {
       a += part[((long)j * nblk + q) * 2];
       b += part[((long)j * nblk + q) * 2 + 1];
     }
+  }
   red[0][wave][lane] = a;
   red[1][wave][lane] = b;
   __syncthreads();
   if (wave == 0 && j < k) {
-    const double su = ((red[0][0][lane] + red[0][1][lane]) + red[0][2][lane]) + red[0][3][lane];
-    const double sr = ((red[1][0][lane] + red[1][1][lane]) + red[1][2][lane]) + red[1][3][lane];
+    double su = 0.0, sr = 0.0;
+    for (int w = 0; w < 16; ++w) {
+      su += red[0][w][lane];
+      sr += red[1][w][lane];
+    }
     alpha[j] = fabs(su) > 1e-300 ? sr / su : 0.0;
   }
 }
@@ -481,7 +488,7 @@ int mbpt_jd_correction_f64(int n, int k, const double *U, int ldu, const double 
   k_jd_partial<<<dim3(nblk, ncg), 256, 0, st>>>(n, k, U, ldu, R, ldr, diag, theta, floor, work);
   int rc = launch_status("k_jd_partial");
   if (rc) return rc;
-  k_jd_alpha<<<ncg, 256, 0, st>>>(k, nblk, work, alpha);
+  k_jd_alpha<<<ncg, 1024, 0, st>>>(k, nblk, work, alpha);
   if ((rc = launch_status("k_jd_alpha"))) return rc;
   k_jd_apply<<<dim3(nblk, ncg), 256, 0, st>>>(n, k, U, ldu, R, ldr, diag, theta, floor, alpha,
                                               T, ldt);
