@@ -589,7 +589,10 @@ __global__ __launch_bounds__(1024) void k_ritz_back(int m, int k, const double *
   }
 }
 
-// y_g^T G y_g and y_g^T H y_g, one workgroup per eigenvector, a wave per row
+// y_g^T G y_g and y_g^T H y_g, one workgroup per eigenvector: lane l keeps
+// the column sums sum_i y_i G_il over its columns l = lane + 64 q for every
+// fourth row i (the rows read whole by a wave), and the products with y_l
+// are summed over the wave once, then over the four waves in order
 __global__ __launch_bounds__(256) void k_ritz_quad(int m, int k, const double *__restrict__ H,
                                                    int ldh, const double *__restrict__ G,
                                                    int ldg, const double *__restrict__ Y,
@@ -598,24 +601,41 @@ __global__ __launch_bounds__(256) void k_ritz_quad(int m, int k, const double *_
   const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < m; i += 256) y[i] = Y[(long)i * ldy + g];
   __syncthreads();
-  double sg = 0.0, sh = 0.0;
+  constexpr int NQ = (kRMax + 63) / 64;
+  double cg[NQ], ch[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) cg[q] = ch[q] = 0.0;
+#pragma unroll 4
   for (int i = wave; i < m; i += 4) {
-    double ug = 0.0, uh = 0.0;
-    for (int l = lane; l < m; l += 64) {
-      ug += G[(long)i * ldg + l] * y[l];
-      uh += H[(long)i * ldh + l] * y[l];
+    const double yi = y[i];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int l = lane + 64 * q;
+      if (l < m) {
+        cg[q] = fma(yi, G[(long)i * ldg + l], cg[q]);
+        ch[q] = fma(yi, H[(long)i * ldh + l], ch[q]);
+      }
     }
-    sg += wave_sum(ug) * y[i];
-    sh += wave_sum(uh) * y[i];
   }
+  double sg = 0.0, sh = 0.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int l = lane + 64 * q;
+    if (l < m) {
+      sg = fma(cg[q], y[l], sg);
+      sh = fma(ch[q], y[l], sh);
+    }
+  }
+  sg = wave_sum(sg);
+  sh = wave_sum(sh);
   if (lane == 0) {
     red[0][wave] = sg;
     red[1][wave] = sh;
   }
   __syncthreads();
   if (tid == 0) {
-    wq[k + g] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    wq[2 * k + g] = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    wq[k + g] = ((red[0][0] + red[0][1]) + red[0][2]) + red[0][3];
+    wq[2 * k + g] = ((red[1][0] + red[1][1]) + red[1][2]) + red[1][3];
   }
 }
 
