@@ -63,7 +63,7 @@ DAVIDSON_FOLLOW_OVERLAP_TIE = 1e-6      # :2129 overlaps this close are a tie
 #: Smallest Rayleigh matrix whose Ritz pairs the device eigensolver
 #: (ritz.hip) computes in place of host LAPACK.  Inside the Davidson the host
 #: step also pays for its products yTGy, yTHy and the upload of the vectors;
-#: on one MI355X box the headline plain FP64 solve (Rayleigh matrices of 40
+#: on one MI355X the plain FP64 solve at 100/700/6000 (Rayleigh matrices of 40
 #: to 161) took 64.3 ms with the host step throughout and 63.2 ms with the
 #: device solver from 70 or from 40.  Not a reference constant.
 DAVIDSON_DEVICE_RITZ_MIN = 64
